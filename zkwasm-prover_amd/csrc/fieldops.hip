@@ -1,0 +1,132 @@
+// Element-wise field / curve entry points used by the parity tests, plus the
+// calibration kernels whose timings give the VALU roofline (Montgomery
+// multiplies per second, single-wave latency of a multiply, a doubling and a
+// mixed add) and the HBM copy baseline used to scale rocprof byte counters.
+#include "zkcore.h"
+
+using namespace zk;
+
+namespace {
+
+template <class P>
+__global__ void k_fe_binop(Fe<P> *out, const Fe<P> *a, const Fe<P> *b, int op, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Fe<P> x = fe_load<P>(&a[i]);
+    Fe<P> y = b ? fe_load<P>(&b[i]) : fe_zero<P>();
+    Fe<P> r;
+    switch (op) {
+    case 0: r = fe_add<P>(x, y); break;
+    case 1: r = fe_mul<P>(x, y); break;
+    case 2: r = fe_sub<P>(x, y); break;
+    case 3: r = fe_inv<P>(x); break;
+    case 4: r = fe_sqr<P>(x); break;
+    case 5: r = fe_neg<P>(x); break;
+    case 6: r = fe_to_mont<P>(x); break;
+    default: r = fe_from_mont<P>(x); break;
+    }
+    fe_store<P>(&out[i], r);
+}
+
+// out[i] = a[i] + b[i] on G1, with a, b affine; op selects the formula path
+// 0: XYZZ mixed add -> Jacobian, 1: Jacobian add, 2: XYZZ dbl of a,
+// 3: Jacobian dbl of a, 4: normalise(a + b) (z = 1)
+__global__ void k_g1_op(G1Jac *out, const G1Aff *a, const G1Aff *b, int op, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    G1Aff pa = aff_load(&a[i]);
+    G1Aff pb = aff_load(&b[i]);
+    G1Xyzz xa = xyzz_from_aff(pa, false);
+    G1Jac r;
+    if (op == 0) {
+        G1Xyzz s = aff_is_id(pb) ? xa : xyzz_add_aff(xa, pb.x, pb.y);
+        r = xyzz_to_jac(s);
+    } else if (op == 1) {
+        G1Jac ja = xyzz_to_jac(xa), jb = xyzz_to_jac(xyzz_from_aff(pb, false));
+        r = jac_add(ja, jb);
+    } else if (op == 2) {
+        r = xyzz_to_jac(xyzz_dbl(xa));
+    } else if (op == 3) {
+        r = jac_dbl(xyzz_to_jac(xa));
+    } else {
+        G1Xyzz s = xyzz_add(xa, xyzz_from_aff(pb, false));
+        r = jac_normalize(xyzz_to_jac(s));
+    }
+    jac_store(&out[i], r);
+}
+
+// throughput: each thread runs `iters` dependent multiplies of its own element
+__global__ void k_bench_mul(Fr *buf, uint32_t iters) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Fr x = fe_load<FrParams>(&buf[i]);
+    Fr y = fe_load<FrParams>(&buf[i ^ 1]);
+    for (uint32_t k = 0; k < iters; k++) {
+        x = fe_mul<FrParams>(x, y);
+        y = fe_mul<FrParams>(y, x);
+    }
+    fe_store<FrParams>(&buf[i], x);
+}
+__global__ void k_bench_dbl(G1Jac *buf, uint32_t iters) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    G1Jac p = jac_load(&buf[i]);
+    for (uint32_t k = 0; k < iters; k++) p = jac_dbl(p);
+    jac_store(&buf[i], p);
+}
+__global__ void k_bench_madd(G1Xyzz *acc, const G1Aff *pts, uint32_t npts, uint32_t iters) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    G1Xyzz a = xyzz_load(&acc[i]);
+    for (uint32_t k = 0; k < iters; k++) {
+        G1Aff q = aff_load(&pts[(i * 7 + k * 131) % npts]);
+        a = xyzz_add_aff(a, q.x, q.y);
+    }
+    xyzz_store(&acc[i], a);
+}
+__global__ void k_copy(uint4 *dst, const uint4 *src, size_t n16) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+hipError_t zk_fr_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
+    if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    k_fe_binop<FrParams><<<(n + 255) / 256, 256, 0, stream>>>((Fr *)out, (const Fr *)a, (const Fr *)b, op, n);
+    return hipGetLastError();
+}
+hipError_t zk_fp_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
+    if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    k_fe_binop<FpParams><<<(n + 255) / 256, 256, 0, stream>>>((Fp *)out, (const Fp *)a, (const Fp *)b, op, n);
+    return hipGetLastError();
+}
+hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
+    if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    k_g1_op<<<(n + 63) / 64, 64, 0, stream>>>((G1Jac *)out, (const G1Aff *)a, (const G1Aff *)b, op, n);
+    return hipGetLastError();
+}
+hipError_t zk_bench_mul(void *buf, int threads, int block, int iters, hipStream_t stream) {
+    if (threads % block) return hipErrorInvalidValue;
+    k_bench_mul<<<threads / block, block, 0, stream>>>((Fr *)buf, iters);
+    return hipGetLastError();
+}
+hipError_t zk_bench_dbl(void *buf, int threads, int block, int iters, hipStream_t stream) {
+    if (threads % block) return hipErrorInvalidValue;
+    k_bench_dbl<<<threads / block, block, 0, stream>>>((G1Jac *)buf, iters);
+    return hipGetLastError();
+}
+hipError_t zk_bench_madd(void *acc, const void *pts, int npts, int threads, int block, int iters,
+                         hipStream_t stream) {
+    if (threads % block || npts <= 0) return hipErrorInvalidValue;
+    k_bench_madd<<<threads / block, block, 0, stream>>>((G1Xyzz *)acc, (const G1Aff *)pts, npts, iters);
+    return hipGetLastError();
+}
+hipError_t zk_copy(void *dst, const void *src, size_t bytes, hipStream_t stream) {
+    if (bytes % 16) return hipErrorInvalidValue;
+    size_t n16 = bytes / 16;
+    k_copy<<<4096, 256, 0, stream>>>((uint4 *)dst, (const uint4 *)src, n16);
+    return hipGetLastError();
+}
+
+}  // extern "C"
