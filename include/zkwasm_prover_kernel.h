@@ -1,0 +1,210 @@
+/*
+ * zkwasm_prover_kernel.h -- C ABI of the MI355X (gfx950) arithmetic core.
+ *
+ * Drop-in for the `extern "C"` block the reference prover links as the static
+ * library `zkwasm_prover_kernel` (src/cuda/bn254_c.rs:4-309).  Every symbol of
+ * that block that is on the KZG hot path is exported here under the same name
+ * with the same argument order and meaning; `CUstream_st *` becomes
+ * hipStream_t and `cudaError` becomes hipError_t (0 = success).  Each
+ * declaration names the reference declaration it replaces.
+ *
+ * Conventions (as in the reference):
+ *   - every buffer is a device pointer; Fr/Fp elements are 32 B, 4 x u64
+ *     little-endian in Montgomery form (R = 2^256); G1 affine 64 B {x, y}
+ *     (identity (0, 0)), G1 Jacobian 96 B {x, y, z} (identity z = 0);
+ *   - scalars such as beta, gamma, y, n^-1 are passed by device pointer;
+ *   - "arrays of buffers" are device arrays of device pointers;
+ *   - sizes are element counts; n is a power of two unless stated otherwise;
+ *   - calls are asynchronous on `stream` (0 = the null stream) and never
+ *     allocate device memory; entries the reference runs on the default
+ *     stream (no stream argument) run on the null stream here too.
+ * Invalid arguments return hipErrorInvalidValue instead of asserting.
+ */
+#ifndef ZKWASM_PROVER_KERNEL_H
+#define ZKWASM_PROVER_KERNEL_H
+
+#include <hip/hip_runtime.h>
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- NTT (src/cuda/bn254_c.rs:6-16, :196, :31-39) --------------------- */
+
+/* bn254_c.rs:6 `ntt`.  Natural-order DFT of buf (2^log_n Fr) with the
+ * w^i table from expand_omega_buffer; n_inv != NULL scales by *n_inv (iNTT).
+ * The result is in tmp when *swap is set on return, else in buf.  pq and
+ * max_deg are accepted and unused (the in-pass twiddles come from omegas). */
+hipError_t ntt(void *buf, void *tmp, const void *pq, const void *omegas, const void *n_inv, int log_n,
+               int max_deg, bool *swap, hipStream_t stream);
+
+/* bn254_c.rs:196 `expand_omega_buffer(buf, n, may_bit_reversed)`; the
+ * second argument is log2 of the length.  buf[0] = 1, buf[1] = w on entry;
+ * fills buf[i] = w^i for i < 2^log_n (natural order for every size). */
+hipError_t expand_omega_buffer(void *buf, int log_n, int may_bit_reversed);
+
+/* bn254_c.rs:31 `extended_prepare`.  to_coset == 0: zero [size,
+ * extended_size) then s[i] *= coset_powers[(i % n) - 1] for i < size with
+ * i % n != 0; to_coset != 0: the same scaling over [0, extended_size). */
+hipError_t extended_prepare(void *s, const void *coset_powers, unsigned coset_powers_n, int size,
+                            int extended_size, int to_coset, hipStream_t stream);
+
+/* ---- MSM (src/cuda/bn254_c.rs:248-280) ---------------------------------- */
+
+/* bn254_c.rs:261 `msm`.  Buffers per src/cuda/msm.rs:297-311: res = this
+ * MSM's curve slice ((windows << window_bits) + 2 * workers Jacobian points),
+ * four index buffers of array_len * windows u32, sort_buf of 3 * array_len *
+ * windows u32.  They are used as scratch; the window is chosen internally.
+ * Call batch_msm_collect afterwards for the result. */
+hipError_t msm(void *res, const void *points, const void *scalars, void *indices_buf0, void *indices_buf1,
+               void *indices_buf2, void *indices_buf3, void *sort_buf, void *acc_indices_buf, int array_len,
+               int windows, int window_bits, int threads, int workers,
+               int prepared_sort_indices_temp_storage_bytes, int skip_zero, hipStream_t stream);
+
+/* bn254_c.rs:248 `batch_msm_collect`.  buckets: device array of batch_size
+ * pointers to the curve slices given to msm(); afterwards each slice starts
+ * with the MSM result, a Jacobian point with z = 1 (or the identity
+ * (0, 1, 0)).  The remain arrays are unused. */
+hipError_t batch_msm_collect(void *remain_indices, void *remain_acc, void *next_remain_indices,
+                             void *next_remain_acc, void *buckets, unsigned workers, unsigned windows,
+                             unsigned window_bits, unsigned batch_size, hipStream_t stream);
+
+/* ---- batch inversion / logup grand sum (bn254_c.rs:149, :226, :238) ---- */
+
+/* bn254_c.rs:149 `logup_sum_input_inv`: sum[i] = (init == 0) ? 1/(input[i]+beta)
+ * : sum[i] + 1/(input[i]+beta).  temp: n Fr of scratch. */
+hipError_t logup_sum_input_inv(void *sum, const void *input, void *temp, const void *beta, int init, int n,
+                               hipStream_t stream);
+
+/* bn254_c.rs:226 `eval_logup_z_compose`: z[i] = input[i] - m[i]/(table[i]+beta)
+ * then the exclusive scan z[i] = last_z[0] + sum_{j<i} z[j]; finally
+ * last_z[0] = z[last_z_index].  input and table are scratch afterwards. */
+hipError_t eval_logup_z_compose(void *z, void *input, void *table, const void *multiplicity, const void *beta,
+                                void *last_z, int last_z_index, int n, hipStream_t stream);
+
+/* bn254_c.rs:238 `eval_logup_z_pure`: in-place exclusive scan of z with carry
+ * last_z[0]; last_z[0] = z[last_z_index] afterwards.  input and table are
+ * scratch (n Fr each). */
+hipError_t eval_logup_z_pure(void *z, void *input, void *table, void *last_z, int last_z_index, int n,
+                             hipStream_t stream);
+
+/* ---- eval_h evaluator and argument kernels ---------------------------- */
+
+/* bn254_c.rs:18 `field_op`: res[i] = L op R with L = l[(i+l_rot) mod n]
+ * (* l_c) or l_c when l is NULL, R likewise; op 0 add, 1 mul, 2 copy L,
+ * 3 sub.  Negative rotations wrap (mod n). */
+hipError_t field_op(void *res, const void *l, int l_rot, const void *l_c, const void *r, int r_rot,
+                    const void *r_c, int size, int op, hipStream_t stream);
+
+/* bn254_c.rs:81 `field_op_batch_mul_sum`: v = [c0, a00, a01, NULL, c1, a10, NULL,
+ * ...] (v_n entries), rot = one i32 per column pointer;
+ * res[i] += sum_t c_t * prod_j v_tj[(i + rot_tj) & (n-1)]. */
+hipError_t field_op_batch_mul_sum(void *res, const void *v, const void *rot, int v_n, int n,
+                                  hipStream_t stream);
+
+/* bn254_c.rs:198 `field_mul_zip`: buf[i] *= coeff[i % coeff_n]. */
+hipError_t field_mul_zip(void *buf, const void *coeff, int coeff_n, int n);
+
+/* bn254_c.rs:41 `permutation_eval_h_p1`: res = (res*y + l0*(1 - first))*y
+ * + l_last*(last^2 - last). */
+hipError_t permutation_eval_h_p1(void *res, const void *first_set, const void *last_set, const void *l0,
+                                 const void *l_last, const void *y, int n);
+
+/* bn254_c.rs:51 `permutation_eval_h_p2`: for j in 1..n_set:
+ * res = res*y + l0*(set[j][i] - set[j-1][(i+rot) & (n-1)]). */
+hipError_t permutation_eval_h_p2(void *res, const void *set, const void *l0, const void *l_last, const void *y,
+                                 int n_set, int rot, int n);
+
+/* bn254_c.rs:62 `permutation_eval_h_l`: res[i] += beta*p[i] (+ gamma at i=0). */
+hipError_t permutation_eval_h_l(void *res, const void *beta, const void *gamma, const void *p, int n,
+                                hipStream_t stream);
+
+/* bn254_c.rs:108 `logup_eval_h` (cuda/bn254.cu:867-918 formula). */
+hipError_t logup_eval_h(void *res, const void *input_product, const void *input_product_sum, const void *table,
+                        const void *multiplicity, const void *z_first, const void *z_last, const void *l0,
+                        const void *l_last, const void *l_active_row, const void *y, int rot, int n,
+                        hipStream_t stream);
+
+/* bn254_c.rs:125 `logup_eval_h_extra_inputs` (cuda/bn254.cu:920-949). */
+hipError_t logup_eval_h_extra_inputs(void *res, const void *input_product, const void *input_product_sum,
+                                     const void *z, const void *l_active_row, const void *y, int rot, int n,
+                                     hipStream_t stream);
+
+/* bn254_c.rs:137 `logup_eval_h_z_set`: permutation_eval_h_p2 on `stream`. */
+hipError_t logup_eval_h_z_set(void *res, const void *set, const void *l0, const void *l_last, const void *y,
+                              int n_set, int rot, int n, hipStream_t stream);
+
+/* bn254_c.rs:159 `logup_eval_h_inputs_product_sum`: product = prod_j set[j],
+ * product_sum = sum_j prod_{k != j} set[k]. */
+hipError_t logup_eval_h_inputs_product_sum(void *product, void *product_sum, const void *sets, int n_set, int n,
+                                           hipStream_t stream);
+
+/* bn254_c.rs:168 `shuffle_eval_h` (cuda/bn254.cu:975-1016). */
+hipError_t shuffle_eval_h(void *res, const void *input, const void *table, const void *z, const void *l0,
+                          const void *l_last, const void *l_active_row, const void *y, int rot, int n);
+
+/* bn254_c.rs:181 `shuffle_eval_h_v2` (cuda/bn254.cu:1018-1071). */
+hipError_t shuffle_eval_h_v2(void *res, const void *inputs, const void *tables, const void *betas, int group_len,
+                             const void *z, const void *l0, const void *l_last, const void *l_active_row,
+                             const void *y, int rot, int n);
+
+/* ---- extensions: no counterpart in the reference's C ABI --------------- */
+
+/* Batch inversion in place, zeros stay zero (halo2 BatchInvert, used at
+ * src/permutation.rs:73-75 and src/shuffle.rs:163-165).  temp: n Fr. */
+hipError_t zk_batch_invert(void *a, void *temp, int n, hipStream_t stream);
+
+/* One permutation set's grand product on the device, replacing the CPU
+ * src/permutation.rs:44-128 for one chunk of columns: values/sigmas are
+ * device arrays of ncols column pointers; delta_start = DELTA^(set*chunk_len).
+ * z[i] = exclusive running product of
+ *   prod_j (v_j + beta*delta_start*DELTA^j*omega^i + gamma) / prod_j (v_j + beta*sigma_j + gamma).
+ * temp: 2n Fr of scratch. */
+hipError_t zk_permutation_z_set(void *z, const void *values, const void *sigmas, int ncols, const void *beta,
+                                const void *gamma, const void *delta_start, const void *delta,
+                                const void *omega, void *temp, int n, hipStream_t stream);
+
+/* Chains the sets (src/permutation.rs:96-147): zs is a device array of nsets
+ * pointers; z_s[0..=u] *= prod_{s'<s} z_{s'}[u] (tails taken before
+ * chaining).  tails: nsets Fr of scratch. */
+hipError_t zk_permutation_chain(void *zs, int nsets, int u, int n, void *tails, hipStream_t stream);
+
+/* Shuffle grand product (src/shuffle.rs:142-209): inputs/shuffles are device
+ * arrays of group_len pointers, betas[j] = beta^(j+1); z = exclusive running
+ * product of prod_j (input_j + betas[j]) / prod_j (shuffle_j + betas[j]).
+ * temp: 2n Fr. */
+hipError_t zk_shuffle_z(void *z, const void *inputs, const void *shuffles, int group_len, const void *betas,
+                        void *temp, int n, hipStream_t stream);
+
+/* MSM with caller-owned scratch of zk_msm_scratch_bytes(n, c) bytes; the
+ * result (Jacobian, z = 1) lands in out[0..96); out must hold 96 * 130 bytes.
+ * c = 0 picks the default window. */
+size_t zk_msm_scratch_bytes(int n, int c);
+hipError_t zk_msm(void *out, const void *points, const void *scalars, int n, int c, void *scratch,
+                  size_t scratch_bytes, hipStream_t stream);
+
+/* Element-wise field / curve operations and calibration kernels (tests and
+ * bench only). */
+hipError_t zk_fr_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
+hipError_t zk_fp_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
+hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
+hipError_t zk_bench_mul(void *buf, int threads, int block, int iters, hipStream_t stream);
+hipError_t zk_bench_dbl(void *buf, int threads, int block, int iters, hipStream_t stream);
+hipError_t zk_bench_madd(void *acc, const void *pts, int npts, int threads, int block, int iters,
+                         hipStream_t stream);
+hipError_t zk_copy(void *dst, const void *src, size_t bytes, hipStream_t stream);
+
+/* ---- host-side mirror of the reference's MSM driver -------------------- */
+/* src/cuda/msm.rs:274-283 window policy */
+void zkp_msm_policy(int count, int len, int *window_bits, int *windows, int *workers);
+/* src/cuda/msm.rs:239-455 batch_msm over device-resident scalar sets;
+ * out_affine: count affine points in host memory. */
+hipError_t zkp_batch_msm(void *out_affine, const void *points_dev, const void *const *scalars_dev, int count,
+                         int len, int skip_zero);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

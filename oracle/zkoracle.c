@@ -362,6 +362,18 @@ void zo_gen_bases_ap(zo_g1a *out, size_t n, uint64_t seed) {
 }
 
 /* ---- MSM ----------------------------------------------------------------- */
+/* out = sum_i s[i] * c[i mod m]: the closed-form expectation of the
+ * reference's test_msm (src/cuda/msm.rs:477-511). */
+void zo_fr_inner_cycled(zo_fr *out, const zo_fr *s, size_t n, const zo_fr *c, size_t m) {
+    zo_fr *part = (zo_fr *)calloc(m, sizeof(zo_fr));
+    for (size_t i = 0; i < n; i++) fr_add(&part[i % m], &part[i % m], &s[i]);
+    zo_fr acc = {{0, 0, 0, 0}};
+    for (size_t j = 0; j < m; j++) {
+        zo_fr t; fr_mul(&t, &part[j], &c[j]); fr_add(&acc, &acc, &t);
+    }
+    *out = acc;
+    free(part);
+}
 void zo_msm_naive(zo_g1j *r, const zo_g1a *p, const zo_fr *s, size_t n) {
     zo_g1j acc; g1j_set_id(&acc);
     for (size_t i = 0; i < n; i++) {

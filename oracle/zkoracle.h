@@ -79,6 +79,7 @@ int zo_g1a_is_on_curve(const zo_g1a *p);
 void zo_g1_batch_to_affine(zo_g1a *out, const zo_g1j *in, size_t n);
 
 /* ---- MSM (cuda/msm.cu:9-467, src/cuda/msm.rs:239-455) ------------------ */
+void zo_fr_inner_cycled(zo_fr *out, const zo_fr *s, size_t n, const zo_fr *c, size_t m);
 void zo_msm_naive(zo_g1j *r, const zo_g1a *p, const zo_fr *s, size_t n);
 /* Pippenger with unsigned c-bit windows over canonical scalars; c = 0 picks
  * a window from n.  Threads split the windows. */

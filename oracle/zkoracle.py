@@ -244,6 +244,13 @@ def g1a_to_ints(p_aff):
 
 
 # ---- MSM -------------------------------------------------------------------
+def fr_inner_cycled(s, c):
+    out = fr_zeros(1)
+    lib().zo_fr_inner_cycled(_p(out), _p(np.ascontiguousarray(s)), ctypes.c_size_t(len(s)),
+                             _p(np.ascontiguousarray(c)), ctypes.c_size_t(len(c)))
+    return out
+
+
 def msm_naive(points, scalars):
     out = np.zeros((1, 12), dtype=np.uint64)
     lib().zo_msm_naive(_p(out), _p(points), _p(scalars), ctypes.c_size_t(len(scalars)))

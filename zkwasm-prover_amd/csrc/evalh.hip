@@ -1,0 +1,383 @@
+// eval_h expression evaluator and constraint kernels for gfx950.
+//
+// Drop-in for field_op, field_op_batch_mul_sum, field_mul_zip, the
+// permutation / logup / shuffle eval_h kernels and
+// logup_eval_h_inputs_product_sum (cuda/bn254.cu:542-1213 and the wrappers at
+// :1215-1635; callers src/eval_h.rs, src/expr.rs, src/cuda/bn254.rs).
+//
+// Every kernel is element-wise over the (extended) domain: grid-stride loops
+// with 256-thread workgroups, each row reading its operands once (rotations
+// are constant index shifts, so loads stay coalesced).  The per-row formulas
+// and the order of the `t = t*y + ...` folds are exactly the reference's,
+// which fixes h(X) bit for bit.  The term list of field_op_batch_mul_sum is
+// staged once per workgroup in LDS instead of being re-read per row.
+#include "zkcore.h"
+
+using namespace zk;
+
+namespace {
+
+constexpr uint32_t kThreads = 256;
+constexpr int kMaxLdsTerms = 2048;
+
+uint32_t grid_for(uint32_t n) {
+    uint32_t b = (n + kThreads - 1) / kThreads;
+    return b > 32768 ? 32768 : (b ? b : 1);
+}
+
+__device__ __forceinline__ Fr ld(const Fr *p) { return fe_load<FrParams>(p); }
+__device__ __forceinline__ void st(Fr *p, const Fr &v) { fe_store<FrParams>(p, v); }
+
+#define GRID_LOOP(i, n)                                                \
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
+         i += gridDim.x * blockDim.x)
+
+__device__ __forceinline__ uint32_t rot_mod(uint32_t i, int rot, uint32_t n) {
+    int64_t v = ((int64_t)i + rot) % (int64_t)n;
+    return (uint32_t)(v < 0 ? v + n : v);
+}
+
+// cuda/bn254.cu:585-639
+__global__ void k_field_op(Fr *res, const Fr *l, int l_rot, const Fr *l_c, const Fr *r, int r_rot, const Fr *r_c,
+                           uint32_t n, int op) {
+    Fr lc = l_c ? ld(l_c) : fe_zero<FrParams>();
+    Fr rc = r_c ? ld(r_c) : fe_zero<FrParams>();
+    GRID_LOOP(i, n) {
+        Fr fl, fr = fe_zero<FrParams>();
+        if (l) {
+            fl = ld(&l[rot_mod(i, l_rot, n)]);
+            if (l_c) fl = fl * lc;
+        } else {
+            fl = lc;
+        }
+        if (r) {
+            fr = ld(&r[rot_mod(i, r_rot, n)]);
+            if (r_c) fr = fr * rc;
+        } else if (r_c) {
+            fr = rc;
+        }
+        Fr o;
+        if (op == 0) o = fl + fr;
+        else if (op == 1) o = fl * fr;
+        else if (op == 2) o = fl;
+        else o = fl - fr;
+        // res may alias l or r with a rotation: every row reads its operands
+        // before the (same-row) write, as in the reference
+        st(&res[i], o);
+    }
+}
+
+// cuda/bn254.cu:542-572: res[i] += sum_t c_t * prod_j v_tj[(i + rot_tj) & (n-1)]
+// v = [c0, a00, a01, NULL, c1, a10, NULL, ...]; rot has one entry per column
+// pointer (the non-NULL entries that are not coefficients).
+__global__ __launch_bounds__(kThreads) void k_batch_mul_sum(Fr *res, const Fr *const *__restrict__ v,
+                                                            const int *__restrict__ rot, int v_n, uint32_t n) {
+    __shared__ const Fr *sv[kMaxLdsTerms];
+    __shared__ int sr[kMaxLdsTerms];
+    __shared__ int n_cols;
+    const bool in_lds = v_n <= kMaxLdsTerms;
+    if (in_lds) {
+        for (int k = threadIdx.x; k < v_n; k += blockDim.x) sv[k] = v[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int vi = 0, cols = 0;
+            while (vi < v_n) {
+                vi++;  // coefficient
+                while (vi < v_n && sv[vi]) { vi++; cols++; }
+                vi++;  // separator
+            }
+            n_cols = cols;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < n_cols; k += blockDim.x) sr[k] = rot[k];
+    }
+    __syncthreads();
+    const Fr *const *V = in_lds ? (const Fr *const *)sv : v;
+    const int *Rt = in_lds ? (const int *)sr : rot;
+    const uint32_t mask = n - 1;
+    GRID_LOOP(i, n) {
+        Fr acc = fe_zero<FrParams>();
+        int vi = 0, ri = 0;
+        while (vi < v_n) {
+            Fr term = ld(V[vi++]);  // coefficient
+            while (vi < v_n && V[vi]) {
+                term = term * ld(&V[vi][(i + (uint32_t)Rt[ri]) & mask]);
+                vi++;
+                ri++;
+            }
+            acc = acc + term;
+            vi++;  // NULL separator
+        }
+        st(&res[i], ld(&res[i]) + acc);
+    }
+}
+
+// cuda/bn254.cu:1121-1131
+__global__ void k_field_mul_zip(Fr *buf, const Fr *__restrict__ coeff, uint32_t coeff_n, uint32_t n) {
+    GRID_LOOP(i, n) st(&buf[i], ld(&buf[i]) * ld(&coeff[i % coeff_n]));
+}
+
+// cuda/bn254.cu:664-702
+__global__ void k_perm_p1(Fr *res, const Fr *__restrict__ first, const Fr *__restrict__ last,
+                          const Fr *__restrict__ l0, const Fr *__restrict__ l_last, const Fr *__restrict__ yp,
+                          uint32_t n) {
+    Fr y = ld(yp), one = fe_one<FrParams>();
+    GRID_LOOP(i, n) {
+        Fr t1 = ld(&res[i]) * y;
+        t1 = t1 + (one - ld(&first[i])) * ld(&l0[i]);
+        t1 = t1 * y;
+        Fr ls = ld(&last[i]);
+        t1 = t1 + (fe_sqr<FrParams>(ls) - ls) * ld(&l_last[i]);
+        st(&res[i], t1);
+    }
+}
+
+// cuda/bn254.cu:704-739 (also logup_eval_h_z_set)
+__global__ void k_perm_p2(Fr *res, const Fr *const *__restrict__ set, const Fr *__restrict__ l0,
+                          const Fr *__restrict__ yp, int n_set, int rot, uint32_t n) {
+    Fr y = ld(yp);
+    GRID_LOOP(i, n) {
+        uint32_t rp = (i + n + (uint32_t)rot) & (n - 1);
+        Fr t1 = ld(&res[i]);
+        Fr l = ld(&l0[i]);
+        for (int j = 1; j < n_set; j++) {
+            t1 = t1 * y;
+            t1 = t1 + (ld(&set[j][i]) - ld(&set[j - 1][rp])) * l;
+        }
+        st(&res[i], t1);
+    }
+}
+
+// cuda/bn254.cu:741-765
+__global__ void k_perm_l(Fr *res, const Fr *__restrict__ beta, const Fr *__restrict__ gamma, const Fr *__restrict__ p,
+                         uint32_t n) {
+    Fr b = ld(beta), g = ld(gamma);
+    GRID_LOOP(i, n) {
+        Fr t = ld(&p[i]) * b;
+        if (i == 0) t = t + g;
+        st(&res[i], ld(&res[i]) + t);
+    }
+}
+
+// cuda/bn254.cu:867-918
+__global__ void k_logup_eval_h(Fr *res, const Fr *__restrict__ ip, const Fr *__restrict__ ips,
+                               const Fr *__restrict__ table, const Fr *__restrict__ m, const Fr *__restrict__ zf,
+                               const Fr *__restrict__ zl, const Fr *__restrict__ l0, const Fr *__restrict__ l_last,
+                               const Fr *__restrict__ l_active, const Fr *__restrict__ yp, int rot, uint32_t n) {
+    Fr y = ld(yp);
+    GRID_LOOP(i, n) {
+        uint32_t rn = (i + (uint32_t)rot) & (n - 1);
+        Fr t = ld(&res[i]) * y;
+        Fr z = ld(&zf[i]);
+        t = t + ld(&l0[i]) * z;
+        t = t * y;
+        t = t + ld(&l_last[i]) * ld(&zl[i]);
+        t = t * y;
+        Fr tb = ld(&table[i]);
+        Fr u = (ld(&zf[rn]) - z) * tb;
+        u = u + ld(&m[i]);
+        u = u * ld(&ip[i]);
+        u = u - tb * ld(&ips[i]);
+        u = u * ld(&l_active[i]);
+        st(&res[i], t + u);
+    }
+}
+
+// cuda/bn254.cu:920-949
+__global__ void k_logup_extra(Fr *res, const Fr *__restrict__ ip, const Fr *__restrict__ ips, const Fr *__restrict__ zp,
+                              const Fr *__restrict__ l_active, const Fr *__restrict__ yp, int rot, uint32_t n) {
+    Fr y = ld(yp);
+    GRID_LOOP(i, n) {
+        uint32_t rn = (i + (uint32_t)rot) & (n - 1);
+        Fr t = ld(&res[i]) * y;
+        Fr u = (ld(&zp[rn]) - ld(&zp[i])) * ld(&ip[i]);
+        u = u - ld(&ips[i]);
+        u = u * ld(&l_active[i]);
+        st(&res[i], t + u);
+    }
+}
+
+// cuda/bn254.cu:1172-1213 (the product-sum rebuilt per j like the reference,
+// which costs n_set^2 multiplies; n_set is small)
+__global__ void k_inputs_product_sum(Fr *__restrict__ prod, Fr *__restrict__ psum, const Fr *const *__restrict__ set,
+                                     int n_set, uint32_t n) {
+    GRID_LOOP(i, n) {
+        Fr p = fe_one<FrParams>();
+        for (int j = 0; j < n_set; j++) p = p * ld(&set[j][i]);
+        st(&prod[i], p);
+        Fr sum = fe_zero<FrParams>();
+        for (int j = 0; j < n_set; j++) {
+            Fr u = fe_one<FrParams>();
+            for (int k = 0; k < n_set; k++)
+                if (k != j) u = u * ld(&set[k][i]);
+            sum = sum + u;
+        }
+        st(&psum[i], sum);
+    }
+}
+
+// shared tail of the two shuffle kernels (cuda/bn254.cu:990-1013, :1049-1068)
+__device__ __forceinline__ Fr shuffle_fold(const Fr &res, const Fr &lhs, const Fr &rhs, const Fr &z, const Fr &zn,
+                                           const Fr &l0, const Fr &l_last, const Fr &l_active, const Fr &y) {
+    Fr one = fe_one<FrParams>();
+    Fr t = res * y;
+    t = t + l0 * (one - z);
+    t = t * y;
+    t = t + l_last * (z * z - z);
+    t = t * y;
+    Fr u = rhs * zn - lhs * z;
+    return t + u * l_active;
+}
+
+__global__ void k_shuffle(Fr *res, const Fr *__restrict__ input, const Fr *__restrict__ table, const Fr *__restrict__ zp,
+                          const Fr *__restrict__ l0, const Fr *__restrict__ l_last, const Fr *__restrict__ l_active,
+                          const Fr *__restrict__ yp, int rot, uint32_t n) {
+    Fr y = ld(yp);
+    GRID_LOOP(i, n) {
+        uint32_t rn = (i + (uint32_t)rot) & (n - 1);
+        st(&res[i], shuffle_fold(ld(&res[i]), ld(&input[i]), ld(&table[i]), ld(&zp[i]), ld(&zp[rn]), ld(&l0[i]),
+                                 ld(&l_last[i]), ld(&l_active[i]), y));
+    }
+}
+
+__global__ void k_shuffle_v2(Fr *res, const Fr *const *__restrict__ inputs, const Fr *const *__restrict__ tables,
+                             const Fr *__restrict__ betas, int group_len, const Fr *__restrict__ zp,
+                             const Fr *__restrict__ l0, const Fr *__restrict__ l_last, const Fr *__restrict__ l_active,
+                             const Fr *__restrict__ yp, int rot, uint32_t n) {
+    Fr y = ld(yp);
+    GRID_LOOP(i, n) {
+        uint32_t rn = (i + (uint32_t)rot) & (n - 1);
+        Fr b0 = ld(&betas[0]);
+        Fr l = ld(&inputs[0][i]) + b0;
+        Fr r = ld(&tables[0][i]) + b0;
+        for (int j = 1; j < group_len; j++) {
+            Fr bj = ld(&betas[j]);
+            l = l * (ld(&inputs[j][i]) + bj);
+            r = r * (ld(&tables[j][i]) + bj);
+        }
+        st(&res[i], shuffle_fold(ld(&res[i]), l, r, ld(&zp[i]), ld(&zp[rn]), ld(&l0[i]), ld(&l_last[i]),
+                                 ld(&l_active[i]), y));
+    }
+}
+
+bool bad_n(int n) { return n <= 0 || (n & (n - 1)); }
+
+}  // namespace
+
+extern "C" {
+
+// cuda/bn254.cu:1269-1286 (the reference asserts 64 | n; any n >= 1 works here)
+hipError_t field_op(void *res, const void *l, int l_rot, const void *l_c, const void *r, int r_rot, const void *r_c,
+                    int size, int op, hipStream_t stream) {
+    if (size <= 0 || !res || op < 0 || op > 3 || (!l && !l_c)) return hipErrorInvalidValue;
+    k_field_op<<<grid_for(size), kThreads, 0, stream>>>((Fr *)res, (const Fr *)l, l_rot, (const Fr *)l_c,
+                                                        (const Fr *)r, r_rot, (const Fr *)r_c, (uint32_t)size, op);
+    return hipGetLastError();
+}
+
+// cuda/bn254.cu:1255-1267.  rot holds one entry per column pointer; the
+// number of entries is v_n minus the coefficient and separator slots.
+hipError_t field_op_batch_mul_sum(void *res, const void *v, const void *rot, int v_n, int n, hipStream_t stream) {
+    if (bad_n(n) || !res || v_n < 0 || (v_n && !v)) return hipErrorInvalidValue;
+    if (v_n == 0) return hipSuccess;
+    k_batch_mul_sum<<<grid_for(n), kThreads, 0, stream>>>((Fr *)res, (const Fr *const *)v, (const int *)rot, v_n,
+                                                          (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t field_mul_zip(void *buf, const void *coeff, int coeff_n, int n) {
+    if (n <= 0 || coeff_n <= 0 || !buf || !coeff) return hipErrorInvalidValue;
+    k_field_mul_zip<<<grid_for(n), kThreads, 0, 0>>>((Fr *)buf, (const Fr *)coeff, (uint32_t)coeff_n, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t permutation_eval_h_p1(void *res, const void *first_set, const void *last_set, const void *l0,
+                                 const void *l_last, const void *y, int n) {
+    if (n <= 0 || !res || !first_set || !last_set || !l0 || !l_last || !y) return hipErrorInvalidValue;
+    k_perm_p1<<<grid_for(n), kThreads, 0, 0>>>((Fr *)res, (const Fr *)first_set, (const Fr *)last_set, (const Fr *)l0,
+                                               (const Fr *)l_last, (const Fr *)y, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t permutation_eval_h_p2(void *res, const void *set, const void *l0, const void *l_last, const void *y,
+                                 int n_set, int rot, int n) {
+    (void)l_last;
+    if (bad_n(n) || !res || n_set < 0 || (n_set > 1 && !set) || !l0 || !y) return hipErrorInvalidValue;
+    k_perm_p2<<<grid_for(n), kThreads, 0, 0>>>((Fr *)res, (const Fr *const *)set, (const Fr *)l0, (const Fr *)y, n_set,
+                                               rot, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t permutation_eval_h_l(void *res, const void *beta, const void *gamma, const void *p, int n,
+                                hipStream_t stream) {
+    if (n <= 0 || !res || !beta || !gamma || !p) return hipErrorInvalidValue;
+    k_perm_l<<<grid_for(n), kThreads, 0, stream>>>((Fr *)res, (const Fr *)beta, (const Fr *)gamma, (const Fr *)p,
+                                                   (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t logup_eval_h(void *res, const void *input_product, const void *input_product_sum, const void *table,
+                        const void *multiplicity, const void *z_first, const void *z_last, const void *l0,
+                        const void *l_last, const void *l_active_row, const void *y, int rot, int n,
+                        hipStream_t stream) {
+    if (bad_n(n) || !res || !input_product || !input_product_sum || !table || !multiplicity || !z_first || !z_last ||
+        !l0 || !l_last || !l_active_row || !y)
+        return hipErrorInvalidValue;
+    k_logup_eval_h<<<grid_for(n), kThreads, 0, stream>>>(
+        (Fr *)res, (const Fr *)input_product, (const Fr *)input_product_sum, (const Fr *)table,
+        (const Fr *)multiplicity, (const Fr *)z_first, (const Fr *)z_last, (const Fr *)l0, (const Fr *)l_last,
+        (const Fr *)l_active_row, (const Fr *)y, rot, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t logup_eval_h_extra_inputs(void *res, const void *input_product, const void *input_product_sum,
+                                     const void *z, const void *l_active_row, const void *y, int rot, int n,
+                                     hipStream_t stream) {
+    if (bad_n(n) || !res || !input_product || !input_product_sum || !z || !l_active_row || !y)
+        return hipErrorInvalidValue;
+    k_logup_extra<<<grid_for(n), kThreads, 0, stream>>>((Fr *)res, (const Fr *)input_product,
+                                                        (const Fr *)input_product_sum, (const Fr *)z,
+                                                        (const Fr *)l_active_row, (const Fr *)y, rot, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t logup_eval_h_z_set(void *res, const void *set, const void *l0, const void *l_last, const void *y, int n_set,
+                              int rot, int n, hipStream_t stream) {
+    (void)l_last;
+    if (bad_n(n) || !res || n_set < 0 || (n_set > 1 && !set) || !l0 || !y) return hipErrorInvalidValue;
+    k_perm_p2<<<grid_for(n), kThreads, 0, stream>>>((Fr *)res, (const Fr *const *)set, (const Fr *)l0, (const Fr *)y,
+                                                    n_set, rot, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t logup_eval_h_inputs_product_sum(void *product, void *product_sum, const void *sets, int n_set, int n,
+                                           hipStream_t stream) {
+    if (n <= 0 || n_set <= 0 || !product || !product_sum || !sets) return hipErrorInvalidValue;
+    k_inputs_product_sum<<<grid_for(n), kThreads, 0, stream>>>((Fr *)product, (Fr *)product_sum,
+                                                               (const Fr *const *)sets, n_set, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t shuffle_eval_h(void *res, const void *input, const void *table, const void *z, const void *l0,
+                          const void *l_last, const void *l_active_row, const void *y, int rot, int n) {
+    if (bad_n(n) || !res || !input || !table || !z || !l0 || !l_last || !l_active_row || !y) return hipErrorInvalidValue;
+    k_shuffle<<<grid_for(n), kThreads, 0, 0>>>((Fr *)res, (const Fr *)input, (const Fr *)table, (const Fr *)z,
+                                               (const Fr *)l0, (const Fr *)l_last, (const Fr *)l_active_row,
+                                               (const Fr *)y, rot, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t shuffle_eval_h_v2(void *res, const void *inputs, const void *tables, const void *betas, int group_len,
+                             const void *z, const void *l0, const void *l_last, const void *l_active_row,
+                             const void *y, int rot, int n) {
+    if (bad_n(n) || !res || !inputs || !tables || !betas || group_len <= 0 || !z || !l0 || !l_last || !l_active_row ||
+        !y)
+        return hipErrorInvalidValue;
+    k_shuffle_v2<<<grid_for(n), kThreads, 0, 0>>>((Fr *)res, (const Fr *const *)inputs, (const Fr *const *)tables,
+                                                  (const Fr *)betas, group_len, (const Fr *)z, (const Fr *)l0,
+                                                  (const Fr *)l_last, (const Fr *)l_active_row, (const Fr *)y, rot,
+                                                  (uint32_t)n);
+    return hipGetLastError();
+}
+
+}  // extern "C"

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "ec.h"
+#include "zkwasm_prover_kernel.h"
 #include "ff.h"
 
 #define ZK_CHECK(x)                         \
@@ -25,5 +26,12 @@ __host__ __device__ inline bool is_pow2(uint64_t n) { return n && !(n & (n - 1))
 __device__ __forceinline__ uint32_t bitrev32(uint32_t x, uint32_t bits) {
     return bits ? (__builtin_bitreverse32(x) >> (32 - bits)) : 0u;
 }
+
+// cross-unit helpers (defined in ntt.hip)
+hipError_t fill_zero(void *p, size_t bytes, hipStream_t stream);
+hipError_t coset_scale(Fr *s, const Fr *cp, uint32_t cn, uint32_t n, hipStream_t stream);
+hipError_t ntt_run(Fr *a, Fr *b, const Fr *omegas, const Fr *n_inv, uint32_t log_n, hipStream_t stream,
+                   int *passes_out);
+int ntt_plan(uint32_t log_n, uint32_t *degs);
 
 }  // namespace zk

@@ -42,3 +42,27 @@ def ptr_array(tensors):
     """device array of device pointers (u64), as the reference passes lists"""
     arr = np.array([t.data_ptr() for t in tensors], dtype=np.uint64)
     return torch.from_numpy(arr.view(np.int64)).to(device())
+
+
+class Arena:
+    """Keeps device tensors alive while asynchronous kernels may still use
+    them (a pointer taken from a temporary tensor would otherwise be handed
+    back to the caching allocator and could alias the next allocation)."""
+
+    def __init__(self):
+        self.items = []
+
+    def hold(self, t):
+        self.items.append(t)
+        return t.data_ptr()
+
+    def put(self, arr):
+        return self.hold(to_dev(arr))
+
+    def ptrs(self, tensors):
+        for t in tensors:
+            self.items.append(t)
+        return self.hold(ptr_array(tensors))
+
+    def put_all(self, arrs):
+        return self.ptrs([to_dev(a) for a in arrs])
