@@ -1,0 +1,243 @@
+"""Benchmark of the halo2 KZG arithmetic core at zkWasm size on one MI355X.
+
+One step = one pass over the primitive set, inputs already resident in HBM:
+  G1 MSM n = 2^22 (msm + batch_msm_collect, the reference host-driver buffer
+  policy), Fr NTT 2^22, iNTT 2^22, coset NTT 2^22 -> 2^24 (extended_prepare +
+  NTT 2^24) and batch inversion of 2^22 nonzero Fr.
+`value` is the 2^22 MSM time in ms (lower is better); every primitive's time
+is reported beside it.  The roofline object covers the dominant kernel (the
+MSM bucket accumulation), timed with HIP events on its own stream.
+The CPU baseline is the C oracle's MSM on the same shape, on host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "zkwasm-prover_amd"))
+
+METRIC = "k=22 proof time (s); 2^22 MSM ms; batched proofs/hr vs plain at p99"
+LOG_N = 22
+EXT = 2
+# gfx950 VALU: 256 CUs x 64 lanes x 2.4 GHz, one 32-bit op per lane per cycle
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+HBM_PEAK_GBS = 8000.0
+# canonical work of one point-to-bucket addition: the Jacobian mixed add
+# (7M + 4S = 11 Fp multiplies), each a 8x8-limb CIOS product = 136 mad ops
+MULS_PER_BUCKET_ADD = 11
+OPS_PER_MONT_MUL = 136
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from zkprover import device as dv
+    from zkprover._lib import call, lib
+
+    L = lib()
+    L.zk_rand_fr.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_void_p]
+    L.zk_gen_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_void_p]
+    L.zk_msm_profile_enable.argtypes = [ctypes.c_int]
+    L.zk_msm_profile_enable.restype = None
+    L.zk_msm_profile_read.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.zk_msm_profile_read.restype = ctypes.c_int
+    L.zk_msm_default_window.argtypes = [ctypes.c_int]
+    L.zk_msm_default_window.restype = ctypes.c_int
+    L.zkp_msm_policy.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+    L.zkp_msm_policy.restype = None
+
+    dev = dv.device()
+    st = dv.stream_handle()
+    n = 1 << LOG_N
+    N = 1 << (LOG_N + EXT)
+
+    def u8(nbytes):
+        return torch.empty(nbytes, dtype=torch.uint8, device=dev)
+
+    # ---- inputs (seeded, generated on the device) -------------------------
+    points = u8(n * 64)
+    scalars = u8(n * 32)
+    call("zk_gen_points", points.data_ptr(), n, 4, st)
+    call("zk_rand_fr", scalars.data_ptr(), n, 3, st)
+    ntt_in = u8(n * 32)
+    call("zk_rand_fr", ntt_in.data_ptr(), n, 5, st)
+    coset_in = u8(n * 32)
+    call("zk_rand_fr", coset_in.data_ptr(), n, 6, st)
+    inv_in = u8(n * 32)
+    call("zk_rand_fr", inv_in.data_ptr(), n, 7, st)  # zero has probability ~2^-232
+
+    # twiddle tables as src/cuda/ntt.rs:36-59 builds them
+    def omega_table(log_n, inverse):
+        w = _root_of_unity_int(log_n)
+        if inverse:
+            w = pow(w, -1, _R)
+        init = np.zeros((1 << log_n, 4), dtype=np.uint64)
+        init[0] = _to_mont_limbs(1)
+        init[1] = _to_mont_limbs(w)
+        t = dv.to_dev(init)
+        call("expand_omega_buffer", t.data_ptr(), log_n, 1)
+        return t
+
+    om_n, om_n_inv, om_N = omega_table(LOG_N, False), omega_table(LOG_N, True), omega_table(LOG_N + EXT, False)
+    n_inv = dv.to_dev(np.array([_to_mont_limbs(pow(n, -1, _R))], dtype=np.uint64))
+    zeta = pow(7, (_R - 1) // 3, _R)
+    coset_powers = dv.to_dev(np.array([_to_mont_limbs(zeta), _to_mont_limbs(zeta * zeta % _R)], dtype=np.uint64))
+
+    # MSM buffers per the reference's host driver (src/cuda/msm.rs:284-311)
+    wb, windows, workers = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L.zkp_msm_policy(1, n, ctypes.byref(wb), ctypes.byref(windows), ctypes.byref(workers))
+    curve_slice = u8((((windows.value << wb.value) + 2 * workers.value) * 96))
+    single = n * windows.value
+    idx = [u8(single * 4) for _ in range(4)]
+    sort_buf = u8(single * 3 * 4)
+    acc_idx = u8(workers.value * 2 * 4)
+    slice_ptrs = dv.to_dev(np.array([curve_slice.data_ptr()], dtype=np.uint64).reshape(1, 1))
+
+    buf_a, buf_b = u8(N * 32), u8(N * 32)
+    inv_buf, inv_tmp = u8(n * 32), u8(n * 32)
+    swap = ctypes.c_bool(False)
+
+    def run_msm():
+        call("msm", curve_slice.data_ptr(), points.data_ptr(), scalars.data_ptr(), idx[0].data_ptr(),
+             idx[1].data_ptr(), idx[2].data_ptr(), idx[3].data_ptr(), sort_buf.data_ptr(), acc_idx.data_ptr(),
+             n, windows.value, wb.value, 64, workers.value, min(single * 12, 0x7fffffff), 0, st)
+        call("batch_msm_collect", None, None, None, None, slice_ptrs.data_ptr(), workers.value, windows.value,
+             wb.value, 1, st)
+
+    def run_ntt(log_n, om, ninv):
+        call("ntt", buf_a.data_ptr(), buf_b.data_ptr(), None, om.data_ptr(), dv.ptr(ninv), log_n, 8,
+             ctypes.addressof(swap), st)
+
+    def run_coset():
+        call("extended_prepare", buf_a.data_ptr(), coset_powers.data_ptr(), 3, n, N, 0, st)
+        call("ntt", buf_a.data_ptr(), buf_b.data_ptr(), None, om_N.data_ptr(), None, LOG_N + EXT, 8,
+             ctypes.addressof(swap), st)
+
+    def restore(dst, src):
+        return lambda: dst[: src.numel()].copy_(src)
+
+    # (name, untimed input restore, timed primitive)
+    prims = [("msm_2^22", None, run_msm),
+             ("ntt_2^22", restore(buf_a, ntt_in), lambda: run_ntt(LOG_N, om_n, None)),
+             ("intt_2^22", restore(buf_a, ntt_in), lambda: run_ntt(LOG_N, om_n_inv, n_inv)),
+             ("coset_ntt_2^22_to_2^24", restore(buf_a, coset_in), run_coset),
+             ("batch_inversion_2^22", restore(inv_buf, inv_in),
+              lambda: call("zk_batch_invert", inv_buf.data_ptr(), inv_tmp.data_ptr(), n, st))]
+
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        for _, prep, fn in prims:
+            if prep:
+                prep()
+            fn()
+    torch.cuda.synchronize()
+
+    # ---- timed region --------------------------------------------------------
+    L.zk_msm_profile_enable(1)
+    times = np.zeros(8, dtype=np.float64)
+    L.zk_msm_profile_read(times.ctypes.data, 8, 1)
+    ev = {name: [] for name, _, _ in prims}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for name, prep, fn in prims:
+            if prep:
+                prep()
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record()
+            fn()
+            e.record()
+            ev[name].append((s, e))
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    calls = L.zk_msm_profile_read(times.ctypes.data, 8, 1)
+    L.zk_msm_profile_enable(0)
+    prim_ms = {name: sum(s.elapsed_time(e) for s, e in lst) / len(lst) for name, lst in ev.items()}
+    phase_names = ["count", "scan", "scatter", "accumulate", "fixup", "reduce", "finalize"]
+    phases = {k: times[i] / max(calls, 1) for i, k in enumerate(phase_names)}
+    step_ms = wall_ms / args.steps  # includes the untimed input restores (D2D copies)
+
+    # ---- roofline of the dominant kernel (bucket accumulation) --------------
+    c = L.zk_msm_default_window(n)
+    w_used = math.ceil(255 / c)
+    units = n * w_used  # point-to-bucket additions per launch (zero digits have probability ~2^-c)
+    ops = units * MULS_PER_BUCKET_ADD * OPS_PER_MONT_MUL
+    acc_ms = phases["accumulate"]
+    achieved = ops / (acc_ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "valu", "kernel": "k_msm_accumulate", "achieved": round(achieved, 3),
+        "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+        "traffic": None, "avg_launch_ms": round(acc_ms, 4), "window_bits": c,
+        "algorithmic": f"{units} bucket additions x {MULS_PER_BUCKET_ADD} Fp mont-mul x "
+                       f"{OPS_PER_MONT_MUL} 32-bit mad ops",
+    }
+
+    out = {
+        "metric": METRIC, "value": round(prim_ms["msm_2^22"], 3), "unit": "ms (G1 MSM, n=2^22)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 limbs (exact 256-bit Montgomery)", "data": "synthetic",
+        "config": {"workload": "zkWasm-size primitives: BN254 G1 MSM n=2^22; Fr NTT and iNTT 2^22; "
+                               "coset NTT 2^22->2^24; batch inversion 2^22", "msm_window_bits": c,
+                   "parallelism": "single GPU"},
+        "primitives_ms": {k: round(v, 4) for k, v in prim_ms.items()},
+        "msm_phases_ms": {k: round(v, 4) for k, v in phases.items()},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n)
+    print(json.dumps(out), flush=True)
+
+
+_R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+
+
+def _root_of_unity_int(log_n):
+    w = pow(7, (_R - 1) >> 28, _R)
+    for _ in range(log_n, 28):
+        w = w * w % _R
+    return w
+
+
+def _to_mont_limbs(v):
+    m = (v << 256) % _R
+    return [(m >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(4)]
+
+
+def cpu_baseline(n):
+    """The C oracle's Pippenger MSM on host cores, same shape (n = 2^22)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import zkoracle as zo
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    zo.set_threads(cores)
+    pts = zo.gen_bases_ap(n, 4)
+    sc = zo.rand_fr(n, 3)
+    t0 = time.perf_counter()
+    zo.msm(pts, sc)
+    dt = time.perf_counter() - t0
+    return {"value": round(dt * 1e3, 1), "unit": "ms (G1 MSM, n=2^22)", "cores": cores, "kind": "port",
+            "sample": "one full n=2^22 MSM (C oracle Pippenger, unsigned windows, one thread per window)"}
+
+
+if __name__ == "__main__":
+    main()

@@ -185,6 +185,13 @@ size_t zk_msm_scratch_bytes(int n, int c);
 hipError_t zk_msm(void *out, const void *points, const void *scalars, int n, int c, void *scratch,
                   size_t scratch_bytes, hipStream_t stream);
 
+/* MSM phase timing for the bench: HIP events on the launch stream around each
+ * phase (count, scan, scatter, accumulate, fixup, reduce, finalize); reading
+ * returns the number of profiled calls.  Profiling waits for each MSM. */
+void zk_msm_profile_enable(int on);
+int zk_msm_profile_read(double *times, int ntimes, int reset);
+int zk_msm_default_window(int n);
+
 /* Element-wise field / curve operations and calibration kernels (tests and
  * bench only). */
 hipError_t zk_fr_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
@@ -195,6 +202,9 @@ hipError_t zk_bench_dbl(void *buf, int threads, int block, int iters, hipStream_
 hipError_t zk_bench_madd(void *acc, const void *pts, int npts, int threads, int block, int iters,
                          hipStream_t stream);
 hipError_t zk_copy(void *dst, const void *src, size_t bytes, hipStream_t stream);
+/* seeded synthetic inputs (bench): uniform Fr (Montgomery) and points k_i * G */
+hipError_t zk_rand_fr(void *out, int n, unsigned long long seed, hipStream_t stream);
+hipError_t zk_gen_points(void *out, int n, unsigned long long seed, hipStream_t stream);
 
 /* ---- host-side mirror of the reference's MSM driver -------------------- */
 /* src/cuda/msm.rs:274-283 window policy */

@@ -31,7 +31,7 @@ __global__ void k_fe_binop(Fe<P> *out, const Fe<P> *a, const Fe<P> *b, int op, u
 // out[i] = a[i] + b[i] on G1, with a, b affine; op selects the formula path
 // 0: XYZZ mixed add -> Jacobian, 1: Jacobian add, 2: XYZZ dbl of a,
 // 3: Jacobian dbl of a, 4: normalise(a + b) (z = 1)
-__global__ void k_g1_op(G1Jac *out, const G1Aff *a, const G1Aff *b, int op, uint32_t n) {
+__global__ __launch_bounds__(64) void k_g1_op(G1Jac *out, const G1Aff *a, const G1Aff *b, int op, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     G1Aff pa = aff_load(&a[i]);
@@ -87,9 +87,72 @@ __global__ void k_copy(uint4 *dst, const uint4 *src, size_t n16) {
     for (; i < n16; i += stride) dst[i] = src[i];
 }
 
+__device__ __forceinline__ uint64_t splitmix64(uint64_t &s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// counter-based uniform value < r: element i draws 4 x u64 from a SplitMix64
+// stream seeded by (seed, i), top limb masked to 254 bits, reduced once
+__device__ __forceinline__ Fr rand_canonical(uint64_t seed, uint64_t i) {
+    uint64_t st = seed ^ (i * 0xd1342543de82ef95ull + 0x632be59bd9b4e019ull);
+    Fr c;
+    for (int k = 0; k < 4; k++) {
+        uint64_t v = splitmix64(st);
+        c.v[2 * k] = (uint32_t)v;
+        c.v[2 * k + 1] = (uint32_t)(v >> 32);
+    }
+    c.v[7] &= 0x3fffffffu;
+    return fe_reduce_once<FrParams>(fe_reduce_once<FrParams>(c));
+}
+
+__global__ void k_rand_fr(Fr *out, uint32_t n, uint64_t seed) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fe_store<FrParams>(&out[i], fe_to_mont<FrParams>(rand_canonical(seed, i)));
+}
+
+// out[i] = k_i * G (affine, Montgomery) with k_i = rand_canonical(seed, i)
+__global__ __launch_bounds__(64) void k_gen_points(G1Aff *out, uint32_t n, uint64_t seed) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Fr k = rand_canonical(seed, i);
+    Fp gx = fe_from_u32<FpParams>(1), gy = fe_from_u32<FpParams>(2);
+    G1Xyzz acc = xyzz_id();
+    for (int b = 253; b >= 0; b--) {
+        acc = xyzz_dbl(acc);
+        if ((k.v[b >> 5] >> (b & 31)) & 1u) acc = xyzz_add_aff(acc, gx, gy);
+    }
+    G1Jac j = jac_normalize(xyzz_to_jac(acc));
+    G1Aff r;
+    if (fe_is_zero<FpParams>(j.z)) {
+        r.x = fe_zero<FpParams>();
+        r.y = fe_zero<FpParams>();
+    } else {
+        r.x = j.x;
+        r.y = j.y;
+    }
+    fe_store<FpParams>(&out[i].x, r.x);
+    fe_store<FpParams>(&out[i].y, r.y);
+}
+
 }  // namespace
 
 extern "C" {
+
+// Seeded synthetic inputs for the bench (not the oracle's SplitMix64 stream).
+hipError_t zk_rand_fr(void *out, int n, unsigned long long seed, hipStream_t stream) {
+    if (n <= 0 || !out) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    k_rand_fr<<<(n + 255) / 256, 256, 0, stream>>>((Fr *)out, (uint32_t)n, seed);
+    return hipGetLastError();
+}
+hipError_t zk_gen_points(void *out, int n, unsigned long long seed, hipStream_t stream) {
+    if (n <= 0 || !out) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    k_gen_points<<<(n + 63) / 64, 64, 0, stream>>>((G1Aff *)out, (uint32_t)n, seed);
+    return hipGetLastError();
+}
 
 hipError_t zk_fr_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
     if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;

@@ -37,7 +37,7 @@ __device__ __forceinline__ Fr load_plus(const Fr *x, size_t i, const Fr *beta, b
 }
 
 // tot[t] = prod_k x'[t + k*T] (zeros skipped)
-__global__ void k_binv_totals(const Fr *__restrict__ x, const Fr *__restrict__ beta, Fr *__restrict__ tot,
+__global__ __launch_bounds__(256) void k_binv_totals(const Fr *__restrict__ x, const Fr *__restrict__ beta, Fr *__restrict__ tot,
                               uint32_t n, uint32_t T) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
@@ -53,7 +53,7 @@ __global__ void k_binv_totals(const Fr *__restrict__ x, const Fr *__restrict__ b
 
 // inverse of each element from the inverse of its thread's product
 template <int MODE>
-__global__ void k_binv_apply(const Fr *__restrict__ x, const Fr *__restrict__ beta,
+__global__ __launch_bounds__(256) void k_binv_apply(const Fr *__restrict__ x, const Fr *__restrict__ beta,
                              const Fr *__restrict__ inv_tot, Fr *out, uint32_t n, uint32_t T) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
@@ -61,21 +61,16 @@ __global__ void k_binv_apply(const Fr *__restrict__ x, const Fr *__restrict__ be
     Fr b = hb ? fe_load<FrParams>(beta) : fe_zero<FrParams>();
     Fr v[kM], pre[kM];
     Fr acc = fe_one<FrParams>();
-    int cnt = 0;
-#pragma unroll
+#pragma unroll 8
     for (int k = 0; k < kM; k++) {
         uint32_t i = t + (uint32_t)k * T;
-        if (i < n) {
-            v[k] = load_plus(x, i, beta, hb, b);
-            pre[k] = acc;
-            if (!fe_is_zero<FrParams>(v[k])) acc = acc * v[k];
-            cnt = k + 1;
-        }
+        v[k] = i < n ? load_plus(x, i, beta, hb, b) : fe_zero<FrParams>();
+        pre[k] = acc;
+        if (!fe_is_zero<FrParams>(v[k])) acc = acc * v[k];
     }
     Fr inv = fe_load<FrParams>(&inv_tot[t]);
-#pragma unroll
+#pragma unroll 8
     for (int k = kM - 1; k >= 0; k--) {
-        if (k >= cnt) continue;
         uint32_t i = t + (uint32_t)k * T;
         Fr r;
         if (fe_is_zero<FrParams>(v[k])) {
@@ -84,8 +79,10 @@ __global__ void k_binv_apply(const Fr *__restrict__ x, const Fr *__restrict__ be
             r = inv * pre[k];
             inv = inv * v[k];
         }
-        if (MODE == OUT_ADD) r = fe_load<FrParams>(&out[i]) + r;
-        fe_store<FrParams>(&out[i], r);
+        if (i < n) {
+            if (MODE == OUT_ADD) r = fe_load<FrParams>(&out[i]) + r;
+            fe_store<FrParams>(&out[i], r);
+        }
     }
 }
 
@@ -177,6 +174,7 @@ __global__ __launch_bounds__(256) void k_fscan_top(Fr *tot, uint32_t nb, const F
         uint32_t b0 = base + threadIdx.x * 8;
         Fr loc[8];
         Fr acc = op_identity<MUL>();
+#pragma unroll 8
         for (uint32_t k = 0; k < 8; k++) {
             loc[k] = b0 + k < nb ? fe_load<FrParams>(&tot[b0 + k]) : op_identity<MUL>();
             acc = op_combine<MUL>(acc, loc[k]);
@@ -191,6 +189,7 @@ __global__ __launch_bounds__(256) void k_fscan_top(Fr *tot, uint32_t nb, const F
             __syncthreads();
         }
         Fr pre = op_combine<MUL>(run, threadIdx.x ? sh[threadIdx.x - 1] : op_identity<MUL>());
+#pragma unroll 8
         for (uint32_t k = 0; k < 8; k++) {
             if (b0 + k < nb) fe_store<FrParams>(&tot[b0 + k], pre);
             pre = op_combine<MUL>(pre, loc[k]);
@@ -209,6 +208,7 @@ __global__ __launch_bounds__(256) void k_fscan_apply(Fr *z, uint32_t n, const Fr
     uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 8;
     Fr loc[8];
     Fr acc = op_identity<MUL>();
+#pragma unroll 8
     for (uint32_t k = 0; k < 8; k++) {
         loc[k] = base + k < n ? fe_load<FrParams>(&z[base + k]) : op_identity<MUL>();
         acc = op_combine<MUL>(acc, loc[k]);
@@ -223,6 +223,7 @@ __global__ __launch_bounds__(256) void k_fscan_apply(Fr *z, uint32_t n, const Fr
     }
     Fr pre = op_combine<MUL>(fe_load<FrParams>(&tot[blockIdx.x]),
                              threadIdx.x ? sh[threadIdx.x - 1] : op_identity<MUL>());
+#pragma unroll 8
     for (uint32_t k = 0; k < 8; k++) {
         uint32_t i = base + k;
         if (i < n) {

@@ -6,12 +6,15 @@
 // bytes after batch_msm_collect.
 //
 // Pipeline (one MSM):
-//   1. count    one thread per scalar: Montgomery -> canonical, signed c-bit
-//               digits d_w in (-2^(c-1), 2^(c-1)], atomic histogram of
-//               buckets (w, |d_w| - 1).  Zero digits add nothing.
-//   2. scan     exclusive prefix sum of the histogram -> bucket offsets.
-//   3. scatter  recompute digits, place (point index | sign << 31) at an
-//               atomically reserved slot of its bucket (a counting sort).
+//   1. digits   one thread per scalar: Montgomery -> canonical, signed c-bit
+//               digits d_w in (-2^(c-1), 2^(c-1)], written window-major.
+//               Zero digits add nothing.
+//   2. sort     two-level counting sort of (bucket, point index | sign):
+//               per-tile LDS histograms of the high 8 bucket bits, one global
+//               scan, scatter into (window, high bits) groups with LDS ranks,
+//               then an LDS counting sort of each group on the low 8 bits that
+//               also emits the bucket offsets.  Writes land in short
+//               contiguous runs instead of 4-byte random scatters.
 //   4. accumulate  threads own equal chunks of the sorted entries (perfect
 //               load balance for any digit distribution); runs of one bucket
 //               are summed with XYZZ mixed additions; runs cut by a chunk
@@ -38,13 +41,22 @@ constexpr uint32_t kChunk = 48;      // sorted entries per accumulation thread
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMagic = 0x4d534d31u;  // "MSM1": window sums present in the slice
 
+// Arrays placed into caller memory, in two lifetimes: phase 1 (the sort) and
+// phase 2 (accumulation and reduction).  offsets and entries live across both;
+// the phase-1 temporaries (tmp, hist, starts, scan) are free again in phase 2.
 enum MsmArr {
-    A_COUNTS, A_OFFSETS, A_CURSOR, A_ENTRIES, A_BUCKETS, A_PHEAD, A_PTAIL, A_HB, A_TB, A_WS1, A_SUM1, A_Q,
-    A_SCAN, A_COUNT
+    A_OFFSETS, A_ENTRIES,                                          // both phases
+    A_TMP, A_HIST, A_STARTS, A_SCAN,                               // phase 1
+    A_BUCKETS, A_PHEAD, A_PTAIL, A_HB, A_TB, A_WS1, A_SUM1, A_Q,   // phase 2
+    A_COUNT
 };
+constexpr int kFirstPhase2 = A_BUCKETS;
+
+constexpr uint32_t kSortTile = 4096;  // scalars per histogram / scatter tile
 
 struct MsmPlan {
     uint32_t n, c, W, B, NB, T1, nbits;
+    uint32_t hi_bits, lo_bits, nbins, ntiles, nhist;
     uint64_t max_entries, chunks;
     size_t size[A_COUNT];
     uint8_t *ptr[A_COUNT];
@@ -64,10 +76,18 @@ MsmPlan make_plan(uint32_t n, uint32_t c) {
     p.nbits = log2_exact(p.T1);
     p.max_entries = (uint64_t)n * p.W;
     p.chunks = (p.max_entries + kChunk - 1) / kChunk;
-    p.size[A_COUNTS] = (size_t)p.NB * 4;
+    p.hi_bits = c - 1 > 8 ? c - 1 - 8 : 0;
+    p.lo_bits = (c - 1) - p.hi_bits;
+    p.nbins = 1u << p.hi_bits;
+    p.ntiles = (n + kSortTile - 1) / kSortTile;
+    p.nhist = p.W * p.nbins * p.ntiles;
     p.size[A_OFFSETS] = (size_t)(p.NB + 1) * 4;
-    p.size[A_CURSOR] = (size_t)p.NB * 4;
+    // digits (window-major, n * W u32) share this array with the final entries
     p.size[A_ENTRIES] = (size_t)p.max_entries * 4;
+    p.size[A_TMP] = (size_t)p.max_entries * 8;
+    p.size[A_HIST] = (size_t)p.nhist * 4;
+    p.size[A_STARTS] = (size_t)(p.nhist + 1) * 4;
+    p.size[A_SCAN] = ((size_t)(p.nhist + 1 + 1023) / 1024 + 1) * 4;
     p.size[A_BUCKETS] = (size_t)p.NB * sizeof(G1Xyzz);
     p.size[A_PHEAD] = (size_t)p.chunks * sizeof(G1Xyzz);
     p.size[A_PTAIL] = (size_t)p.chunks * sizeof(G1Xyzz);
@@ -76,44 +96,61 @@ MsmPlan make_plan(uint32_t n, uint32_t c) {
     p.size[A_WS1] = (size_t)p.W * p.T1 * sizeof(G1Xyzz);
     p.size[A_SUM1] = (size_t)p.W * p.T1 * sizeof(G1Xyzz);
     p.size[A_Q] = (size_t)p.W * (p.nbits + 1) * sizeof(G1Xyzz);
-    p.size[A_SCAN] = ((size_t)(p.NB + 1 + 1023) / 1024 + 1) * 4;
     return p;
 }
 
+// bytes that always suffice in one region: every array gets its own block
+// (placement may still reuse phase-1 blocks, which only leaves room unused)
 size_t plan_total(const MsmPlan &p) {
     size_t t = 0;
     for (int a = 0; a < A_COUNT; a++) t += align256(p.size[a]);
-    return t;
+    return t + 256;
 }
 
-// Places the plan's arrays into caller memory regions, largest array first,
-// each into the region with the most room left.  Returns false if they do
-// not fit.  Region bases are aligned up to 256 B first.
-bool place(MsmPlan &p, uint8_t *const *base, const size_t *bytes, int nreg) {
-    uint8_t *cur[8];
-    size_t left[8];
-    for (int r = 0; r < nreg; r++) {
-        uintptr_t b = (uintptr_t)base[r];
-        uintptr_t a = (b + 255) & ~(uintptr_t)255;
-        cur[r] = (uint8_t *)a;
-        left[r] = (base[r] && bytes[r] > a - b) ? bytes[r] - (a - b) : 0;
-    }
+struct Seg {
+    uint8_t *ptr;
+    size_t left;
+};
+
+// Largest array first, each into the free segment with the most room.
+bool place_range(MsmPlan &p, int lo, int hi, Seg *segs, int &nseg) {
     bool done[A_COUNT] = {false};
-    for (int k = 0; k < A_COUNT; k++) {
+    for (int k = lo; k < hi; k++) {
         int best = -1;
-        for (int a = 0; a < A_COUNT; a++)
+        for (int a = lo; a < hi; a++)
             if (!done[a] && (best < 0 || p.size[a] > p.size[best])) best = a;
         size_t need = align256(p.size[best]);
         int rr = -1;
-        for (int r = 0; r < nreg; r++)
-            if (left[r] >= need && (rr < 0 || left[r] > left[rr])) rr = r;
+        for (int r = 0; r < nseg; r++)
+            if (segs[r].left >= need && (rr < 0 || segs[r].left > segs[rr].left)) rr = r;
         if (rr < 0) return false;
-        p.ptr[best] = cur[rr];
-        cur[rr] += need;
-        left[rr] -= need;
+        p.ptr[best] = segs[rr].ptr;
+        segs[rr].ptr += need;
+        segs[rr].left -= need;
         done[best] = true;
     }
     return true;
+}
+
+// Places the plan into caller regions (bases aligned up to 256 B).  Phase-2
+// arrays may reuse the blocks of the phase-1 temporaries.
+bool place(MsmPlan &p, uint8_t *const *base, const size_t *bytes, int nreg) {
+    Seg segs[16];
+    int nseg = 0;
+    for (int r = 0; r < nreg && r < 8; r++) {
+        uintptr_t b = (uintptr_t)base[r];
+        uintptr_t a = (b + 255) & ~(uintptr_t)255;
+        segs[nseg].ptr = (uint8_t *)a;
+        segs[nseg].left = (base[r] && bytes[r] > a - b) ? bytes[r] - (a - b) : 0;
+        nseg++;
+    }
+    if (!place_range(p, A_OFFSETS, kFirstPhase2, segs, nseg)) return false;
+    for (int a = A_TMP; a < kFirstPhase2; a++) {  // phase-1 temporaries become free
+        segs[nseg].ptr = p.ptr[a];
+        segs[nseg].left = align256(p.size[a]);
+        nseg++;
+    }
+    return place_range(p, kFirstPhase2, A_COUNT, segs, nseg);
 }
 
 // Default window for a single MSM of n points (the autotune table).
@@ -146,33 +183,93 @@ __device__ __forceinline__ int32_t signed_digit(const uint32_t s[8], uint32_t w,
     return (int32_t)v;
 }
 
-__global__ void k_msm_count(const Fr *__restrict__ scalars, uint32_t n, uint32_t c, uint32_t W,
-                            uint32_t *__restrict__ counts) {
+// digits[w * n + i] = 0 for a zero digit, else |d| | (d < 0) << 31 with
+// |d| in [1, 2^(c-1)] (bucket |d| - 1)
+__global__ __launch_bounds__(256) void k_msm_digits(const Fr *__restrict__ scalars, uint32_t n, uint32_t c,
+                                                    uint32_t W, uint32_t *__restrict__ digits) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Fr s = fe_from_mont<FrParams>(fe_load<FrParams>(&scalars[i]));
-    const uint32_t B = 1u << (c - 1);
     uint32_t carry = 0;
     for (uint32_t w = 0; w < W; w++) {
         int32_t d = signed_digit(s.v, w, c, carry);
-        if (d) atomicAdd(&counts[w * B + (uint32_t)(d < 0 ? -d : d) - 1], 1u);
+        digits[(size_t)w * n + i] = d ? ((uint32_t)(d < 0 ? -d : d) | (d < 0 ? 0x80000000u : 0u)) : 0u;
     }
 }
 
-__global__ void k_msm_scatter(const Fr *__restrict__ scalars, uint32_t n, uint32_t c, uint32_t W,
-                              uint32_t *__restrict__ cursor, uint32_t *__restrict__ entries) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Fr s = fe_from_mont<FrParams>(fe_load<FrParams>(&scalars[i]));
-    const uint32_t B = 1u << (c - 1);
-    uint32_t carry = 0;
-    for (uint32_t w = 0; w < W; w++) {
-        int32_t d = signed_digit(s.v, w, c, carry);
+// per (tile, window): histogram of the high bucket bits
+// hist[(w * nbins + bin) * ntiles + tile]
+__global__ __launch_bounds__(256) void k_msm_hist(const uint32_t *__restrict__ digits, uint32_t n, uint32_t lo_bits,
+                                                  uint32_t nbins, uint32_t ntiles, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[256];
+    const uint32_t tile = blockIdx.x, w = blockIdx.y;
+    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    const uint32_t *dw = digits + (size_t)w * n;
+    uint32_t end = (tile + 1) * kSortTile < n ? (tile + 1) * kSortTile : n;
+    for (uint32_t i = tile * kSortTile + threadIdx.x; i < end; i += blockDim.x) {
+        uint32_t d = dw[i];
+        if (d) atomicAdd(&h[((d & 0x7fffffffu) - 1) >> lo_bits], 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) hist[((size_t)w * nbins + k) * ntiles + tile] = h[k];
+}
+
+// per (tile, window): place each nonzero digit into its (window, bin) group
+// at starts[...] + rank; tmp entry = (low bucket bits, index | sign)
+__global__ __launch_bounds__(256) void k_msm_scatter_hi(const uint32_t *__restrict__ digits, uint32_t n,
+                                                        uint32_t lo_bits, uint32_t nbins, uint32_t ntiles,
+                                                        const uint32_t *__restrict__ starts, uint2 *__restrict__ tmp) {
+    __shared__ uint32_t cur[256];
+    const uint32_t tile = blockIdx.x, w = blockIdx.y;
+    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) cur[k] = starts[((size_t)w * nbins + k) * ntiles + tile];
+    __syncthreads();
+    const uint32_t *dw = digits + (size_t)w * n;
+    const uint32_t lo_mask = (1u << lo_bits) - 1;
+    uint32_t end = (tile + 1) * kSortTile < n ? (tile + 1) * kSortTile : n;
+    for (uint32_t i = tile * kSortTile + threadIdx.x; i < end; i += blockDim.x) {
+        uint32_t d = dw[i];
         if (d) {
-            uint32_t b = w * B + (uint32_t)(d < 0 ? -d : d) - 1;
-            uint32_t pos = atomicAdd(&cursor[b], 1u);
-            entries[pos] = i | (d < 0 ? 0x80000000u : 0u);
+            uint32_t b = (d & 0x7fffffffu) - 1;
+            uint32_t pos = atomicAdd(&cur[b >> lo_bits], 1u);
+            tmp[pos] = make_uint2(b & lo_mask, i | (d & 0x80000000u));
         }
+    }
+}
+
+// per (window, bin) group: counting sort on the low bucket bits in LDS;
+// writes the final entries and the offsets of the group's buckets
+__global__ __launch_bounds__(256) void k_msm_sort_lo(const uint2 *__restrict__ tmp, const uint32_t *__restrict__ starts,
+                                                     uint32_t nbins, uint32_t ntiles, uint32_t lo_bits, uint32_t B,
+                                                     uint32_t ngroups, uint32_t *__restrict__ entries,
+                                                     uint32_t *__restrict__ offsets) {
+    __shared__ uint32_t h[256];
+    __shared__ uint32_t cur[256];
+    const uint32_t g = blockIdx.x;
+    const uint32_t w = g / nbins, bin = g % nbins;
+    const uint32_t nlo = 1u << lo_bits;
+    const uint32_t gs = starts[(size_t)g * ntiles];
+    const uint32_t ge = starts[(size_t)(g + 1) * ntiles];  // starts has a final total entry
+    for (uint32_t k = threadIdx.x; k < nlo; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    for (uint32_t p = gs + threadIdx.x; p < ge; p += blockDim.x) atomicAdd(&h[tmp[p].x], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = gs;
+        for (uint32_t k = 0; k < nlo; k++) {
+            cur[k] = run;
+            run += h[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t base = w * B + (bin << lo_bits);
+    for (uint32_t k = threadIdx.x; k < nlo; k += blockDim.x) offsets[base + k] = cur[k];
+    if (g == ngroups - 1 && threadIdx.x == 0) offsets[(size_t)w * B + B] = ge;  // = total: offsets[NB]
+    __syncthreads();
+    for (uint32_t p = gs + threadIdx.x; p < ge; p += blockDim.x) {
+        uint2 t = tmp[p];
+        uint32_t pos = atomicAdd(&cur[t.x], 1u);
+        entries[pos] = t.y;
     }
 }
 
@@ -257,7 +354,7 @@ __device__ __forceinline__ G1Xyzz xyzz_add_entry(const G1Xyzz &acc, const G1Aff 
     return xyzz_add_aff(acc, q.x, y);
 }
 
-__global__ void k_msm_accumulate(const uint32_t *__restrict__ entries, const uint32_t *__restrict__ off,
+__global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restrict__ entries, const uint32_t *__restrict__ off,
                                  uint32_t NB, const G1Aff *__restrict__ pts, G1Xyzz *__restrict__ buckets,
                                  G1Xyzz *__restrict__ phead, G1Xyzz *__restrict__ ptail,
                                  uint32_t *__restrict__ hb, uint32_t *__restrict__ tb, uint64_t nchunks) {
@@ -302,7 +399,7 @@ __global__ void k_msm_accumulate(const uint32_t *__restrict__ entries, const uin
 }
 
 // merges the partial sums of buckets that span several chunks
-__global__ void k_msm_fixup(G1Xyzz *__restrict__ buckets, const G1Xyzz *__restrict__ phead,
+__global__ __launch_bounds__(256) void k_msm_fixup(G1Xyzz *__restrict__ buckets, const G1Xyzz *__restrict__ phead,
                             const G1Xyzz *__restrict__ ptail, const uint32_t *__restrict__ hb,
                             const uint32_t *__restrict__ tb, uint64_t nchunks) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,7 +417,7 @@ __global__ void k_msm_fixup(G1Xyzz *__restrict__ buckets, const G1Xyzz *__restri
 
 // ---- reduction -------------------------------------------------------------
 // one thread per (window, segment of kL1 buckets)
-__global__ void k_msm_reduce_l1(const G1Xyzz *__restrict__ buckets, const uint32_t *__restrict__ off,
+__global__ __launch_bounds__(64) void k_msm_reduce_l1(const G1Xyzz *__restrict__ buckets, const uint32_t *__restrict__ off,
                                 uint32_t W, uint32_t B, uint32_t T1, G1Xyzz *__restrict__ ws1,
                                 G1Xyzz *__restrict__ sum1) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,7 +461,7 @@ struct SliceHeader {
 };
 
 // one thread per window: S_w = A + L1 * sum_bit 2^bit Q_bit -> slice
-__global__ void k_msm_window_finalize(const G1Xyzz *__restrict__ q, uint32_t W, uint32_t nbits, uint32_t c,
+__global__ __launch_bounds__(64) void k_msm_window_finalize(const G1Xyzz *__restrict__ q, uint32_t W, uint32_t nbits, uint32_t c,
                                       uint8_t *__restrict__ slice) {
     uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= W) return;
@@ -409,18 +506,51 @@ __device__ void collect_slice(uint8_t *slice) {
     jac_store(reinterpret_cast<G1Jac *>(slice), jac_normalize(acc));
 }
 
-__global__ void k_msm_collect(uint8_t *const *__restrict__ slices, uint32_t batch) {
+__global__ __launch_bounds__(64) void k_msm_collect(uint8_t *const *__restrict__ slices, uint32_t batch) {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= batch) return;
     collect_slice(slices[m]);
 }
-__global__ void k_msm_collect_one(uint8_t *slice) {
+__global__ __launch_bounds__(64) void k_msm_collect_one(uint8_t *slice) {
     if (threadIdx.x == 0 && blockIdx.x == 0) collect_slice(slice);
 }
 
 // device-side slice header reset so a stale header is never trusted
 __global__ void k_slice_clear(uint8_t *slice) {
     if (threadIdx.x == 0) reinterpret_cast<SliceHeader *>(slice + 64)->magic = 0;
+}
+
+// Opt-in phase timing: when enabled, msm_windows records a HIP event on the
+// launch stream after each phase, waits for the last one and accumulates
+// the per-phase durations (ms) and call counts.
+enum MsmPhase { PH_COUNT = 0, PH_SCAN, PH_SCATTER, PH_ACCUMULATE, PH_FIXUP, PH_REDUCE, PH_FINALIZE, PH_NUM };
+struct PhaseProf {
+    bool on = false;
+    double ms[PH_NUM] = {0};
+    uint64_t calls = 0;
+    hipEvent_t ev[PH_NUM + 1] = {};
+    bool init = false;
+};
+PhaseProf g_prof;
+
+hipError_t prof_mark(int k, hipStream_t stream) {
+    if (!g_prof.on) return hipSuccess;
+    if (!g_prof.init) {
+        for (auto &e : g_prof.ev) ZK_CHECK(hipEventCreate(&e));
+        g_prof.init = true;
+    }
+    return hipEventRecord(g_prof.ev[k], stream);
+}
+hipError_t prof_finish() {
+    if (!g_prof.on) return hipSuccess;
+    ZK_CHECK(hipEventSynchronize(g_prof.ev[PH_NUM]));
+    for (int k = 0; k < PH_NUM; k++) {
+        float ms = 0;
+        ZK_CHECK(hipEventElapsedTime(&ms, g_prof.ev[k], g_prof.ev[k + 1]));
+        g_prof.ms[k] += ms;
+    }
+    g_prof.calls++;
+    return hipSuccess;
 }
 
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *cursor, uint32_t *sums,
@@ -450,21 +580,30 @@ hipError_t msm_windows(uint8_t *slice, size_t slice_bytes, const G1Aff *points, 
                        hipStream_t stream) {
     if (n == 0 || n > 0x7fffffffu || !slice || !points || !scalars || nreg < 1 || nreg > 8)
         return hipErrorInvalidValue;
-    if (c != 0 && (c < 2 || c > 24)) return hipErrorInvalidValue;
+    if (c != 0 && (c < 2 || c > 17)) return hipErrorInvalidValue;  // sort kernels: <= 8 + 8 bucket bits
     MsmPlan p;
     bool ok = false;
-    uint32_t c0 = c ? c : default_window(n);
-    for (uint32_t cc = c0; cc >= 2 && !ok; cc--) {
-        p = make_plan(n, cc);
+    const uint32_t c0 = c ? c : default_window(n);
+    // the default window first, then larger windows (fewer entries), then smaller
+    uint32_t order[32];
+    int norder = 0;
+    order[norder++] = c0;
+    if (!c) {
+        for (uint32_t cc = c0 + 1; cc <= 17; cc++) order[norder++] = cc;
+        for (uint32_t cc = c0 - 1; cc >= 2; cc--) order[norder++] = cc;
+    }
+    for (int k = 0; k < norder && !ok; k++) {
+        p = make_plan(n, order[k]);
         ok = (size_t)sizeof(G1Jac) * (p.W + 1) <= slice_bytes && place(p, regions, region_bytes, nreg);
-        if (c) break;
     }
     if (!ok) return hipErrorOutOfMemory;
     c = p.c;
-    uint32_t *counts = (uint32_t *)p.ptr[A_COUNTS];
+    uint32_t *hist = (uint32_t *)p.ptr[A_HIST];
     uint32_t *offsets = (uint32_t *)p.ptr[A_OFFSETS];
-    uint32_t *cursor = (uint32_t *)p.ptr[A_CURSOR];
+    uint32_t *starts = (uint32_t *)p.ptr[A_STARTS];  // dead after k_msm_sort_lo
     uint32_t *entries = (uint32_t *)p.ptr[A_ENTRIES];
+    uint32_t *digits = entries;  // dead before the final entries are written
+    uint2 *tmp = (uint2 *)p.ptr[A_TMP];  // dead after k_msm_sort_lo
     G1Xyzz *buckets = (G1Xyzz *)p.ptr[A_BUCKETS];
     G1Xyzz *phead = (G1Xyzz *)p.ptr[A_PHEAD];
     G1Xyzz *ptail = (G1Xyzz *)p.ptr[A_PTAIL];
@@ -475,28 +614,41 @@ hipError_t msm_windows(uint8_t *slice, size_t slice_bytes, const G1Aff *points, 
     G1Xyzz *q = (G1Xyzz *)p.ptr[A_Q];
     uint32_t *scan_sums = (uint32_t *)p.ptr[A_SCAN];
 
+    ZK_CHECK(prof_mark(PH_COUNT, stream));
     k_slice_clear<<<1, 1, 0, stream>>>(slice);
     ZK_CHECK(hipGetLastError());
-    ZK_CHECK(fill_zero(counts, align256(p.size[A_COUNTS]), stream));
     const uint32_t blocks = (n + 255) / 256;
-    k_msm_count<<<blocks, 256, 0, stream>>>(scalars, n, c, p.W, counts);
+    k_msm_digits<<<blocks, 256, 0, stream>>>(scalars, n, c, p.W, digits);
     ZK_CHECK(hipGetLastError());
-    ZK_CHECK(launch_scan(counts, p.NB, offsets, cursor, scan_sums, stream));
-    k_msm_scatter<<<blocks, 256, 0, stream>>>(scalars, n, c, p.W, cursor, entries);
+    k_msm_hist<<<dim3(p.ntiles, p.W), 256, 0, stream>>>(digits, n, p.lo_bits, p.nbins, p.ntiles, hist);
     ZK_CHECK(hipGetLastError());
+    ZK_CHECK(prof_mark(PH_SCAN, stream));
+    ZK_CHECK(launch_scan(hist, p.nhist, starts, nullptr, scan_sums, stream));
+    ZK_CHECK(prof_mark(PH_SCATTER, stream));
+    k_msm_scatter_hi<<<dim3(p.ntiles, p.W), 256, 0, stream>>>(digits, n, p.lo_bits, p.nbins, p.ntiles, starts, tmp);
+    ZK_CHECK(hipGetLastError());
+    k_msm_sort_lo<<<p.W * p.nbins, 256, 0, stream>>>(tmp, starts, p.nbins, p.ntiles, p.lo_bits, p.B, p.W * p.nbins,
+                                                     entries, offsets);
+    ZK_CHECK(hipGetLastError());
+    ZK_CHECK(prof_mark(PH_ACCUMULATE, stream));
     const uint32_t cblocks = (uint32_t)((p.chunks + 255) / 256);
     k_msm_accumulate<<<cblocks, 256, 0, stream>>>(entries, offsets, p.NB, points, buckets, phead, ptail, hb,
                                                   tb, p.chunks);
     ZK_CHECK(hipGetLastError());
+    ZK_CHECK(prof_mark(PH_FIXUP, stream));
     k_msm_fixup<<<cblocks, 256, 0, stream>>>(buckets, phead, ptail, hb, tb, p.chunks);
     ZK_CHECK(hipGetLastError());
+    ZK_CHECK(prof_mark(PH_REDUCE, stream));
     const uint32_t nseg = p.W * p.T1;
     k_msm_reduce_l1<<<(nseg + 63) / 64, 64, 0, stream>>>(buckets, offsets, p.W, p.B, p.T1, ws1, sum1);
     ZK_CHECK(hipGetLastError());
     k_msm_masked_sums<<<dim3(p.nbits + 1, p.W), 256, 0, stream>>>(ws1, sum1, p.T1, p.nbits, q);
     ZK_CHECK(hipGetLastError());
+    ZK_CHECK(prof_mark(PH_FINALIZE, stream));
     k_msm_window_finalize<<<(p.W + 63) / 64, 64, 0, stream>>>(q, p.W, p.nbits, c, slice);
-    return hipGetLastError();
+    ZK_CHECK(hipGetLastError());
+    ZK_CHECK(prof_mark(PH_NUM, stream));
+    return prof_finish();
 }
 
 hipError_t msm_collect(uint8_t *const *slices_dev, uint32_t batch, hipStream_t stream) {
@@ -553,6 +705,22 @@ hipError_t batch_msm_collect(void *remain_indices, void *remain_acc, void *next_
     if (!buckets && batch_size) return hipErrorInvalidValue;
     return msm_collect((uint8_t *const *)buckets, batch_size, stream);
 }
+
+// Phase timing (bench instrumentation): enable / read / reset.  times[k]
+// receives the accumulated ms of phase k (count, scan, scatter, accumulate,
+// fixup, reduce, window-finalize); returns the number of profiled calls.
+void zk_msm_profile_enable(int on) { g_prof.on = on != 0; }
+int zk_msm_profile_read(double *times, int ntimes, int reset) {
+    for (int k = 0; k < ntimes && k < PH_NUM; k++) times[k] = g_prof.ms[k];
+    int calls = (int)g_prof.calls;
+    if (reset) {
+        for (double &m : g_prof.ms) m = 0;
+        g_prof.calls = 0;
+    }
+    return calls;
+}
+// The window zk_msm / msm() pick for n points when not told otherwise.
+int zk_msm_default_window(int n) { return n > 0 ? (int)default_window((uint32_t)n) : 0; }
 
 // Scratch bytes zk_msm needs for n points at window c (0: default window).
 size_t zk_msm_scratch_bytes(int n, int c) {
