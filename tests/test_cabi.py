@@ -1,0 +1,102 @@
+"""CPU: the C-ABI library loads and exports every symbol include/*.h declares,
+and the host-side logic mirrors the reference's (no GPU calls)."""
+import ctypes
+import math
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER_DIR = os.path.join(ROOT, "include")
+
+# symbols of src/cuda/bn254_c.rs:5-309 that are on the hot path
+REFERENCE_HOT_PATH = [
+    "ntt", "expand_omega_buffer", "extended_prepare", "msm", "batch_msm_collect", "logup_sum_input_inv",
+    "eval_logup_z_compose", "eval_logup_z_pure", "field_op_batch_mul_sum", "field_op", "field_mul_zip",
+    "permutation_eval_h_p1", "permutation_eval_h_p2", "permutation_eval_h_l", "logup_eval_h",
+    "logup_eval_h_extra_inputs", "logup_eval_h_z_set", "logup_eval_h_inputs_product_sum", "shuffle_eval_h",
+    "shuffle_eval_h_v2",
+]
+
+
+def declared_symbols():
+    names = []
+    for f in sorted(os.listdir(HEADER_DIR)):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(HEADER_DIR, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:hipError_t|void|size_t|int)\s+(\w+)\s*\(", text, flags=re.M)
+    return names
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from zkprover._lib import LIB_PATH
+    if not os.path.exists(LIB_PATH):
+        pytest.fail(f"kernel library not built: {LIB_PATH}")
+    return ctypes.CDLL(LIB_PATH)
+
+
+def test_header_declares_reference_hot_path():
+    decl = set(declared_symbols())
+    missing = [s for s in REFERENCE_HOT_PATH if s not in decl]
+    assert not missing, missing
+
+
+def test_library_exports_every_declared_symbol(lib):
+    decl = declared_symbols()
+    assert len(decl) >= len(REFERENCE_HOT_PATH)
+    missing = [s for s in decl if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_binding_table_matches_header():
+    from zkprover._lib import SIGNATURES
+    decl = set(declared_symbols())
+    assert set(SIGNATURES) <= decl, sorted(set(SIGNATURES) - decl)
+
+
+def _ref_policy(count, length):
+    """src/cuda/msm.rs:274-283 (ark_std::log2 is a ceiling)"""
+    def log2c(x):
+        return math.ceil(math.log2(x)) if x > 1 else 0
+    wb = min(log2c(count), 3) + 10 + max(log2c(length), 22) - 22
+    windows = (254 + wb - 1) // wb
+    workers = (length + 63) // 64 * 64 if length < 128 * 512 else 128 * 512
+    return wb, windows, workers
+
+
+@pytest.mark.parametrize("count,length", [(1, 1 << 16), (1, 1 << 22), (2, 1 << 22), (3, 1000), (8, 1 << 20),
+                                          (64, 1 << 12), (5, (1 << 23) + 1), (1, 1 << 24)])
+def test_msm_window_policy_matches_reference(lib, count, length):
+    wb, w, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib.zkp_msm_policy.restype = None
+    lib.zkp_msm_policy(count, length, ctypes.byref(wb), ctypes.byref(w), ctypes.byref(k))
+    assert (wb.value, w.value, k.value) == _ref_policy(count, length)
+
+
+def test_msm_scratch_and_window_queries(lib):
+    lib.zk_msm_scratch_bytes.restype = ctypes.c_size_t
+    lib.zk_msm_scratch_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.zk_msm_default_window.restype = ctypes.c_int
+    sizes = [lib.zk_msm_scratch_bytes(1 << k, 0) for k in range(4, 23)]
+    assert all(b > a for a, b in zip(sizes, sizes[1:]))
+    assert lib.zk_msm_scratch_bytes(0, 0) == 0
+    assert lib.zk_msm_default_window(1 << 22) == 17
+    assert all(2 <= lib.zk_msm_default_window(1 << k) <= 17 for k in range(0, 26))
+
+
+def test_invalid_arguments_rejected_without_gpu(lib):
+    """argument checks happen on the host before any launch"""
+    lib.ntt.restype = ctypes.c_int
+    assert lib.ntt(None, None, None, None, None, -1, 8, None, None) != 0
+    lib.field_op.restype = ctypes.c_int
+    assert lib.field_op(None, None, 0, None, None, 0, None, 64, 1, None) != 0
+    lib.zk_msm.restype = ctypes.c_int
+    lib.zk_msm.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                                     ctypes.c_void_p]
+    assert lib.zk_msm(None, None, None, 0, 0, None, 0, None) != 0
+    lib.eval_logup_z_pure.restype = ctypes.c_int
+    assert lib.eval_logup_z_pure(None, None, None, None, 5, 4, None) != 0

@@ -354,44 +354,63 @@ __device__ __forceinline__ G1Xyzz xyzz_add_entry(const G1Xyzz &acc, const G1Aff 
     return xyzz_add_aff(acc, q.x, y);
 }
 
-__global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restrict__ entries, const uint32_t *__restrict__ off,
-                                 uint32_t NB, const G1Aff *__restrict__ pts, G1Xyzz *__restrict__ buckets,
-                                 G1Xyzz *__restrict__ phead, G1Xyzz *__restrict__ ptail,
-                                 uint32_t *__restrict__ hb, uint32_t *__restrict__ tb, uint64_t nchunks) {
+__global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restrict__ entries,
+                                                        const uint32_t *__restrict__ off, uint32_t NB,
+                                                        const G1Aff *__restrict__ pts, G1Xyzz *__restrict__ buckets,
+                                                        G1Xyzz *__restrict__ phead, G1Xyzz *__restrict__ ptail,
+                                                        uint32_t *__restrict__ hb, uint32_t *__restrict__ tb,
+                                                        uint64_t nchunks) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nchunks) return;
     const uint32_t total = off[NB];
-    uint32_t s = (uint32_t)(t * kChunk);
+    const uint32_t s = (uint32_t)(t * kChunk);
     uint32_t head_bucket = kNone, tail_bucket = kNone;
     if (s < total) {
-        uint32_t e = s + kChunk < total ? s + kChunk : total;
+        const uint32_t e = s + kChunk < total ? s + kChunk : total;
         uint32_t b = find_bucket(off, NB, s);
-        uint32_t pos = s;
-        bool first = true;
-        while (pos < e) {
-            uint32_t bstart = off[b], bend = off[b + 1];
-            uint32_t runend = bend < e ? bend : e;
-            G1Xyzz acc = xyzz_id();
-            for (; pos < runend; pos++) acc = xyzz_add_entry(acc, pts, entries[pos]);
-            bool before = first && bstart < s;
-            bool after = bend > e;
-            if (before) {
-                xyzz_store(&phead[t], acc);
-                head_bucket = b;
-            } else if (after) {
-                xyzz_store(&ptail[t], acc);
-                tail_bucket = b;
-            } else {
-                xyzz_store(&buckets[b], acc);
+        uint32_t bend = off[b + 1];
+        bool first = true, before = off[b] < s;
+        G1Xyzz acc = xyzz_id();
+        // one uniform loop over the chunk (every lane runs e - s steps); the
+        // next entry and its point are loaded one step ahead
+        uint32_t ecur = entries[s];
+        G1Aff qcur = aff_load(&pts[ecur & 0x7fffffffu]);
+        for (uint32_t pos = s; pos < e; pos++) {
+            const bool more = pos + 1 < e;
+            const uint32_t enext = more ? entries[pos + 1] : 0u;
+            const G1Aff qnext = more ? aff_load(&pts[enext & 0x7fffffffu]) : qcur;
+            if (!aff_is_id(qcur)) {
+                Fp y = (ecur >> 31) ? fe_neg<FpParams>(qcur.y) : qcur.y;
+                acc = xyzz_add_aff(acc, qcur.x, y);
             }
-            first = false;
-            if (pos < e) {
-                // next non-empty bucket starting at pos
-                uint32_t nb = b + 1, steps = 0;
-                while (off[nb + 1] == pos && steps < 16) { nb++; steps++; }
-                if (off[nb + 1] == pos) nb = find_bucket(off, NB, pos);
-                b = nb;
+            if (pos + 1 == bend || !more) {
+                // the run of bucket b ends at this step
+                if (first && before) {
+                    xyzz_store(&phead[t], acc);
+                    head_bucket = b;
+                } else if (bend > e) {
+                    xyzz_store(&ptail[t], acc);
+                    tail_bucket = b;
+                } else {
+                    xyzz_store(&buckets[b], acc);
+                }
+                first = false;
+                acc = xyzz_id();
+                if (more) {
+                    // next non-empty bucket starting at pos + 1
+                    const uint32_t np = pos + 1;
+                    uint32_t nb = b + 1, steps = 0;
+                    while (off[nb + 1] == np && steps < 16) {
+                        nb++;
+                        steps++;
+                    }
+                    if (off[nb + 1] == np) nb = find_bucket(off, NB, np);
+                    b = nb;
+                    bend = off[b + 1];
+                }
             }
+            ecur = enext;
+            qcur = qnext;
         }
     }
     hb[t] = head_bucket;
