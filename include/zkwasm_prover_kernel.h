@@ -198,6 +198,7 @@ hipError_t zk_fr_op(void *out, const void *a, const void *b, int op, int n, hipS
 hipError_t zk_fp_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
 hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
 hipError_t zk_bench_mul(void *buf, int threads, int block, int iters, hipStream_t stream);
+hipError_t zk_bench_mul_cios(void *buf, int threads, int block, int iters, hipStream_t stream);
 hipError_t zk_bench_dbl(void *buf, int threads, int block, int iters, hipStream_t stream);
 hipError_t zk_bench_madd(void *acc, const void *pts, int npts, int threads, int block, int iters,
                          hipStream_t stream);

@@ -2,10 +2,12 @@
 // eight 32-bit limbs.  Byte layout equals halo2's Fr([u64; 4]) / Fq([u64; 4])
 // in Montgomery form, so device buffers are the caller's buffers unchanged.
 //
-// Multiplication is CIOS with every 32x32+64 step written as a 64-bit
-// multiply-add, which gfx950 executes as v_mad_u64_u32; carries propagate with
-// v_add_co/v_addc.  All results are fully reduced (< modulus), which is what
-// makes every output bit-comparable with the CPU restatement.
+// Multiplication (fe_mul) is Montgomery product scanning with the reduction
+// folded into the same columns (FIPS), each 32x32 product accumulated by one
+// v_mad_u64_u32 whose carry-out feeds v_addc_co_u32 (see mac96).  A row-wise
+// CIOS variant (fe_mul_cios) is kept for A/B measurement.  All results are
+// fully reduced (< modulus), which is what makes every output bit-comparable
+// with the CPU restatement.
 //
 // Reference semantics: cuda/zprize/ff_dispatch_st.cuh:192-217 (add/sub),
 // :428-529 (mul, to/from Montgomery), :655-686 (inverse, 0 -> 0),
@@ -135,7 +137,7 @@ __device__ __forceinline__ Fe<P> fe_neg(const Fe<P> &a) {
 // The inner step (uint64)a*b + t + carry fits 64 bits exactly:
 // (2^32-1)^2 + 2(2^32-1) = 2^64-1.
 template <class P>
-__device__ __forceinline__ Fe<P> fe_mul(const Fe<P> &a, const Fe<P> &b) {
+__device__ __forceinline__ Fe<P> fe_mul_cios(const Fe<P> &a, const Fe<P> &b) {
     uint32_t t[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) t[i] = 0;
@@ -164,6 +166,70 @@ __device__ __forceinline__ Fe<P> fe_mul(const Fe<P> &a, const Fe<P> &b) {
     for (int i = 0; i < 8; i++) r.v[i] = t[i];
     return fe_reduce_once<P>(r);
 }
+// (acc:t) += x * y as a 96-bit accumulator: acc is a 64-bit VGPR pair and t
+// the third word.  v_mad_u64_u32 writes its 64-bit carry-out to an SGPR lane
+// mask, which v_addc_co_u32 folds into t (two VALU instructions, no moves).
+__device__ __forceinline__ void mac96(uint64_t &acc, uint32_t &t, uint32_t x, uint32_t y) {
+    uint64_t cc;
+    asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+        "v_addc_co_u32_e64 %2, %1, %2, 0, %1"
+        : "+v"(acc), "=&s"(cc), "+v"(t)
+        : "v"(x), "v"(y));
+}
+// same with an SGPR second operand (the modulus words are uniform constants)
+__device__ __forceinline__ void mac96s(uint64_t &acc, uint32_t &t, uint32_t x, uint32_t y) {
+    uint64_t cc;
+    asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+        "v_addc_co_u32_e64 %2, %1, %2, 0, %1"
+        : "+v"(acc), "=&s"(cc), "+v"(t)
+        : "v"(x), "s"(y));
+}
+
+// Montgomery multiplication by product scanning with the reduction folded
+// into the same columns (FIPS): column i accumulates a_j*b_(i-j) and
+// m_j*M_(i-j) into a 96-bit accumulator; m_i is chosen so the column's low
+// word cancels, then the accumulator shifts down one word.  128 multiply-adds
+// with carries caught by the 96-bit accumulator replace the 64-bit-operand
+// shuffling of row-wise CIOS.  Inputs < M, output < M.
+template <class P>
+__device__ __forceinline__ Fe<P> fe_mul_fips(const Fe<P> &a, const Fe<P> &b) {
+    uint32_t m[8];
+    Fe<P> r;
+    uint64_t acc = 0;
+    uint32_t t = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+#pragma unroll
+        for (int j = 0; j < i; j++) {
+            mac96(acc, t, a.v[j], b.v[i - j]);
+            mac96s(acc, t, m[j], P::M[i - j]);
+        }
+        mac96(acc, t, a.v[i], b.v[0]);
+        m[i] = (uint32_t)acc * P::INV;
+        mac96s(acc, t, m[i], P::M[0]);
+        acc = (acc >> 32) | ((uint64_t)t << 32);
+        t = 0;
+    }
+#pragma unroll
+    for (int i = 8; i < 16; i++) {
+#pragma unroll
+        for (int j = i - 7; j < 8; j++) {
+            mac96(acc, t, a.v[j], b.v[i - j]);
+            mac96s(acc, t, m[j], P::M[i - j]);
+        }
+        r.v[i - 8] = (uint32_t)acc;
+        acc = (acc >> 32) | ((uint64_t)t << 32);
+        t = 0;
+    }
+    // the result is < 2M < 2^255, so the word above r.v[7] is zero
+    return fe_reduce_once<P>(r);
+}
+
+template <class P>
+__device__ __forceinline__ Fe<P> fe_mul(const Fe<P> &a, const Fe<P> &b) {
+    return fe_mul_fips<P>(a, b);
+}
+
 template <class P>
 __device__ __forceinline__ Fe<P> fe_sqr(const Fe<P> &a) { return fe_mul<P>(a, a); }
 

@@ -66,6 +66,16 @@ __global__ void k_bench_mul(Fr *buf, uint32_t iters) {
     }
     fe_store<FrParams>(&buf[i], x);
 }
+__global__ void k_bench_mul_cios(Fr *buf, uint32_t iters) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Fr x = fe_load<FrParams>(&buf[i]);
+    Fr y = fe_load<FrParams>(&buf[i ^ 1]);
+    for (uint32_t k = 0; k < iters; k++) {
+        x = fe_mul_cios<FrParams>(x, y);
+        y = fe_mul_cios<FrParams>(y, x);
+    }
+    fe_store<FrParams>(&buf[i], x);
+}
 __global__ void k_bench_dbl(G1Jac *buf, uint32_t iters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     G1Jac p = jac_load(&buf[i]);
@@ -172,6 +182,11 @@ hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipS
 hipError_t zk_bench_mul(void *buf, int threads, int block, int iters, hipStream_t stream) {
     if (threads % block) return hipErrorInvalidValue;
     k_bench_mul<<<threads / block, block, 0, stream>>>((Fr *)buf, iters);
+    return hipGetLastError();
+}
+hipError_t zk_bench_mul_cios(void *buf, int threads, int block, int iters, hipStream_t stream) {
+    if (threads % block) return hipErrorInvalidValue;
+    k_bench_mul_cios<<<threads / block, block, 0, stream>>>((Fr *)buf, iters);
     return hipGetLastError();
 }
 hipError_t zk_bench_dbl(void *buf, int threads, int block, int iters, hipStream_t stream) {
