@@ -161,7 +161,7 @@ hipError_t zk_batch_invert(void *a, void *temp, int n, hipStream_t stream);
  * device arrays of ncols column pointers; delta_start = DELTA^(set*chunk_len).
  * z[i] = exclusive running product of
  *   prod_j (v_j + beta*delta_start*DELTA^j*omega^i + gamma) / prod_j (v_j + beta*sigma_j + gamma).
- * temp: 2n Fr of scratch. */
+ * temp: n Fr of scratch. */
 hipError_t zk_permutation_z_set(void *z, const void *values, const void *sigmas, int ncols, const void *beta,
                                 const void *gamma, const void *delta_start, const void *delta,
                                 const void *omega, void *temp, int n, hipStream_t stream);
@@ -174,7 +174,7 @@ hipError_t zk_permutation_chain(void *zs, int nsets, int u, int n, void *tails, 
 /* Shuffle grand product (src/shuffle.rs:142-209): inputs/shuffles are device
  * arrays of group_len pointers, betas[j] = beta^(j+1); z = exclusive running
  * product of prod_j (input_j + betas[j]) / prod_j (shuffle_j + betas[j]).
- * temp: 2n Fr. */
+ * temp: n Fr. */
 hipError_t zk_shuffle_z(void *z, const void *inputs, const void *shuffles, int group_len, const void *betas,
                         void *temp, int n, hipStream_t stream);
 
