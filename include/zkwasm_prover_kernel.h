@@ -66,7 +66,11 @@ hipError_t msm(void *res, const void *points, const void *scalars, void *indices
 /* bn254_c.rs:248 `batch_msm_collect`.  buckets: device array of batch_size
  * pointers to the curve slices given to msm(); afterwards each slice starts
  * with the MSM result, a Jacobian point with z = 1 (or the identity
- * (0, 1, 0)).  The remain arrays are unused. */
+ * (0, 1, 0)).  The remain arrays are unused.  The final window combination
+ * runs on host cores and the call returns after the results are written
+ * (the reference's caller synchronises right after it, src/cuda/msm.rs:436-439);
+ * with ZK_MSM_TAIL=device in the environment it stays on the GPU and is
+ * asynchronous. */
 hipError_t batch_msm_collect(void *remain_indices, void *remain_acc, void *next_remain_indices,
                              void *next_remain_acc, void *buckets, unsigned workers, unsigned windows,
                              unsigned window_bits, unsigned batch_size, hipStream_t stream);
@@ -178,12 +182,68 @@ hipError_t zk_permutation_chain(void *zs, int nsets, int u, int n, void *tails, 
 hipError_t zk_shuffle_z(void *z, const void *inputs, const void *shuffles, int group_len, const void *betas,
                         void *temp, int n, hipStream_t stream);
 
+/* Batched NTT: count independent transforms (sizes may differ) with one
+ * launch per pass for the whole batch instead of one `ntt` call per
+ * polynomial (src/cuda/ntt.rs:61-93).  Each item is an `ntt` call's
+ * arguments; items is a HOST array; swapped[m] (host) is set when result m
+ * landed in tmp.  Asynchronous on stream. */
+typedef struct {
+    void *buf;
+    void *tmp;
+    const void *omegas;  /* expand_omega_buffer table of 2^log_n entries */
+    const void *n_inv;   /* null, or n^-1 (iNTT) */
+    int log_n;
+    int reserved;
+} zk_ntt_item;
+hipError_t zk_ntt_batch(const zk_ntt_item *items, int count, unsigned char *swapped, hipStream_t stream);
+
+/* Radix choice per size (autotuner hook): largest pass degree for 2^log_n
+ * transforms, 1..8 (0 = default 8). */
+hipError_t zk_ntt_set_max_deg(int log_n, int max_deg);
+int zk_ntt_get_max_deg(int log_n);
+/* Autotuner: times each distinct radix plan (largest pass degree 4..8) for
+ * 2^log_n on the given buffers (overwritten) and keeps the fastest; ms
+ * (optional, 9 floats) gets the time per max degree. */
+hipError_t zk_ntt_autotune(void *buf, void *tmp, const void *omegas, int log_n, int reps, hipStream_t stream,
+                           int *best, float *ms);
+
+
 /* MSM with caller-owned scratch of zk_msm_scratch_bytes(n, c) bytes; the
- * result (Jacobian, z = 1) lands in out[0..96); out must hold 96 * 130 bytes.
- * c = 0 picks the default window. */
+ * result (Jacobian, z = 1) lands in out[0..96); out must hold
+ * ZK_MSM_OUT_BYTES bytes.  c = 0 picks the default window.  Synchronous. */
+#define ZK_MSM_OUT_BYTES (64 * 1024)
 size_t zk_msm_scratch_bytes(int n, int c);
 hipError_t zk_msm(void *out, const void *points, const void *scalars, int n, int c, void *scratch,
                   size_t scratch_bytes, hipStream_t stream);
+
+/* Batched MSM: count independent MSMs (device point / scalar arrays, any
+ * sizes; items may share points) through one pass of the pipeline, one
+ * launch per phase for the whole batch.  Replaces the per-polynomial loop of
+ * the reference's batch_msm driver (src/cuda/msm.rs:369-455), whose MSMs go
+ * one by one over two streams.  items is a HOST array; result m (Jacobian,
+ * z = 1) lands in out + m * ZK_MSM_OUT_BYTES.  Synchronous. */
+typedef struct {
+    const void *points;
+    const void *scalars;
+    int n;
+    int reserved;
+} zk_msm_item;
+size_t zk_msm_batch_scratch_bytes(const zk_msm_item *items, int count, int c);
+hipError_t zk_msm_batch(void *out, const zk_msm_item *items, int count, int c, void *scratch,
+                        size_t scratch_bytes, hipStream_t stream);
+/* The same without host waits: the window combination runs on the device,
+ * so the call only enqueues kernels (capturable into a HIP graph). */
+hipError_t zk_msm_batch_async(void *out, const zk_msm_item *items, int count, int c, void *scratch,
+                              size_t scratch_bytes, hipStream_t stream);
+
+/* HIP graph capture / replay of a launch sequence (no reference counterpart:
+ * the reference re-issues every launch per proof, src/eval_h.rs, src/lib.rs).
+ * Between begin and end only asynchronous entries may be called on stream
+ * (not the legacy default stream); exec is an opaque executable graph. */
+hipError_t zk_graph_begin(hipStream_t stream);
+hipError_t zk_graph_end(hipStream_t stream, void **exec);
+hipError_t zk_graph_launch(void *exec, hipStream_t stream);
+hipError_t zk_graph_destroy(void *exec);
 
 /* MSM phase timing for the bench: HIP events on the launch stream around each
  * phase (count, scan, scatter, accumulate, fixup, reduce, finalize); reading
@@ -191,6 +251,18 @@ hipError_t zk_msm(void *out, const void *points, const void *scalars, int n, int
 void zk_msm_profile_enable(int on);
 int zk_msm_profile_read(double *times, int ntimes, int reset);
 int zk_msm_default_window(int n);
+/* Window table per size class ceil(log2 n) (0 = model default), and the
+ * autotuner that fills it: times zk_msm for windows around the model's choice
+ * on the given device points / scalars and keeps the fastest for every later
+ * MSM of that class (msm, zk_msm, zk_msm_batch).  ms: optional 18 floats. */
+/* The window msm() picks with the reference driver's buffers (sizes of
+ * src/cuda/msm.rs:284-311) for array_len points, 0 if they cannot hold its
+ * scratch.  Host-only. */
+int zk_msm_plan_window(int array_len, int windows, int window_bits, int workers);
+hipError_t zk_msm_set_window(int log_n, int c);
+int zk_msm_get_window(int log_n);
+hipError_t zk_msm_autotune(const void *points, const void *scalars, int n, int reps, hipStream_t stream, int *best,
+                           float *ms);
 
 /* Element-wise field / curve operations and calibration kernels (tests and
  * bench only). */
@@ -214,6 +286,21 @@ void zkp_msm_policy(int count, int len, int *window_bits, int *windows, int *wor
  * out_affine: count affine points in host memory. */
 hipError_t zkp_batch_msm(void *out_affine, const void *points_dev, const void *const *scalars_dev, int count,
                          int len, int skip_zero);
+
+/* ---- host-side mirror of the vanishing construction ------------------- */
+/* src/eval_h.rs:464-540: h (2^log_ext Fr, device) *= t_evals[i mod t_len];
+ * extended iNTT (ext_intt_omegas, ext_divisor); coset un-distribution with
+ * coset_powers = [g_coset_inv, g_coset]; one commitment per quotient piece of
+ * 2^log_n coefficients against g (device, 2^log_n affine).  commitments
+ * (host): quotient_degree affine points.  *coeffs = h or tmp, whichever holds
+ * the coefficients.  batched = 0: the reference driver's MSMs (zkp_batch_msm);
+ * 1: one zk_msm_batch pass. */
+hipError_t zkp_vanishing_commit(void *h, void *tmp, const void *t_evals, int t_len, const void *ext_intt_omegas,
+                                const void *ext_divisor, const void *coset_powers, int log_n, int log_ext,
+                                int quotient_degree, const void *g, void *commitments, void **coeffs, int batched);
+/* src/eval_h.rs:546-583: h_pieces (host, 2^log_n Fr) = sum_i piece_i xn^i by
+ * Horner, folded in place into the last piece of coeffs; xn: device x^n. */
+hipError_t zkp_vanishing_pieces(void *coeffs, const void *xn, int log_n, int quotient_degree, void *h_pieces);
 
 #ifdef __cplusplus
 }

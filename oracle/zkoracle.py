@@ -453,3 +453,22 @@ def shuffle_eval_h_v2(res, inputs, tables, betas, z, l0, l_last, l_active_row, y
                                ctypes.c_int(len(inputs)), _p(z), _p(l0), _p(l_last),
                                _p(l_active_row), _p(y), ctypes.c_int(rot), ctypes.c_size_t(len(res)))
     return res
+
+
+# ---- vanishing construction (composition of the primitives above) ----------
+def vanishing_construct(h, t_evals, log_n, log_ext, ext_omega_inv, ext_divisor, coset_powers, g, q, xn):
+    """src/eval_h.rs:464-583: h *= t_evals[i mod |t|]; extended iNTT with the
+    divisor; coset un-distribution (powers [g_coset_inv, g_coset], whole
+    extended domain); commitments of the q pieces of n against g; h_pieces by
+    Horner in xn.  Returns (coeffs (N), commitments (q affine), h_pieces (n))."""
+    n = 1 << log_n
+    N = 1 << log_ext
+    x = field_mul_zip(h, t_evals)
+    x = ntt(x, log_ext, ext_omega_inv, ext_divisor)
+    x = extended_prepare(x, coset_powers, n, N, 1)
+    commits = np.concatenate([g1_to_affine(msm(g, np.ascontiguousarray(x[i * n:(i + 1) * n])))
+                              for i in range(q)])
+    last = np.array(x[(q - 1) * n:q * n], copy=True)
+    for i in range(q - 2, -1, -1):
+        last = field_op(last, 0, xn, np.ascontiguousarray(x[i * n:(i + 1) * n]), 0, None, n, 0)
+    return x, commits, last

@@ -88,6 +88,37 @@ def test_msm_scratch_and_window_queries(lib):
     assert all(2 <= lib.zk_msm_default_window(1 << k) <= 17 for k in range(0, 26))
 
 
+class _Item(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("scalars", ctypes.c_void_p), ("n", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
+def test_msm_batch_scratch_query(lib):
+    """batch sizing: one item sizes like zk_msm; more items need more; a
+    non-positive size or count is rejected with 0"""
+    f = lib.zk_msm_batch_scratch_bytes
+    f.restype = ctypes.c_size_t
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    lib.zk_msm_scratch_bytes.restype = ctypes.c_size_t
+    lib.zk_msm_scratch_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+    for n in (1, 1000, 1 << 16, 1 << 20):
+        one = (_Item * 1)(_Item(None, None, n, 0))
+        for c in (0, 8, 13):
+            assert f(ctypes.addressof(one), 1, c) == lib.zk_msm_scratch_bytes(n, c)
+    two = (_Item * 2)(_Item(None, None, 1 << 16, 0), _Item(None, None, 1 << 16, 0))
+    one = (_Item * 1)(_Item(None, None, 1 << 16, 0))
+    assert f(ctypes.addressof(two), 2, 12) > f(ctypes.addressof(one), 1, 12)
+    bad = (_Item * 2)(_Item(None, None, 1 << 16, 0), _Item(None, None, 0, 0))
+    assert f(ctypes.addressof(bad), 2, 0) == 0
+    assert f(ctypes.addressof(one), 0, 0) == 0
+    lib.zk_msm_batch.restype = ctypes.c_int
+    lib.zk_msm_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.c_size_t, ctypes.c_void_p]
+    assert lib.zk_msm_batch(None, ctypes.addressof(one), 1, 0, None, 0, None) != 0
+    assert lib.zk_msm_batch(ctypes.addressof(one), ctypes.addressof(bad), 2, 0, ctypes.addressof(one), 16,
+                            None) != 0
+
+
 def test_invalid_arguments_rejected_without_gpu(lib):
     """argument checks happen on the host before any launch"""
     lib.ntt.restype = ctypes.c_int
@@ -100,3 +131,54 @@ def test_invalid_arguments_rejected_without_gpu(lib):
     assert lib.zk_msm(None, None, None, 0, 0, None, 0, None) != 0
     lib.eval_logup_z_pure.restype = ctypes.c_int
     assert lib.eval_logup_z_pure(None, None, None, None, 5, 4, None) != 0
+
+
+def test_tuning_tables_host_only(lib, tmp_path):
+    """the per-size window / radix tables are host state: set, read back,
+    reset, reject out-of-range values; autotune.save/load round-trips them"""
+    lib.zk_msm_set_window.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.zk_msm_get_window.restype = ctypes.c_int
+    lib.zk_ntt_set_max_deg.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.zk_ntt_get_max_deg.restype = ctypes.c_int
+    lib.zk_msm_default_window.restype = ctypes.c_int
+    default12 = lib.zk_msm_get_window(12)
+    assert default12 == lib.zk_msm_default_window(1 << 12)
+    assert lib.zk_msm_set_window(12, 9) == 0 and lib.zk_msm_get_window(12) == 9
+    assert lib.zk_msm_default_window(4000) == 9  # ceil(log2 4000) = 12
+    assert lib.zk_msm_set_window(12, 0) == 0 and lib.zk_msm_get_window(12) == default12
+    assert lib.zk_msm_set_window(12, 18) != 0 and lib.zk_msm_set_window(40, 8) != 0
+    assert lib.zk_ntt_get_max_deg(20) == 8
+    assert lib.zk_ntt_set_max_deg(20, 6) == 0 and lib.zk_ntt_get_max_deg(20) == 6
+    assert lib.zk_ntt_set_max_deg(20, 0) == 0 and lib.zk_ntt_get_max_deg(20) == 8
+    assert lib.zk_ntt_set_max_deg(20, 9) != 0
+    from zkprover import autotune
+    path = str(tmp_path / "tune.json")
+    autotune.save(path, msm={14: (10, {}), 16: (12, {})}, ntt={18: (7, {})})
+    autotune.load(path)
+    try:
+        assert lib.zk_msm_get_window(14) == 10 and lib.zk_msm_get_window(16) == 12
+        assert lib.zk_ntt_get_max_deg(18) == 7
+        t = autotune.table()
+        assert t["msm_window"][14] == 10 and t["ntt_max_deg"][18] == 7
+    finally:
+        lib.zk_msm_set_window(14, 0)
+        lib.zk_msm_set_window(16, 0)
+        lib.zk_ntt_set_max_deg(18, 0)
+
+
+@pytest.mark.parametrize("count", [1, 2, 3, 8])
+def test_msm_fits_reference_buffers(lib, count):
+    """for every size the reference driver may pass, msm() finds a window
+    whose scratch fits the index / sort buffers and the curve slice it gets
+    (src/cuda/msm.rs:274-311 sizes); at 2^22 it keeps the default window"""
+    wb, w, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib.zkp_msm_policy.restype = None
+    lib.zk_msm_plan_window.restype = ctypes.c_int
+    lib.zk_msm_default_window.restype = ctypes.c_int
+    for n in [1, 2, 3, 63, 64, 65, 100, 255, 1000, 4095, 4096, 4097, 1 << 12, 1 << 14, 12345, 1 << 16,
+              1 << 18, (1 << 20) + 7, 1 << 22, (1 << 23) + 1, 1 << 24]:
+        lib.zkp_msm_policy(count, n, ctypes.byref(wb), ctypes.byref(w), ctypes.byref(k))
+        c = lib.zk_msm_plan_window(n, w.value, wb.value, k.value)
+        assert 2 <= c <= 17, (n, count)
+        if n >= 1 << 12:
+            assert c >= lib.zk_msm_default_window(n), (n, count, c)

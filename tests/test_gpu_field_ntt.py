@@ -165,3 +165,83 @@ def test_coset_extend_matches_oracle(gpu, zo, log_n, ext):
          dv.stream_handle())
     got2 = dv.from_dev(tmp if swap.value else s)
     assert (got2 == zo.extended_prepare(got, cp, n, N, 1)).all()
+
+
+
+def _ntt_batch(zo, specs):
+    """zk_ntt_batch over [(x, log_n, omega, n_inv or None)]; returns outputs"""
+    import ctypes
+    from zkprover import device as dv
+    from zkprover._lib import call, lib
+
+    class Item(ctypes.Structure):
+        _fields_ = [("buf", ctypes.c_void_p), ("tmp", ctypes.c_void_p), ("omegas", ctypes.c_void_p),
+                    ("n_inv", ctypes.c_void_p), ("log_n", ctypes.c_int), ("reserved", ctypes.c_int)]
+    keep, bufs = [], []
+    items = (Item * len(specs))()
+    for m, (x, log_n, omega, n_inv) in enumerate(specs):
+        om = _omegas_dev(zo, log_n, omega)
+        b, t = dv.to_dev(x), dv.empty(len(x))
+        ni = dv.to_dev(n_inv) if n_inv is not None else None
+        keep += [om, b, t, ni]
+        bufs.append((b, t))
+        items[m] = Item(dv.ptr(b), dv.ptr(t), dv.ptr(om), dv.ptr(ni) if ni is not None else None, log_n, 0)
+    swapped = (ctypes.c_ubyte * len(specs))()
+    lib().zk_ntt_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    call("zk_ntt_batch", ctypes.addressof(items), len(specs), ctypes.addressof(swapped), dv.stream_handle())
+    return [dv.from_dev(t if swapped[m] else b) for m, (b, t) in enumerate(bufs)]
+
+
+def test_ntt_batch_mixed_sizes(gpu, zo):
+    """NTTs and iNTTs of sizes 2^0..2^17 in one batch (one launch per pass
+    round); each equals the oracle's transform of its input"""
+    logs = [0, 1, 3, 9, 10, 11, 14, 16, 17, 12, 5]
+    specs, exps = [], []
+    for k, log_n in enumerate(logs):
+        n = 1 << log_n
+        x = zo.rand_fr(n, 300 + k)
+        w = zo.root_of_unity(log_n)
+        if k % 2:
+            w_inv = zo.fr_inv(w)
+            n_inv = zo.fr_inv(zo.fr_from_u64(n))
+            specs.append((x, log_n, w_inv, n_inv))
+            exps.append(zo.ntt(x, log_n, w_inv, n_inv))
+        else:
+            specs.append((x, log_n, w, None))
+            exps.append(zo.ntt(x, log_n, w))
+    got = _ntt_batch(zo, specs)
+    for k in range(len(logs)):
+        assert (got[k] == exps[k]).all(), logs[k]
+
+
+def test_ntt_batch_many_items(gpu, zo):
+    """more transforms than one launch carries (48): the pass rounds split
+    over several launches"""
+    specs, exps = [], []
+    for k in range(100):
+        log_n = 6 + (k % 7)
+        x = zo.rand_fr(1 << log_n, 500 + k)
+        w = zo.root_of_unity(log_n)
+        specs.append((x, log_n, w, None))
+        exps.append(zo.ntt(x, log_n, w))
+    got = _ntt_batch(zo, specs)
+    for k in range(100):
+        assert (got[k] == exps[k]).all(), k
+
+
+@pytest.mark.parametrize("max_deg", [1, 3, 5, 6, 7, 8])
+def test_ntt_radix_choice_same_result(gpu, zo, max_deg):
+    """every radix plan the autotuner may pick gives the same transform"""
+    import ctypes
+    from zkprover._lib import lib
+    log_n = 13
+    x = zo.rand_fr(1 << log_n, 77)
+    w = zo.root_of_unity(log_n)
+    L = lib()
+    L.zk_ntt_set_max_deg.argtypes = [ctypes.c_int, ctypes.c_int]
+    assert L.zk_ntt_set_max_deg(log_n, max_deg) == 0
+    try:
+        got = _gpu_ntt(zo, x, log_n, w)
+    finally:
+        L.zk_ntt_set_max_deg(log_n, 0)
+    assert (got == zo.ntt(x, log_n, w)).all()

@@ -1,4 +1,4 @@
-"""GPU parity: BN254 G1 MSM (msm + batch_msm_collect drop-in, zk_msm).
+"""GPU parity: BN254 G1 MSM (msm + batch_msm_collect drop-in, zk_msm, zk_msm_batch).
 
 Pinning: the CPU oracle's MSM is itself checked against the naive sum of
 scalar multiples and against the reference's known-answer construction
@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
 
-OUT_BYTES = 96 * 130
+OUT_BYTES = 64 * 1024
 
 
 def _zk_msm(points, scalars, c=0):
@@ -37,6 +37,38 @@ def _zk_msm(points, scalars, c=0):
          dv.stream_handle())
     jac = out[:96].cpu().numpy().view(np.uint64).reshape(1, 12).copy()
     return jac
+
+
+class _Item(ctypes.Structure):
+    _fields_ = [("points", ctypes.c_void_p), ("scalars", ctypes.c_void_p), ("n", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
+def _zk_msm_batch(point_sets, scalar_sets, c=0):
+    """zk_msm_batch over (points, scalars) pairs; returns (count, 12) Jacobian"""
+    import torch
+    from zkprover import device as dv
+    from zkprover._lib import call, lib
+    count = len(scalar_sets)
+    keep = []
+    items = (_Item * count)()
+    for m, (p, s) in enumerate(zip(point_sets, scalar_sets)):
+        dp, ds = dv.to_dev(p.reshape(-1, 4)), dv.to_dev(s)
+        keep += [dp, ds]
+        items[m] = _Item(dp.data_ptr(), ds.data_ptr(), len(s), 0)
+    need = lib().zk_msm_batch_scratch_bytes
+    need.restype = ctypes.c_size_t
+    need.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    nbytes = need(ctypes.addressof(items), count, c)
+    assert nbytes > 0
+    scratch = torch.empty(nbytes, dtype=torch.uint8, device=dv.device())
+    out = torch.zeros(count * OUT_BYTES, dtype=torch.uint8, device=dv.device())
+    lib().zk_msm_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_size_t, ctypes.c_void_p]
+    call("zk_msm_batch", out.data_ptr(), ctypes.addressof(items), count, c, scratch.data_ptr(), nbytes,
+         dv.stream_handle())
+    o = out.view(count, OUT_BYTES)[:, :96].cpu().numpy().copy()
+    return o.view(np.uint64).reshape(count, 12)
 
 
 def _affine_of(zo, jac):
@@ -122,3 +154,65 @@ def test_reference_known_answer(gpu, zo, log_n, batch):
         k = zo.fr_inner_cycled(sets[b], cand)
         exp = zo.g1_to_affine(zo.g1_mul(g, k))
         assert (got[b:b + 1] == exp).all(), b
+
+
+@pytest.mark.parametrize("n,batch", [(1000, 1), (1 << 14, 3)])
+def test_device_and_host_tails_agree(gpu, zo, monkeypatch, n, batch):
+    """the window combination on host cores (default) and on the GPU
+    (ZK_MSM_TAIL=device) give the same canonical point"""
+    pts = zo.gen_bases_ap(n, 61)
+    sets = [zo.rand_fr(n, 62 + b) for b in range(batch)]
+    exp = [zo.g1_to_affine(zo.msm(pts, s)) for s in sets]
+    host = _batch_msm(pts, sets)
+    monkeypatch.setenv("ZK_MSM_TAIL", "device")
+    dev = _batch_msm(pts, sets)
+    one = _affine_of(zo, _zk_msm(pts, sets[0]))
+    monkeypatch.delenv("ZK_MSM_TAIL")
+    for b in range(batch):
+        assert (host[b:b + 1] == exp[b]).all()
+        assert (dev[b:b + 1] == exp[b]).all()
+    assert (one == exp[0]).all()
+
+
+@pytest.mark.parametrize("c", [0, 5, 9, 13])
+def test_msm_batch_ragged_matches_oracle(gpu, zo, c):
+    """a batch of MSMs of unequal sizes (1 point, a partial sort tile, several
+    tiles), one with all-zero scalars and two sharing their points, through
+    one pass of the pipeline; each result equals the oracle's MSM"""
+    sizes = [1000, 1, 4097, 1 << 14, 300, 1 << 14]
+    pts = [zo.gen_bases_random(n, 70 + k) for k, n in enumerate(sizes)]
+    pts[5] = pts[3]
+    sc = [zo.rand_fr(n, 80 + k) for k, n in enumerate(sizes)]
+    sc[4] = np.zeros_like(sc[4])
+    got = zo.g1_to_affine(_zk_msm_batch(pts, sc, c))
+    for m in range(len(sizes)):
+        exp = zo.g1_to_affine(zo.msm(pts[m], sc[m]))
+        assert (got[m:m + 1] == exp).all(), (m, sizes[m])
+    assert (got[4] == 0).all()
+
+
+def test_msm_batch_known_answer(gpu, zo):
+    """the reference's known-answer construction for 6 MSMs of 2^18 points
+    sharing the bases, in one batched pass"""
+    n, batch = 1 << 18, 6
+    cand, g, pts = _kat_inputs(zo, n, 4343)
+    sets = [zo.rand_fr(n, 6100 + b) for b in range(batch)]
+    got = zo.g1_to_affine(_zk_msm_batch([pts] * batch, sets))
+    for b in range(batch):
+        exp = zo.g1_to_affine(zo.g1_mul(g, zo.fr_inner_cycled(sets[b], cand)))
+        assert (got[b:b + 1] == exp).all(), b
+
+
+def test_msm_batch_equals_single_calls(gpu, zo, monkeypatch):
+    """batched results equal zk_msm one by one, with the host and the device tail"""
+    sizes = [1 << 12, 3000, 1 << 13]
+    pts = [zo.gen_bases_ap(n, 90 + k) for k, n in enumerate(sizes)]
+    sc = [zo.rand_fr(n, 95 + k) for k, n in enumerate(sizes)]
+    one = [zo.g1_to_affine(_zk_msm(p, s, 10)) for p, s in zip(pts, sc)]
+    host = zo.g1_to_affine(_zk_msm_batch(pts, sc, 10))
+    monkeypatch.setenv("ZK_MSM_TAIL", "device")
+    dev = zo.g1_to_affine(_zk_msm_batch(pts, sc, 10))
+    monkeypatch.delenv("ZK_MSM_TAIL")
+    for m in range(len(sizes)):
+        assert (host[m:m + 1] == one[m]).all()
+        assert (dev[m:m + 1] == one[m]).all()

@@ -1,0 +1,165 @@
+// Host-side BN254 Fp / G1 Jacobian arithmetic for the serial MSM tail
+// (window Horner).  Same byte layout as the device types: four
+// little-endian u64 limbs in Montgomery form (R = 2^256).
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+namespace zkh {
+
+typedef unsigned __int128 u128;
+
+struct Fp {
+    uint64_t l[4];
+};
+struct Jac {
+    Fp x, y, z;
+};
+
+constexpr uint64_t kP[4] = {0x3c208c16d87cfd47ULL, 0x97816a916871ca8dULL, 0xb85045b68181585dULL,
+                            0x30644e72e131a029ULL};
+constexpr uint64_t kPInv = 0x87d20782e4866389ULL;  // -p^-1 mod 2^64
+constexpr uint64_t kOne[4] = {0xd35d438dc58f0d9dULL, 0x0a78eb28f5c70b3dULL, 0x666ea36f7879462cULL,
+                              0x0e0a77c19a07df2fULL};  // R mod p
+
+inline bool geq_p(const uint64_t a[4]) {
+    for (int i = 3; i >= 0; i--) {
+        if (a[i] > kP[i]) return true;
+        if (a[i] < kP[i]) return false;
+    }
+    return true;
+}
+inline void sub_p(uint64_t a[4]) {
+    u128 br = 0;
+    for (int i = 0; i < 4; i++) {
+        u128 t = (u128)a[i] - kP[i] - (uint64_t)br;
+        a[i] = (uint64_t)t;
+        br = (t >> 64) ? 1 : 0;
+    }
+}
+inline Fp fp_zero() { return Fp{{0, 0, 0, 0}}; }
+inline Fp fp_one() { return Fp{{kOne[0], kOne[1], kOne[2], kOne[3]}}; }
+inline bool fp_is_zero(const Fp &a) { return !(a.l[0] | a.l[1] | a.l[2] | a.l[3]); }
+inline bool fp_eq(const Fp &a, const Fp &b) { return memcmp(&a, &b, sizeof(Fp)) == 0; }
+
+inline Fp fp_add(const Fp &a, const Fp &b) {
+    Fp r;
+    u128 c = 0;
+    for (int i = 0; i < 4; i++) {
+        c += (u128)a.l[i] + b.l[i];
+        r.l[i] = (uint64_t)c;
+        c >>= 64;
+    }
+    if (geq_p(r.l)) sub_p(r.l);
+    return r;
+}
+inline Fp fp_sub(const Fp &a, const Fp &b) {
+    Fp r;
+    u128 br = 0;
+    for (int i = 0; i < 4; i++) {
+        u128 t = (u128)a.l[i] - b.l[i] - (uint64_t)br;
+        r.l[i] = (uint64_t)t;
+        br = (t >> 64) ? 1 : 0;
+    }
+    if (br) {
+        u128 c = 0;
+        for (int i = 0; i < 4; i++) {
+            c += (u128)r.l[i] + kP[i];
+            r.l[i] = (uint64_t)c;
+            c >>= 64;
+        }
+    }
+    return r;
+}
+// CIOS Montgomery product on 64-bit limbs
+inline Fp fp_mul(const Fp &a, const Fp &b) {
+    uint64_t t[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+        u128 c = 0;
+        for (int j = 0; j < 4; j++) {
+            c = (u128)a.l[j] * b.l[i] + t[j] + (uint64_t)(c >> 64);
+            t[j] = (uint64_t)c;
+        }
+        c = (u128)t[4] + (uint64_t)(c >> 64);
+        t[4] = (uint64_t)c;
+        t[5] = (uint64_t)(c >> 64);
+        uint64_t m = t[0] * kPInv;
+        c = (u128)m * kP[0] + t[0];
+        for (int j = 1; j < 4; j++) {
+            c = (u128)m * kP[j] + t[j] + (uint64_t)(c >> 64);
+            t[j - 1] = (uint64_t)c;
+        }
+        c = (u128)t[4] + (uint64_t)(c >> 64);
+        t[3] = (uint64_t)c;
+        t[4] = t[5] + (uint64_t)(c >> 64);
+    }
+    Fp r{{t[0], t[1], t[2], t[3]}};
+    if (t[4] || geq_p(r.l)) sub_p(r.l);
+    return r;
+}
+inline Fp fp_sqr(const Fp &a) { return fp_mul(a, a); }
+inline Fp fp_dbl(const Fp &a) { return fp_add(a, a); }
+// a^(p-2) (Fermat); 0 -> 0
+inline Fp fp_inv(const Fp &a) {
+    uint64_t e[4] = {kP[0] - 2, kP[1], kP[2], kP[3]};
+    Fp acc = fp_one(), base = a;
+    for (int i = 0; i < 4; i++)
+        for (int b = 0; b < 64; b++) {
+            if ((e[i] >> b) & 1) acc = fp_mul(acc, base);
+            base = fp_sqr(base);
+        }
+    return acc;
+}
+
+inline Jac jac_id() { return Jac{fp_zero(), fp_one(), fp_zero()}; }
+
+// dbl-2009-l (a = 0)
+inline Jac jac_dbl(const Jac &p) {
+    if (fp_is_zero(p.z)) return p;
+    Fp A = fp_sqr(p.x), B = fp_sqr(p.y), C = fp_sqr(B);
+    Fp D = fp_dbl(fp_sub(fp_sub(fp_sqr(fp_add(p.x, B)), A), C));
+    Fp E = fp_add(fp_dbl(A), A);
+    Fp F = fp_sqr(E);
+    Jac r;
+    r.x = fp_sub(F, fp_dbl(D));
+    r.y = fp_sub(fp_mul(E, fp_sub(D, r.x)), fp_dbl(fp_dbl(fp_dbl(C))));
+    r.z = fp_dbl(fp_mul(p.y, p.z));
+    return r;
+}
+// add-2007-bl with the identity / equal / opposite cases
+inline Jac jac_add(const Jac &p, const Jac &q) {
+    if (fp_is_zero(p.z)) return q;
+    if (fp_is_zero(q.z)) return p;
+    Fp Z1Z1 = fp_sqr(p.z), Z2Z2 = fp_sqr(q.z);
+    Fp U1 = fp_mul(p.x, Z2Z2), U2 = fp_mul(q.x, Z1Z1);
+    Fp S1 = fp_mul(fp_mul(p.y, q.z), Z2Z2), S2 = fp_mul(fp_mul(q.y, p.z), Z1Z1);
+    if (fp_eq(U1, U2)) return fp_eq(S1, S2) ? jac_dbl(p) : jac_id();
+    Fp H = fp_sub(U2, U1);
+    Fp I = fp_sqr(fp_dbl(H));
+    Fp J = fp_mul(H, I);
+    Fp rr = fp_dbl(fp_sub(S2, S1));
+    Fp V = fp_mul(U1, I);
+    Jac r;
+    r.x = fp_sub(fp_sub(fp_sqr(rr), J), fp_dbl(V));
+    r.y = fp_sub(fp_mul(rr, fp_sub(V, r.x)), fp_dbl(fp_mul(S1, J)));
+    r.z = fp_mul(fp_sub(fp_sub(fp_sqr(fp_add(p.z, q.z)), Z1Z1), Z2Z2), H);
+    return r;
+}
+// XYZZ (x = X/ZZ, y = Y/ZZZ) -> Jacobian: Z = ZZ*ZZZ, X = X*ZZ*ZZZ^2, Y = Y*ZZZ^4
+inline Jac jac_from_xyzz(const Fp &X, const Fp &Y, const Fp &ZZ, const Fp &ZZZ) {
+    if (fp_is_zero(ZZ)) return jac_id();
+    Fp zzz2 = fp_sqr(ZZZ);
+    Jac r;
+    r.z = fp_mul(ZZ, ZZZ);
+    r.x = fp_mul(fp_mul(X, ZZ), zzz2);
+    r.y = fp_mul(Y, fp_sqr(zzz2));
+    return r;
+}
+// z = 1 form; the identity as (0, 1, 0)
+inline Jac jac_normalize(const Jac &p) {
+    if (fp_is_zero(p.z)) return jac_id();
+    Fp zi = fp_inv(p.z), zi2 = fp_sqr(zi);
+    return Jac{fp_mul(p.x, zi2), fp_mul(fp_mul(p.y, zi2), zi), fp_one()};
+}
+
+}  // namespace zkh

@@ -5,31 +5,38 @@
 // curve slice receives the MSM result as a Jacobian point in its first 96
 // bytes after batch_msm_collect.
 //
-// Pipeline (one MSM):
+// Pipeline (a batch of MSMs goes through each phase together; buckets are
+// numbered (item, window, bucket)):
 //   1. digits   one thread per scalar: Montgomery -> canonical, signed c-bit
 //               digits d_w in (-2^(c-1), 2^(c-1)], written window-major.
 //               Zero digits add nothing.
 //   2. sort     two-level counting sort of (bucket, point index | sign):
 //               per-tile LDS histograms of the high 8 bucket bits, one global
-//               scan, scatter into (window, high bits) groups with LDS ranks,
-//               then an LDS counting sort of each group on the low 8 bits that
-//               also emits the bucket offsets.  Writes land in short
-//               contiguous runs instead of 4-byte random scatters.
-//   4. accumulate  threads own equal chunks of the sorted entries (perfect
+//               scan, scatter into (item, window, high bits) groups with LDS
+//               ranks, then an LDS counting sort of each group on the low 8
+//               bits that also emits the bucket offsets.  Writes land in
+//               short contiguous runs instead of 4-byte random scatters.
+//   3. accumulate  threads own equal chunks of the sorted entries (perfect
 //               load balance for any digit distribution); runs of one bucket
 //               are summed with XYZZ mixed additions; runs cut by a chunk
 //               border are left as head/tail partials.
-//   5. fixup    partials of buckets that span chunks are merged.
-//   6. reduce   S_w = sum_b (b+1) B_{w,b}: per segment of L1 buckets a
+//   4. fixup    partials of buckets that span chunks are merged.
+//   5. reduce   S_w = sum_b (b+1) B_{w,b}: per segment of L1 buckets a
 //               running sum gives (WS, SUM); then
 //               S_w = sum_s WS_s + L1 * sum_bit 2^bit * (sum_{s has bit} SUM_s),
 //               the masked sums being parallel tree reductions.
-//   7. collect  sum_w 2^(c w) S_w by Horner over windows, normalised to z = 1
-//               so the bytes are canonical.
+//   6. collect  sum_w 2^(c w) S_w by Horner over windows, normalised to z = 1
+//               so the bytes are canonical (host cores by default).
 // The bucket semantics restate cuda/msm.cu:9-181 (digit d of window w adds
 // P_i to bucket (w, d)); signed digits halve the buckets per window.
+#include <stdlib.h>
 #include <string.h>
 
+#include <chrono>
+#include <thread>
+#include <vector>
+
+#include "host_ec.h"
 #include "zkcore.h"
 
 using namespace zk;
@@ -39,50 +46,74 @@ namespace {
 constexpr uint32_t kL1 = 8;          // buckets per first-level reduction segment
 constexpr uint32_t kChunk = 48;      // sorted entries per accumulation thread
 constexpr uint32_t kNone = 0xffffffffu;
-constexpr uint32_t kMagic = 0x4d534d31u;  // "MSM1": window sums present in the slice
+constexpr uint32_t kMagic = 0x4d534d32u;  // "MSM2": masked sums present in the slice
+// Curve-slice layout left by msm() for batch_msm_collect:
+//   [0, 96)          the result (Jacobian), written last
+//   [64, 96)         header {magic, c, W, nbits}
+//   [128, ...)       Q[w][0..nbits]: W * (nbits + 1) XYZZ masked sums
+//   then            S[w]: W Jacobian window sums (device tail only)
+constexpr size_t kSliceQ = 128;
+// bytes of a slice the layout above can use: W * (nbits + 1) masked sums and
+// W window sums stay below 28800 B for every window 2 <= c <= 17
+constexpr size_t kSliceReserve = 32 * 1024;
 
 // Arrays placed into caller memory, in two lifetimes: phase 1 (the sort) and
 // phase 2 (accumulation and reduction).  offsets and entries live across both;
 // the phase-1 temporaries (tmp, hist, starts, scan) are free again in phase 2.
 enum MsmArr {
-    A_OFFSETS, A_ENTRIES,                                          // both phases
+    A_ITEMS, A_SLOTS, A_OFFSETS, A_ENTRIES,                        // both phases
     A_TMP, A_HIST, A_STARTS, A_SCAN,                               // phase 1
-    A_BUCKETS, A_PHEAD, A_PTAIL, A_HB, A_TB, A_WS1, A_SUM1, A_Q,   // phase 2
+    A_BUCKETS, A_PHEAD, A_PTAIL, A_HB, A_TB, A_WS1, A_SUM1,        // phase 2
     A_COUNT
 };
 constexpr int kFirstPhase2 = A_BUCKETS;
 
 constexpr uint32_t kSortTile = 4096;  // scalars per histogram / scatter tile
+constexpr uint32_t kDigitBlock = 256;  // scalars per digit-kernel block
+
+// One MSM of a batch as the kernels see it (built on the device).
+struct MsmItemDev {
+    const G1Aff *pts;
+    const Fr *scalars;
+    uint32_t n, blk0, tile0, ntiles;  // blk0 / tile0: first digit block / sort tile of the item
+    uint64_t dig0;                    // offset of the item's digits (W * scalars before it)
+};
 
 struct MsmPlan {
-    uint32_t n, c, W, B, NB, T1, nbits;
-    uint32_t hi_bits, lo_bits, nbins, ntiles, nhist;
-    uint64_t max_entries, chunks;
+    uint32_t count, c, W, B, T1, nbits;
+    uint32_t hi_bits, lo_bits, nbins;
+    uint64_t ntot, nblk, ntiles, nhist, NB, max_entries, chunks;
     size_t size[A_COUNT];
     uint8_t *ptr[A_COUNT];
 };
 
 __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
 
-MsmPlan make_plan(uint32_t n, uint32_t c) {
+MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c) {
     MsmPlan p;
     memset(&p, 0, sizeof(p));
-    p.n = n;
+    p.count = count;
     p.c = c;
     p.W = (255 + c - 1) / c;
     p.B = 1u << (c - 1);
-    p.NB = p.W * p.B;
     p.T1 = (p.B + kL1 - 1) / kL1;
     p.nbits = log2_exact(p.T1);
-    p.max_entries = (uint64_t)n * p.W;
-    p.chunks = (p.max_entries + kChunk - 1) / kChunk;
     p.hi_bits = c - 1 > 8 ? c - 1 - 8 : 0;
     p.lo_bits = (c - 1) - p.hi_bits;
     p.nbins = 1u << p.hi_bits;
-    p.ntiles = (n + kSortTile - 1) / kSortTile;
-    p.nhist = p.W * p.nbins * p.ntiles;
+    for (uint32_t m = 0; m < count; m++) {
+        p.ntot += ns[m];
+        p.nblk += (ns[m] + kDigitBlock - 1) / kDigitBlock;
+        p.ntiles += (ns[m] + kSortTile - 1) / kSortTile;
+    }
+    p.NB = (uint64_t)count * p.W * p.B;
+    p.nhist = (uint64_t)p.W * p.nbins * p.ntiles;
+    p.max_entries = p.ntot * p.W;
+    p.chunks = (p.max_entries + kChunk - 1) / kChunk;
+    p.size[A_ITEMS] = (size_t)(count + 1) * sizeof(MsmItemDev);
+    p.size[A_SLOTS] = (size_t)count * sizeof(uint8_t *);
     p.size[A_OFFSETS] = (size_t)(p.NB + 1) * 4;
-    // digits (window-major, n * W u32) share this array with the final entries
+    // digits (window-major per item, ntot * W u32) share this array with the entries
     p.size[A_ENTRIES] = (size_t)p.max_entries * 4;
     p.size[A_TMP] = (size_t)p.max_entries * 8;
     p.size[A_HIST] = (size_t)p.nhist * 4;
@@ -93,9 +124,8 @@ MsmPlan make_plan(uint32_t n, uint32_t c) {
     p.size[A_PTAIL] = (size_t)p.chunks * sizeof(G1Xyzz);
     p.size[A_HB] = (size_t)p.chunks * 4;
     p.size[A_TB] = (size_t)p.chunks * 4;
-    p.size[A_WS1] = (size_t)p.W * p.T1 * sizeof(G1Xyzz);
-    p.size[A_SUM1] = (size_t)p.W * p.T1 * sizeof(G1Xyzz);
-    p.size[A_Q] = (size_t)p.W * (p.nbits + 1) * sizeof(G1Xyzz);
+    p.size[A_WS1] = (size_t)count * p.W * p.T1 * sizeof(G1Xyzz);
+    p.size[A_SUM1] = (size_t)count * p.W * p.T1 * sizeof(G1Xyzz);
     return p;
 }
 
@@ -144,7 +174,7 @@ bool place(MsmPlan &p, uint8_t *const *base, const size_t *bytes, int nreg) {
         segs[nseg].left = (base[r] && bytes[r] > a - b) ? bytes[r] - (a - b) : 0;
         nseg++;
     }
-    if (!place_range(p, A_OFFSETS, kFirstPhase2, segs, nseg)) return false;
+    if (!place_range(p, A_ITEMS, kFirstPhase2, segs, nseg)) return false;
     for (int a = A_TMP; a < kFirstPhase2; a++) {  // phase-1 temporaries become free
         segs[nseg].ptr = p.ptr[a];
         segs[nseg].left = align256(p.size[a]);
@@ -153,9 +183,16 @@ bool place(MsmPlan &p, uint8_t *const *base, const size_t *bytes, int nreg) {
     return place_range(p, kFirstPhase2, A_COUNT, segs, nseg);
 }
 
-// Default window for a single MSM of n points (the autotune table).
+// Tuned windows per size class ceil(log2 n); 0 = not tuned.  Written by the
+// autotuner (zk_msm_autotune / zk_msm_set_window) before use.
+uint8_t g_window[32] = {0};
+
+// Window for a single MSM of n points: the tuned entry for its size class,
+// else the model log2(n) - 5 clamped to [6, 17] (bucket reduction ~ 2^c per
+// window against n additions per window).
 uint32_t default_window(uint32_t n) {
     uint32_t lg = log2_exact(n ? n : 1);
+    if (lg < 32 && g_window[lg]) return g_window[lg];
     int c = (int)lg - 5;
     if (c < 6) c = 6;
     if (c > 17) c = 17;
@@ -183,51 +220,94 @@ __device__ __forceinline__ int32_t signed_digit(const uint32_t s[8], uint32_t w,
     return (int32_t)v;
 }
 
-// digits[w * n + i] = 0 for a zero digit, else |d| | (d < 0) << 31 with
-// |d| in [1, 2^(c-1)] (bucket |d| - 1)
-__global__ __launch_bounds__(256) void k_msm_digits(const Fr *__restrict__ scalars, uint32_t n, uint32_t c,
+// item of a global block / tile index: the last item whose first index is <= x
+__device__ __forceinline__ uint32_t item_of(const MsmItemDev *__restrict__ items, uint32_t count, uint32_t x,
+                                            bool by_tile) {
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if ((by_tile ? items[mid].tile0 : items[mid].blk0) <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// The item table (with the prefixes computed on the host) travels in kernel
+// arguments, kItemsPerLaunch items per launch, so no host buffer is copied.
+constexpr uint32_t kItemsPerLaunch = 16;
+struct ItemPack {
+    MsmItemDev it[kItemsPerLaunch];
+    uint8_t *slot[kItemsPerLaunch];  // the item's curve slice
+    uint32_t first, k;
+};
+__global__ void k_msm_put_items(ItemPack pk, MsmItemDev *__restrict__ items, uint8_t **__restrict__ slots) {
+    uint32_t j = threadIdx.x;
+    if (j >= pk.k) return;
+    items[pk.first + j] = pk.it[j];
+    slots[pk.first + j] = pk.slot[j];
+}
+
+// digits[dig0 + w * n + i] = 0 for a zero digit, else |d| | (d < 0) << 31 with
+// |d| in [1, 2^(c-1)] (bucket |d| - 1); one block = 256 scalars of one item
+__global__ __launch_bounds__(256) void k_msm_digits(const MsmItemDev *__restrict__ items, uint32_t count, uint32_t c,
                                                     uint32_t W, uint32_t *__restrict__ digits) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Fr s = fe_from_mont<FrParams>(fe_load<FrParams>(&scalars[i]));
+    const uint32_t m = item_of(items, count, blockIdx.x, false);
+    const MsmItemDev it = items[m];
+    const uint32_t i = (blockIdx.x - it.blk0) * kDigitBlock + threadIdx.x;
+    if (i >= it.n) return;
+    Fr s = fe_from_mont<FrParams>(fe_load<FrParams>(&it.scalars[i]));
+    uint32_t *dst = digits + it.dig0;
     uint32_t carry = 0;
     for (uint32_t w = 0; w < W; w++) {
         int32_t d = signed_digit(s.v, w, c, carry);
-        digits[(size_t)w * n + i] = d ? ((uint32_t)(d < 0 ? -d : d) | (d < 0 ? 0x80000000u : 0u)) : 0u;
+        dst[(size_t)w * it.n + i] = d ? ((uint32_t)(d < 0 ? -d : d) | (d < 0 ? 0x80000000u : 0u)) : 0u;
     }
 }
 
+// hist index of (item m, window w, bin, tile-in-item): groups (m, w, bin) are
+// contiguous and ordered, so their exclusive scan gives every group's start
+__device__ __forceinline__ uint64_t hist_index(const MsmItemDev &it, uint32_t W, uint32_t nbins, uint32_t w,
+                                               uint32_t bin, uint32_t tl) {
+    return (uint64_t)W * nbins * it.tile0 + ((uint64_t)w * nbins + bin) * it.ntiles + tl;
+}
+
 // per (tile, window): histogram of the high bucket bits
-// hist[(w * nbins + bin) * ntiles + tile]
-__global__ __launch_bounds__(256) void k_msm_hist(const uint32_t *__restrict__ digits, uint32_t n, uint32_t lo_bits,
-                                                  uint32_t nbins, uint32_t ntiles, uint32_t *__restrict__ hist) {
+__global__ __launch_bounds__(256) void k_msm_hist(const MsmItemDev *__restrict__ items, uint32_t count, uint32_t W,
+                                                  const uint32_t *__restrict__ digits, uint32_t lo_bits,
+                                                  uint32_t nbins, uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[256];
-    const uint32_t tile = blockIdx.x, w = blockIdx.y;
+    const uint32_t w = blockIdx.y;
+    const uint32_t m = item_of(items, count, blockIdx.x, true);
+    const MsmItemDev it = items[m];
+    const uint32_t tl = blockIdx.x - it.tile0;
     for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
     __syncthreads();
-    const uint32_t *dw = digits + (size_t)w * n;
-    uint32_t end = (tile + 1) * kSortTile < n ? (tile + 1) * kSortTile : n;
-    for (uint32_t i = tile * kSortTile + threadIdx.x; i < end; i += blockDim.x) {
+    const uint32_t *dw = digits + it.dig0 + (size_t)w * it.n;
+    uint32_t end = (tl + 1) * kSortTile < it.n ? (tl + 1) * kSortTile : it.n;
+    for (uint32_t i = tl * kSortTile + threadIdx.x; i < end; i += blockDim.x) {
         uint32_t d = dw[i];
         if (d) atomicAdd(&h[((d & 0x7fffffffu) - 1) >> lo_bits], 1u);
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) hist[((size_t)w * nbins + k) * ntiles + tile] = h[k];
+    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) hist[hist_index(it, W, nbins, w, k, tl)] = h[k];
 }
 
-// per (tile, window): place each nonzero digit into its (window, bin) group
-// at starts[...] + rank; tmp entry = (low bucket bits, index | sign)
-__global__ __launch_bounds__(256) void k_msm_scatter_hi(const uint32_t *__restrict__ digits, uint32_t n,
-                                                        uint32_t lo_bits, uint32_t nbins, uint32_t ntiles,
+// per (tile, window): place each nonzero digit into its (item, window, bin)
+// group at starts[...] + rank; tmp entry = (low bucket bits, index | sign)
+__global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__restrict__ items, uint32_t count,
+                                                        uint32_t W, const uint32_t *__restrict__ digits,
+                                                        uint32_t lo_bits, uint32_t nbins,
                                                         const uint32_t *__restrict__ starts, uint2 *__restrict__ tmp) {
     __shared__ uint32_t cur[256];
-    const uint32_t tile = blockIdx.x, w = blockIdx.y;
-    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) cur[k] = starts[((size_t)w * nbins + k) * ntiles + tile];
+    const uint32_t w = blockIdx.y;
+    const uint32_t m = item_of(items, count, blockIdx.x, true);
+    const MsmItemDev it = items[m];
+    const uint32_t tl = blockIdx.x - it.tile0;
+    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) cur[k] = starts[hist_index(it, W, nbins, w, k, tl)];
     __syncthreads();
-    const uint32_t *dw = digits + (size_t)w * n;
+    const uint32_t *dw = digits + it.dig0 + (size_t)w * it.n;
     const uint32_t lo_mask = (1u << lo_bits) - 1;
-    uint32_t end = (tile + 1) * kSortTile < n ? (tile + 1) * kSortTile : n;
-    for (uint32_t i = tile * kSortTile + threadIdx.x; i < end; i += blockDim.x) {
+    uint32_t end = (tl + 1) * kSortTile < it.n ? (tl + 1) * kSortTile : it.n;
+    for (uint32_t i = tl * kSortTile + threadIdx.x; i < end; i += blockDim.x) {
         uint32_t d = dw[i];
         if (d) {
             uint32_t b = (d & 0x7fffffffu) - 1;
@@ -237,19 +317,23 @@ __global__ __launch_bounds__(256) void k_msm_scatter_hi(const uint32_t *__restri
     }
 }
 
-// per (window, bin) group: counting sort on the low bucket bits in LDS;
+// per (item, window, bin) group: counting sort on the low bucket bits in LDS;
 // writes the final entries and the offsets of the group's buckets
-__global__ __launch_bounds__(256) void k_msm_sort_lo(const uint2 *__restrict__ tmp, const uint32_t *__restrict__ starts,
-                                                     uint32_t nbins, uint32_t ntiles, uint32_t lo_bits, uint32_t B,
-                                                     uint32_t ngroups, uint32_t *__restrict__ entries,
+__global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restrict__ items, uint32_t W, uint32_t nbins,
+                                                     const uint2 *__restrict__ tmp,
+                                                     const uint32_t *__restrict__ starts, uint32_t lo_bits,
+                                                     uint32_t B, uint32_t ngroups, uint32_t *__restrict__ entries,
                                                      uint32_t *__restrict__ offsets) {
     __shared__ uint32_t h[256];
     __shared__ uint32_t cur[256];
     const uint32_t g = blockIdx.x;
-    const uint32_t w = g / nbins, bin = g % nbins;
+    const uint32_t m = g / (W * nbins), r = g % (W * nbins);
+    const uint32_t w = r / nbins, bin = r % nbins;
+    const MsmItemDev it = items[m];
     const uint32_t nlo = 1u << lo_bits;
-    const uint32_t gs = starts[(size_t)g * ntiles];
-    const uint32_t ge = starts[(size_t)(g + 1) * ntiles];  // starts has a final total entry
+    const uint64_t idx = hist_index(it, W, nbins, w, bin, 0);
+    const uint32_t gs = starts[idx];
+    const uint32_t ge = starts[idx + it.ntiles];  // the next group's start (or the total)
     for (uint32_t k = threadIdx.x; k < nlo; k += blockDim.x) h[k] = 0;
     __syncthreads();
     for (uint32_t p = gs + threadIdx.x; p < ge; p += blockDim.x) atomicAdd(&h[tmp[p].x], 1u);
@@ -262,9 +346,9 @@ __global__ __launch_bounds__(256) void k_msm_sort_lo(const uint2 *__restrict__ t
         }
     }
     __syncthreads();
-    const uint32_t base = w * B + (bin << lo_bits);
+    const uint64_t base = ((uint64_t)m * W + w) * B + ((uint64_t)bin << lo_bits);
     for (uint32_t k = threadIdx.x; k < nlo; k += blockDim.x) offsets[base + k] = cur[k];
-    if (g == ngroups - 1 && threadIdx.x == 0) offsets[(size_t)w * B + B] = ge;  // = total: offsets[NB]
+    if (g == ngroups - 1 && threadIdx.x == 0) offsets[base + nlo] = ge;  // = total: offsets[NB]
     __syncthreads();
     for (uint32_t p = gs + threadIdx.x; p < ge; p += blockDim.x) {
         uint2 t = tmp[p];
@@ -347,19 +431,14 @@ __device__ __forceinline__ uint32_t find_bucket(const uint32_t *__restrict__ off
     return lo;
 }
 
-__device__ __forceinline__ G1Xyzz xyzz_add_entry(const G1Xyzz &acc, const G1Aff *__restrict__ pts, uint32_t e) {
-    G1Aff q = aff_load(&pts[e & 0x7fffffffu]);
-    if (aff_is_id(q)) return acc;
-    Fp y = (e >> 31) ? fe_neg<FpParams>(q.y) : q.y;
-    return xyzz_add_aff(acc, q.x, y);
-}
-
+// Buckets are numbered (item, window, bucket) = (m * W + w) * B + b over the
+// whole batch; the entries of item m occupy [off[m * W * B], off[(m + 1) * W * B]).
 __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restrict__ entries,
-                                                        const uint32_t *__restrict__ off, uint32_t NB,
-                                                        const G1Aff *__restrict__ pts, G1Xyzz *__restrict__ buckets,
-                                                        G1Xyzz *__restrict__ phead, G1Xyzz *__restrict__ ptail,
-                                                        uint32_t *__restrict__ hb, uint32_t *__restrict__ tb,
-                                                        uint64_t nchunks) {
+                                                        const uint32_t *__restrict__ off, uint32_t NB, uint32_t WB,
+                                                        const MsmItemDev *__restrict__ items,
+                                                        G1Xyzz *__restrict__ buckets, G1Xyzz *__restrict__ phead,
+                                                        G1Xyzz *__restrict__ ptail, uint32_t *__restrict__ hb,
+                                                        uint32_t *__restrict__ tb, uint64_t nchunks) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nchunks) return;
     const uint32_t total = off[NB];
@@ -371,12 +450,25 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
         uint32_t bend = off[b + 1];
         bool first = true, before = off[b] < s;
         G1Xyzz acc = xyzz_id();
+        // item (and point array) of the entry being prefetched
+        uint32_t m = b / WB;
+        uint32_t mend = off[(m + 1) * WB];
+        const G1Aff *pts = items[m].pts;
         // one uniform loop over the chunk (every lane runs e - s steps); the
         // next entry and its point are loaded one step ahead
         uint32_t ecur = entries[s];
         G1Aff qcur = aff_load(&pts[ecur & 0x7fffffffu]);
         for (uint32_t pos = s; pos < e; pos++) {
             const bool more = pos + 1 < e;
+            if (more && pos + 1 >= mend) {
+                // the next entry belongs to a later item (pos + 1 < total
+                // bounds m + 1 by the item count)
+                do {
+                    m++;
+                    mend = off[(m + 1) * WB];
+                } while (mend <= pos + 1);
+                pts = items[m].pts;
+            }
             const uint32_t enext = more ? entries[pos + 1] : 0u;
             const G1Aff qnext = more ? aff_load(&pts[enext & 0x7fffffffu]) : qcur;
             if (!aff_is_id(qcur)) {
@@ -435,7 +527,7 @@ __global__ __launch_bounds__(256) void k_msm_fixup(G1Xyzz *__restrict__ buckets,
 // directly.  Empty buckets are never read (their count is zero).
 
 // ---- reduction -------------------------------------------------------------
-// one thread per (window, segment of kL1 buckets)
+// one thread per (item window, segment of kL1 buckets); W = items * windows
 __global__ __launch_bounds__(64) void k_msm_reduce_l1(const G1Xyzz *__restrict__ buckets, const uint32_t *__restrict__ off,
                                 uint32_t W, uint32_t B, uint32_t T1, G1Xyzz *__restrict__ ws1,
                                 G1Xyzz *__restrict__ sum1) {
@@ -454,14 +546,16 @@ __global__ __launch_bounds__(64) void k_msm_reduce_l1(const G1Xyzz *__restrict__
     xyzz_store(&sum1[g], run);
 }
 
-// grid (nbits + 1, W): q < nbits -> sum of sum1[w][s] over s with bit q set;
-// q == nbits -> sum of ws1[w][s] over all s.
+// grid (nbits + 1, items * W): q < nbits -> sum of sum1[g][s] over s with
+// bit q set; q == nbits -> sum of ws1[g][s] over all s.  Written into the
+// item's curve slice at Q[w][q].
 __global__ __launch_bounds__(256) void k_msm_masked_sums(const G1Xyzz *__restrict__ ws1,
                                                          const G1Xyzz *__restrict__ sum1, uint32_t T1,
-                                                         uint32_t nbits, G1Xyzz *__restrict__ q) {
+                                                         uint32_t nbits, uint32_t W,
+                                                         uint8_t *const *__restrict__ slots) {
     __shared__ G1Xyzz sh[256];
-    uint32_t bit = blockIdx.x, w = blockIdx.y;
-    const G1Xyzz *src = (bit == nbits ? ws1 : sum1) + (size_t)w * T1;
+    uint32_t bit = blockIdx.x, g = blockIdx.y;
+    const G1Xyzz *src = (bit == nbits ? ws1 : sum1) + (size_t)g * T1;
     G1Xyzz acc = xyzz_id();
     for (uint32_t s = threadIdx.x; s < T1; s += blockDim.x) {
         if (bit == nbits || ((s >> bit) & 1u)) acc = xyzz_add(acc, xyzz_load(&src[s]));
@@ -472,19 +566,47 @@ __global__ __launch_bounds__(256) void k_msm_masked_sums(const G1Xyzz *__restric
         if (threadIdx.x < st) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + st]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) xyzz_store(&q[(size_t)w * (nbits + 1) + bit], sh[0]);
+    if (threadIdx.x == 0) {
+        G1Xyzz *q = reinterpret_cast<G1Xyzz *>(slots[g / W] + kSliceQ);
+        xyzz_store(&q[(size_t)(g % W) * (nbits + 1) + bit], sh[0]);
+    }
 }
 
 struct SliceHeader {
-    uint32_t magic, c, W, pad[5];
+    uint32_t magic, c, W, nbits, pad[4];
 };
 
-// one thread per window: S_w = A + L1 * sum_bit 2^bit Q_bit -> slice
-__global__ __launch_bounds__(64) void k_msm_window_finalize(const G1Xyzz *__restrict__ q, uint32_t W, uint32_t nbits, uint32_t c,
-                                      uint8_t *__restrict__ slice) {
+__host__ __device__ inline size_t slice_sums_off(uint32_t W, uint32_t nbits) {
+    return kSliceQ + (size_t)W * (nbits + 1) * sizeof(G1Xyzz);
+}
+__host__ __device__ inline size_t slice_bytes_needed(uint32_t W, uint32_t nbits) {
+    return slice_sums_off(W, nbits) + (size_t)W * sizeof(G1Jac);
+}
+
+__global__ void k_slice_header(uint8_t *const *slots, uint32_t count, uint32_t c, uint32_t W, uint32_t nbits) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= count) return;
+    SliceHeader *h = reinterpret_cast<SliceHeader *>(slots[m] + 64);
+    h->c = c;
+    h->W = W;
+    h->nbits = nbits;
+    h->magic = kMagic;
+}
+
+__device__ __forceinline__ bool header_ok(const SliceHeader *h) {
+    return h->magic == kMagic && h->W >= 1 && h->W <= 128 && h->c >= 2 && h->c <= 17 && h->nbits <= 16;
+}
+
+// Device tail, step 1 (grid: W threads per slice, y = slice):
+// S_w = A + L1 * sum_bit 2^bit Q_bit, written to the slice's S array.
+__global__ __launch_bounds__(64) void k_msm_window_finalize(uint8_t *const *__restrict__ slices) {
+    uint8_t *slice = slices[blockIdx.y];
+    const SliceHeader *h = reinterpret_cast<const SliceHeader *>(slice + 64);
+    if (!header_ok(h)) return;
+    const uint32_t W = h->W, nbits = h->nbits;
     uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= W) return;
-    const G1Xyzz *qw = q + (size_t)w * (nbits + 1);
+    const G1Xyzz *qw = reinterpret_cast<const G1Xyzz *>(slice + kSliceQ) + (size_t)w * (nbits + 1);
     G1Jac acc = xyzz_to_jac(nbits ? xyzz_load(&qw[nbits - 1]) : xyzz_id());
     for (int b = (int)nbits - 2; b >= 0; b--) {
         acc = jac_dbl(acc);
@@ -492,22 +614,14 @@ __global__ __launch_bounds__(64) void k_msm_window_finalize(const G1Xyzz *__rest
     }
     for (uint32_t k = 1; k < kL1; k <<= 1) acc = jac_dbl(acc);
     acc = jac_add(acc, xyzz_to_jac(xyzz_load(&qw[nbits])));
-    G1Jac *sums = reinterpret_cast<G1Jac *>(slice + sizeof(G1Jac));
-    jac_store(&sums[w], acc);
-    if (w == 0) {
-        SliceHeader *h = reinterpret_cast<SliceHeader *>(slice + 64);
-        h->magic = kMagic;
-        h->c = c;
-        h->W = W;
-    }
+    jac_store(reinterpret_cast<G1Jac *>(slice + slice_sums_off(W, nbits)) + w, acc);
 }
 
-// Horner over windows: result (normalised Jacobian) at slice[0].
+// Device tail, step 2: Horner over windows; result (z = 1) at slice[0].
 __device__ void collect_slice(uint8_t *slice) {
     const SliceHeader *h = reinterpret_cast<const SliceHeader *>(slice + 64);
-    uint32_t c = h->c, W = h->W;
-    if (h->magic != kMagic || W == 0 || W > 128 || c == 0 || c > 32) {
-        // no window sums in this slice (msm() failed or was not run): emit
+    if (!header_ok(h)) {
+        // no masked sums in this slice (msm() failed or was not run): emit
         // the identity rather than reading garbage
         G1Jac id;
         id.x = fe_zero<FpParams>();
@@ -516,7 +630,8 @@ __device__ void collect_slice(uint8_t *slice) {
         jac_store(reinterpret_cast<G1Jac *>(slice), id);
         return;
     }
-    const G1Jac *sums = reinterpret_cast<const G1Jac *>(slice + sizeof(G1Jac));
+    const uint32_t c = h->c, W = h->W;
+    const G1Jac *sums = reinterpret_cast<const G1Jac *>(slice + slice_sums_off(W, h->nbits));
     G1Jac acc = jac_load(&sums[W - 1]);
     for (int w = (int)W - 2; w >= 0; w--) {
         for (uint32_t k = 0; k < c; k++) acc = jac_dbl(acc);
@@ -530,19 +645,17 @@ __global__ __launch_bounds__(64) void k_msm_collect(uint8_t *const *__restrict__
     if (m >= batch) return;
     collect_slice(slices[m]);
 }
-__global__ __launch_bounds__(64) void k_msm_collect_one(uint8_t *slice) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) collect_slice(slice);
-}
 
 // device-side slice header reset so a stale header is never trusted
-__global__ void k_slice_clear(uint8_t *slice) {
-    if (threadIdx.x == 0) reinterpret_cast<SliceHeader *>(slice + 64)->magic = 0;
+__global__ void k_slice_clear(uint8_t *const *slots, uint32_t count) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < count) reinterpret_cast<SliceHeader *>(slots[m] + 64)->magic = 0;
 }
 
-// Opt-in phase timing: when enabled, msm_windows records a HIP event on the
+// Opt-in phase timing: when enabled (and the stream is not being captured), msm_run records a HIP event on the
 // launch stream after each phase, waits for the last one and accumulates
 // the per-phase durations (ms) and call counts.
-enum MsmPhase { PH_COUNT = 0, PH_SCAN, PH_SCATTER, PH_ACCUMULATE, PH_FIXUP, PH_REDUCE, PH_FINALIZE, PH_NUM };
+enum MsmPhase { PH_COUNT = 0, PH_SCAN, PH_SCATTER, PH_ACCUMULATE, PH_FIXUP, PH_REDUCE, PH_HEADER, PH_NUM };
 struct PhaseProf {
     bool on = false;
     double ms[PH_NUM] = {0};
@@ -552,16 +665,16 @@ struct PhaseProf {
 };
 PhaseProf g_prof;
 
-hipError_t prof_mark(int k, hipStream_t stream) {
-    if (!g_prof.on) return hipSuccess;
+hipError_t prof_mark(bool on, int k, hipStream_t stream) {
+    if (!on) return hipSuccess;
     if (!g_prof.init) {
         for (auto &e : g_prof.ev) ZK_CHECK(hipEventCreate(&e));
         g_prof.init = true;
     }
     return hipEventRecord(g_prof.ev[k], stream);
 }
-hipError_t prof_finish() {
-    if (!g_prof.on) return hipSuccess;
+hipError_t prof_finish(bool on) {
+    if (!on) return hipSuccess;
     ZK_CHECK(hipEventSynchronize(g_prof.ev[PH_NUM]));
     for (int k = 0; k < PH_NUM; k++) {
         float ms = 0;
@@ -587,23 +700,32 @@ hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *
 
 namespace zk {
 
-size_t msm_scratch_bytes(uint32_t n, uint32_t c) { return plan_total(make_plan(n, c)); }
-uint32_t msm_default_window(uint32_t n) { return default_window(n); }
+// One MSM of a batch on the host side: device point / scalar arrays and the
+// curve slice that receives the masked sums and, after the tail, the result.
+struct MsmJob {
+    const G1Aff *pts;
+    const Fr *scalars;
+    uint32_t n;
+    uint8_t *slice;
+};
 
-// Runs one MSM up to its per-window sums, written into `slice` (Jacobian
-// sums at slice + 96, header at slice + 64; slice needs 96 * (W + 1) bytes).
-// Scratch arrays are placed into the given regions; no allocation happens.
-// c == 0 picks the largest window <= the default that fits the regions.
-hipError_t msm_windows(uint8_t *slice, size_t slice_bytes, const G1Aff *points, const Fr *scalars, uint32_t n,
-                       uint32_t c, uint8_t *const *regions, const size_t *region_bytes, int nreg,
-                       hipStream_t stream) {
-    if (n == 0 || n > 0x7fffffffu || !slice || !points || !scalars || nreg < 1 || nreg > 8)
-        return hipErrorInvalidValue;
-    if (c != 0 && (c < 2 || c > 17)) return hipErrorInvalidValue;  // sort kernels: <= 8 + 8 bucket bits
-    MsmPlan p;
-    bool ok = false;
-    const uint32_t c0 = c ? c : default_window(n);
-    // the default window first, then larger windows (fewer entries), then smaller
+size_t msm_scratch_bytes(const uint32_t *ns, uint32_t count, uint32_t c) {
+    return plan_total(make_plan(ns, count, c));
+}
+uint32_t msm_default_window(uint32_t n) { return default_window(n); }
+// window for a batch: the default for its mean size
+uint32_t batch_default_window(const uint32_t *ns, uint32_t count) {
+    uint64_t t = 0;
+    for (uint32_t m = 0; m < count; m++) t += ns[m];
+    return default_window((uint32_t)(t / count ? t / count : 1));
+}
+
+// Picks the window and places the arrays: the requested window, or (c == 0)
+// the default window first, then larger windows (fewer entries), then
+// smaller ones, until the arrays fit the regions.  Host-only.
+bool choose_plan(const uint32_t *ns, uint32_t count, size_t slice_bytes, uint32_t c, uint8_t *const *regions,
+                 const size_t *region_bytes, int nreg, MsmPlan &p) {
+    const uint32_t c0 = c ? c : batch_default_window(ns, count);
     uint32_t order[32];
     int norder = 0;
     order[norder++] = c0;
@@ -611,12 +733,44 @@ hipError_t msm_windows(uint8_t *slice, size_t slice_bytes, const G1Aff *points, 
         for (uint32_t cc = c0 + 1; cc <= 17; cc++) order[norder++] = cc;
         for (uint32_t cc = c0 - 1; cc >= 2; cc--) order[norder++] = cc;
     }
-    for (int k = 0; k < norder && !ok; k++) {
-        p = make_plan(n, order[k]);
-        ok = (size_t)sizeof(G1Jac) * (p.W + 1) <= slice_bytes && place(p, regions, region_bytes, nreg);
+    for (int k = 0; k < norder; k++) {
+        p = make_plan(ns, count, order[k]);
+        // entry positions are u32 and the masked-sum grid's y is < 2^16
+        if (p.max_entries < 0x80000000ull && (uint64_t)count * p.W < 65536 &&
+            slice_bytes_needed(p.W, p.nbits) <= slice_bytes && place(p, regions, region_bytes, nreg))
+            return true;
     }
-    if (!ok) return hipErrorOutOfMemory;
+    return false;
+}
+
+// Runs a batch of MSMs up to their masked sums, written into each job's
+// slice (header at slice + 64, Q at slice + 128; a slice needs
+// slice_bytes_needed(W, nbits) <= slice_bytes).  All items go through every
+// phase together: one launch per phase for the whole batch.  Scratch arrays
+// are placed into the given regions; no allocation happens.  c == 0 picks the
+// largest window <= the default that fits the regions.  *slots_dev receives
+// the device array of slice pointers (inside the scratch) for a device tail.
+hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint32_t c, uint8_t *const *regions,
+                   const size_t *region_bytes, int nreg, hipStream_t stream, uint8_t ***slots_dev) {
+    if (count == 0 || !jobs || nreg < 1 || nreg > 8) return hipErrorInvalidValue;
+    if (c != 0 && (c < 2 || c > 17)) return hipErrorInvalidValue;  // sort kernels: <= 8 + 8 bucket bits
+    std::vector<uint32_t> ns(count);
+    uint64_t ntot = 0;
+    for (uint32_t m = 0; m < count; m++) {
+        const MsmJob &j = jobs[m];
+        if (j.n == 0 || j.n > 0x7fffffffu || !j.pts || !j.scalars || !j.slice) return hipErrorInvalidValue;
+        ns[m] = j.n;
+        ntot += j.n;
+    }
+    MsmPlan p;
+    if (!choose_plan(ns.data(), count, slice_bytes, c, regions, region_bytes, nreg, p)) return hipErrorOutOfMemory;
     c = p.c;
+    // phase timing waits on events, which a stream under graph capture cannot
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    ZK_CHECK(hipStreamIsCapturing(stream, &cap));
+    const bool prof = g_prof.on && cap == hipStreamCaptureStatusNone;
+    MsmItemDev *items = (MsmItemDev *)p.ptr[A_ITEMS];
+    uint8_t **slots = (uint8_t **)p.ptr[A_SLOTS];
     uint32_t *hist = (uint32_t *)p.ptr[A_HIST];
     uint32_t *offsets = (uint32_t *)p.ptr[A_OFFSETS];
     uint32_t *starts = (uint32_t *)p.ptr[A_STARTS];  // dead after k_msm_sort_lo
@@ -630,50 +784,162 @@ hipError_t msm_windows(uint8_t *slice, size_t slice_bytes, const G1Aff *points, 
     uint32_t *tb = (uint32_t *)p.ptr[A_TB];
     G1Xyzz *ws1 = (G1Xyzz *)p.ptr[A_WS1];
     G1Xyzz *sum1 = (G1Xyzz *)p.ptr[A_SUM1];
-    G1Xyzz *q = (G1Xyzz *)p.ptr[A_Q];
     uint32_t *scan_sums = (uint32_t *)p.ptr[A_SCAN];
 
-    ZK_CHECK(prof_mark(PH_COUNT, stream));
-    k_slice_clear<<<1, 1, 0, stream>>>(slice);
+    ZK_CHECK(prof_mark(prof, PH_COUNT, stream));
+    // item table with prefixes, written from kernel arguments
+    {
+        uint32_t blk = 0, tile = 0;
+        uint64_t dig = 0;
+        for (uint32_t first = 0; first < count; first += kItemsPerLaunch) {
+            ItemPack pk;
+            memset(&pk, 0, sizeof(pk));
+            pk.first = first;
+            pk.k = count - first < kItemsPerLaunch ? count - first : kItemsPerLaunch;
+            for (uint32_t j = 0; j < pk.k; j++) {
+                const MsmJob &jb = jobs[first + j];
+                MsmItemDev &it = pk.it[j];
+                it.pts = jb.pts;
+                it.scalars = jb.scalars;
+                it.n = jb.n;
+                it.blk0 = blk;
+                it.tile0 = tile;
+                it.ntiles = (jb.n + kSortTile - 1) / kSortTile;
+                it.dig0 = dig;
+                pk.slot[j] = jb.slice;
+                blk += (jb.n + kDigitBlock - 1) / kDigitBlock;
+                tile += it.ntiles;
+                dig += (uint64_t)jb.n * p.W;
+            }
+            k_msm_put_items<<<1, kItemsPerLaunch, 0, stream>>>(pk, items, slots);
+            ZK_CHECK(hipGetLastError());
+        }
+    }
+    k_slice_clear<<<(count + 63) / 64, 64, 0, stream>>>(slots, count);
     ZK_CHECK(hipGetLastError());
-    const uint32_t blocks = (n + 255) / 256;
-    k_msm_digits<<<blocks, 256, 0, stream>>>(scalars, n, c, p.W, digits);
+    k_msm_digits<<<(uint32_t)p.nblk, kDigitBlock, 0, stream>>>(items, count, c, p.W, digits);
     ZK_CHECK(hipGetLastError());
-    k_msm_hist<<<dim3(p.ntiles, p.W), 256, 0, stream>>>(digits, n, p.lo_bits, p.nbins, p.ntiles, hist);
+    k_msm_hist<<<dim3((uint32_t)p.ntiles, p.W), 256, 0, stream>>>(items, count, p.W, digits, p.lo_bits, p.nbins,
+                                                                  hist);
     ZK_CHECK(hipGetLastError());
-    ZK_CHECK(prof_mark(PH_SCAN, stream));
-    ZK_CHECK(launch_scan(hist, p.nhist, starts, nullptr, scan_sums, stream));
-    ZK_CHECK(prof_mark(PH_SCATTER, stream));
-    k_msm_scatter_hi<<<dim3(p.ntiles, p.W), 256, 0, stream>>>(digits, n, p.lo_bits, p.nbins, p.ntiles, starts, tmp);
+    ZK_CHECK(prof_mark(prof, PH_SCAN, stream));
+    ZK_CHECK(launch_scan(hist, (uint32_t)p.nhist, starts, nullptr, scan_sums, stream));
+    ZK_CHECK(prof_mark(prof, PH_SCATTER, stream));
+    k_msm_scatter_hi<<<dim3((uint32_t)p.ntiles, p.W), 256, 0, stream>>>(items, count, p.W, digits, p.lo_bits,
+                                                                        p.nbins, starts, tmp);
     ZK_CHECK(hipGetLastError());
-    k_msm_sort_lo<<<p.W * p.nbins, 256, 0, stream>>>(tmp, starts, p.nbins, p.ntiles, p.lo_bits, p.B, p.W * p.nbins,
-                                                     entries, offsets);
+    const uint32_t ngroups = count * p.W * p.nbins;
+    k_msm_sort_lo<<<ngroups, 256, 0, stream>>>(items, p.W, p.nbins, tmp, starts, p.lo_bits, p.B, ngroups, entries,
+                                                offsets);
     ZK_CHECK(hipGetLastError());
-    ZK_CHECK(prof_mark(PH_ACCUMULATE, stream));
+    ZK_CHECK(prof_mark(prof, PH_ACCUMULATE, stream));
     const uint32_t cblocks = (uint32_t)((p.chunks + 255) / 256);
-    k_msm_accumulate<<<cblocks, 256, 0, stream>>>(entries, offsets, p.NB, points, buckets, phead, ptail, hb,
-                                                  tb, p.chunks);
+    k_msm_accumulate<<<cblocks, 256, 0, stream>>>(entries, offsets, (uint32_t)p.NB, p.W * p.B, items, buckets,
+                                                  phead, ptail, hb, tb, p.chunks);
     ZK_CHECK(hipGetLastError());
-    ZK_CHECK(prof_mark(PH_FIXUP, stream));
+    ZK_CHECK(prof_mark(prof, PH_FIXUP, stream));
     k_msm_fixup<<<cblocks, 256, 0, stream>>>(buckets, phead, ptail, hb, tb, p.chunks);
     ZK_CHECK(hipGetLastError());
-    ZK_CHECK(prof_mark(PH_REDUCE, stream));
-    const uint32_t nseg = p.W * p.T1;
-    k_msm_reduce_l1<<<(nseg + 63) / 64, 64, 0, stream>>>(buckets, offsets, p.W, p.B, p.T1, ws1, sum1);
+    ZK_CHECK(prof_mark(prof, PH_REDUCE, stream));
+    const uint32_t gw = count * p.W;
+    const uint32_t nseg = gw * p.T1;
+    k_msm_reduce_l1<<<(nseg + 63) / 64, 64, 0, stream>>>(buckets, offsets, gw, p.B, p.T1, ws1, sum1);
     ZK_CHECK(hipGetLastError());
-    k_msm_masked_sums<<<dim3(p.nbits + 1, p.W), 256, 0, stream>>>(ws1, sum1, p.T1, p.nbits, q);
+    k_msm_masked_sums<<<dim3(p.nbits + 1, gw), 256, 0, stream>>>(ws1, sum1, p.T1, p.nbits, p.W, slots);
     ZK_CHECK(hipGetLastError());
-    ZK_CHECK(prof_mark(PH_FINALIZE, stream));
-    k_msm_window_finalize<<<(p.W + 63) / 64, 64, 0, stream>>>(q, p.W, p.nbits, c, slice);
+    ZK_CHECK(prof_mark(prof, PH_HEADER, stream));
+    k_slice_header<<<(count + 63) / 64, 64, 0, stream>>>(slots, count, c, p.W, p.nbits);
     ZK_CHECK(hipGetLastError());
-    ZK_CHECK(prof_mark(PH_NUM, stream));
-    return prof_finish();
+    ZK_CHECK(prof_mark(prof, PH_NUM, stream));
+    if (slots_dev) *slots_dev = slots;
+    return prof_finish(prof);
+}
+
+// Tail on the host: the per-window Horner over the masked sums and the Horner
+// over windows form a serial chain of ~300 point operations that a single
+// GPU wave runs in ~3 ms; host cores run it in well under 0.1 ms.  The Q
+// points are copied down, combined exactly as the device tail would, and the
+// normalised result is written back into each slice.
+void host_tail_one(const uint8_t *blob, uint8_t *out96) {
+    const SliceHeader *h = reinterpret_cast<const SliceHeader *>(blob + 64);
+    zkh::Jac res = zkh::jac_id();
+    if (h->magic == kMagic && h->W >= 1 && h->W <= 128 && h->c >= 2 && h->c <= 17 && h->nbits <= 16) {
+        const uint32_t c = h->c, W = h->W, nb = h->nbits;
+        const zkh::Fp *q = reinterpret_cast<const zkh::Fp *>(blob + kSliceQ);  // 4 Fp per XYZZ point
+        auto Q = [&](uint32_t w, uint32_t b) {
+            const zkh::Fp *p = q + ((size_t)w * (nb + 1) + b) * 4;
+            return zkh::jac_from_xyzz(p[0], p[1], p[2], p[3]);
+        };
+        for (int w = (int)W - 1; w >= 0; w--) {
+            zkh::Jac s = nb ? Q(w, nb - 1) : zkh::jac_id();
+            for (int b = (int)nb - 2; b >= 0; b--) s = zkh::jac_add(zkh::jac_dbl(s), Q(w, b));
+            for (uint32_t k = 1; k < kL1; k <<= 1) s = zkh::jac_dbl(s);
+            s = zkh::jac_add(s, Q(w, nb));
+            if (w != (int)W - 1)
+                for (uint32_t k = 0; k < c; k++) res = zkh::jac_dbl(res);
+            res = zkh::jac_add(res, s);
+        }
+    }
+    res = zkh::jac_normalize(res);
+    memcpy(out96, &res, 96);
+}
+
+hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stream) {
+    // the largest slice blob: header + Q for W <= 128, nbits <= 16 is bounded
+    // by slice_sums_off(W, nbits); read the header first to size each copy
+    std::vector<SliceHeader> hdr(batch);
+    for (uint32_t m = 0; m < batch; m++)
+        ZK_CHECK(hipMemcpyAsync(&hdr[m], slices[m] + 64, sizeof(SliceHeader), hipMemcpyDeviceToHost, stream));
+    ZK_CHECK(hipStreamSynchronize(stream));
+    std::vector<std::vector<uint8_t>> blob(batch);
+    for (uint32_t m = 0; m < batch; m++) {
+        const SliceHeader &h = hdr[m];
+        bool ok = h.magic == kMagic && h.W >= 1 && h.W <= 128 && h.nbits <= 16;
+        size_t bytes = ok ? slice_sums_off(h.W, h.nbits) : 128;
+        blob[m].assign(bytes, 0);
+        ZK_CHECK(hipMemcpyAsync(blob[m].data(), slices[m], bytes, hipMemcpyDeviceToHost, stream));
+    }
+    ZK_CHECK(hipStreamSynchronize(stream));
+    std::vector<uint8_t> out((size_t)batch * 96);
+    unsigned hw = std::thread::hardware_concurrency();
+    unsigned nt = batch < 2 ? 1 : (hw ? (hw < batch ? hw : batch) : 1);
+    if (nt > 16) nt = 16;
+    if (nt <= 1) {
+        for (uint32_t m = 0; m < batch; m++) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
+    } else {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; t++)
+            th.emplace_back([&, t]() {
+                for (uint32_t m = t; m < batch; m += nt) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
+            });
+        for (auto &x : th) x.join();
+    }
+    for (uint32_t m = 0; m < batch; m++)
+        ZK_CHECK(hipMemcpyAsync(slices[m], &out[(size_t)m * 96], 96, hipMemcpyHostToDevice, stream));
+    return hipStreamSynchronize(stream);  // the host buffers must outlive the copies
+}
+
+bool device_tail_requested() {
+    const char *e = getenv("ZK_MSM_TAIL");
+    return e && strcmp(e, "device") == 0;
+}
+
+hipError_t device_collect(uint8_t *const *slices_dev, uint32_t batch, hipStream_t stream) {
+    // W <= 128: two blocks of 64 threads per slice cover every window
+    k_msm_window_finalize<<<dim3(2, batch), 64, 0, stream>>>(slices_dev);
+    ZK_CHECK(hipGetLastError());
+    k_msm_collect<<<(batch + 63) / 64, 64, 0, stream>>>(slices_dev, batch);
+    return hipGetLastError();
 }
 
 hipError_t msm_collect(uint8_t *const *slices_dev, uint32_t batch, hipStream_t stream) {
     if (!batch) return hipSuccess;
-    k_msm_collect<<<(batch + 63) / 64, 64, 0, stream>>>(slices_dev, batch);
-    return hipGetLastError();
+    if (device_tail_requested()) return device_collect(slices_dev, batch, stream);
+    std::vector<uint8_t *> slices(batch);
+    ZK_CHECK(hipMemcpyAsync(slices.data(), slices_dev, (size_t)batch * sizeof(uint8_t *), hipMemcpyDeviceToHost,
+                            stream));
+    ZK_CHECK(hipStreamSynchronize(stream));
+    return host_collect(slices.data(), batch, stream);
 }
 
 }  // namespace zk
@@ -699,12 +965,17 @@ hipError_t msm(void *res, const void *points, const void *scalars, void *indices
     if (array_len <= 0 || windows <= 0 || window_bits <= 0 || window_bits > 30 || workers <= 0)
         return hipErrorInvalidValue;
     size_t idx_bytes = (size_t)array_len * (size_t)windows * 4;
-    uint8_t *regions[5] = {(uint8_t *)sort_buf, (uint8_t *)indices_buf0, (uint8_t *)indices_buf1,
-                           (uint8_t *)indices_buf2, (uint8_t *)indices_buf3};
-    size_t bytes[5] = {3 * idx_bytes, idx_bytes, idx_bytes, idx_bytes, idx_bytes};
     size_t slice_bytes = (((size_t)windows << window_bits) + 2 * (size_t)workers) * sizeof(G1Jac);
-    return msm_windows((uint8_t *)res, slice_bytes, (const G1Aff *)points, (const Fr *)scalars,
-                       (uint32_t)array_len, 0, regions, bytes, 5, stream);
+    // the curve slice beyond the first kSliceReserve bytes (header, masked and
+    // window sums) is free scratch as well: at least 2^10 * 26 Jacobian points
+    uint8_t *slice_free = slice_bytes > kSliceReserve ? (uint8_t *)res + kSliceReserve : nullptr;
+    uint8_t *regions[6] = {(uint8_t *)sort_buf, (uint8_t *)indices_buf0, (uint8_t *)indices_buf1,
+                           (uint8_t *)indices_buf2, (uint8_t *)indices_buf3, slice_free};
+    size_t bytes[6] = {3 * idx_bytes, idx_bytes, idx_bytes, idx_bytes, idx_bytes,
+                       slice_free ? slice_bytes - kSliceReserve : 0};
+    MsmJob job{(const G1Aff *)points, (const Fr *)scalars, (uint32_t)array_len, (uint8_t *)res};
+    return msm_run(&job, 1, slice_bytes < kSliceReserve ? slice_bytes : kSliceReserve, 0, regions, bytes, 6,
+                   stream, nullptr);
 }
 
 // cuda/msm.cu:434-467 (caller src/cuda/msm.rs:369-455).  buckets is a device
@@ -726,8 +997,8 @@ hipError_t batch_msm_collect(void *remain_indices, void *remain_acc, void *next_
 }
 
 // Phase timing (bench instrumentation): enable / read / reset.  times[k]
-// receives the accumulated ms of phase k (count, scan, scatter, accumulate,
-// fixup, reduce, window-finalize); returns the number of profiled calls.
+// receives the accumulated ms of phase k (digits+histogram, scan, scatter,
+// accumulate, fixup, reduce, header); returns the number of profiled calls.
 void zk_msm_profile_enable(int on) { g_prof.on = on != 0; }
 int zk_msm_profile_read(double *times, int ntimes, int reset) {
     for (int k = 0; k < ntimes && k < PH_NUM; k++) times[k] = g_prof.ms[k];
@@ -744,23 +1015,173 @@ int zk_msm_default_window(int n) { return n > 0 ? (int)default_window((uint32_t)
 // Scratch bytes zk_msm needs for n points at window c (0: default window).
 size_t zk_msm_scratch_bytes(int n, int c) {
     if (n <= 0) return 0;
-    uint32_t cc = c > 0 ? (uint32_t)c : default_window((uint32_t)n);
-    return msm_scratch_bytes((uint32_t)n, cc) + 256;
+    uint32_t un = (uint32_t)n;
+    uint32_t cc = c > 0 ? (uint32_t)c : default_window(un);
+    return msm_scratch_bytes(&un, 1, cc) + 256;
+}
+
+// Combines the masked sums of the given slices into their results (host or,
+// with ZK_MSM_TAIL=device, device tail) and waits for them.
+static hipError_t finish_batch(uint8_t *const *slices, uint8_t **slots_dev, uint32_t count, hipStream_t stream) {
+    if (device_tail_requested()) {
+        ZK_CHECK(device_collect(slots_dev, count, stream));
+        return hipStreamSynchronize(stream);
+    }
+    return host_collect(slices, count, stream);
 }
 
 // One MSM with caller-owned scratch (no allocation): the result (Jacobian,
 // z = 1, or the identity (0, 1, 0)) lands in out[0..96); out must hold
-// ZK_MSM_OUT_BYTES = 96 * 130 bytes (result, header and window sums).
-// c = 0 picks the default window for n.
+// ZK_MSM_OUT_BYTES (result, header, masked and window sums).  c = 0 picks
+// the default window for n.  Returns after the result is written.
 hipError_t zk_msm(void *out, const void *points, const void *scalars, int n, int c, void *scratch,
                   size_t scratch_bytes, hipStream_t stream) {
     if (n <= 0 || !out || !scratch) return hipErrorInvalidValue;
     uint8_t *regions[1] = {(uint8_t *)scratch};
     size_t bytes[1] = {scratch_bytes};
-    ZK_CHECK(msm_windows((uint8_t *)out, 96 * 130, (const G1Aff *)points, (const Fr *)scalars, (uint32_t)n,
-                         c > 0 ? (uint32_t)c : 0, regions, bytes, 1, stream));
-    k_msm_collect_one<<<1, 64, 0, stream>>>((uint8_t *)out);
-    return hipGetLastError();
+    uint8_t *slice = (uint8_t *)out;
+    MsmJob job{(const G1Aff *)points, (const Fr *)scalars, (uint32_t)n, slice};
+    uint8_t **slots = nullptr;
+    ZK_CHECK(msm_run(&job, 1, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1, stream, &slots));
+    return finish_batch(&slice, slots, 1, stream);
+}
+
+// Scratch bytes zk_msm_batch needs for these items at window c (0: default).
+size_t zk_msm_batch_scratch_bytes(const zk_msm_item *items, int count, int c) {
+    if (count <= 0 || !items) return 0;
+    std::vector<uint32_t> ns(count);
+    for (int m = 0; m < count; m++) {
+        if (items[m].n <= 0) return 0;
+        ns[m] = (uint32_t)items[m].n;
+    }
+    uint32_t cc = c > 0 ? (uint32_t)c : batch_default_window(ns.data(), (uint32_t)count);
+    return msm_scratch_bytes(ns.data(), (uint32_t)count, cc) + 256;
+}
+
+// count MSMs in one pass of the pipeline (one launch per phase for the whole
+// batch).  items is a host array; item m's result lands in
+// out + m * ZK_MSM_OUT_BYTES (Jacobian, z = 1).  Returns after the results
+// are written.
+hipError_t zk_msm_batch(void *out, const zk_msm_item *items, int count, int c, void *scratch,
+                        size_t scratch_bytes, hipStream_t stream) {
+    if (count <= 0 || !items || !out || !scratch) return hipErrorInvalidValue;
+    std::vector<MsmJob> jobs(count);
+    std::vector<uint8_t *> slices(count);
+    for (int m = 0; m < count; m++) {
+        if (items[m].n <= 0) return hipErrorInvalidValue;
+        slices[m] = (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES;
+        jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n,
+                         slices[m]};
+    }
+    uint8_t *regions[1] = {(uint8_t *)scratch};
+    size_t bytes[1] = {scratch_bytes};
+    uint8_t **slots = nullptr;
+    ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1,
+                     stream, &slots));
+    return finish_batch(slices.data(), slots, (uint32_t)count, stream);
+}
+
+// The window msm() would use with the reference driver's buffers for
+// array_len points (src/cuda/msm.rs:284-311 sizes), or 0 if its scratch
+// arrays would not fit.  Host-only: no device memory is touched.
+int zk_msm_plan_window(int array_len, int windows, int window_bits, int workers) {
+    if (array_len <= 0 || windows <= 0 || window_bits <= 0 || window_bits > 30 || workers <= 0) return 0;
+    size_t idx_bytes = (size_t)array_len * (size_t)windows * 4;
+    size_t slice_bytes = (((size_t)windows << window_bits) + 2 * (size_t)workers) * sizeof(G1Jac);
+    // stand-in bases 2^40 apart: placement only does address arithmetic
+    uint8_t *regions[6];
+    for (int r = 0; r < 6; r++) regions[r] = (uint8_t *)((uintptr_t)(r + 1) << 40);
+    size_t bytes[6] = {3 * idx_bytes, idx_bytes, idx_bytes, idx_bytes, idx_bytes,
+                       slice_bytes > kSliceReserve ? slice_bytes - kSliceReserve : 0};
+    uint32_t n = (uint32_t)array_len;
+    MsmPlan p;
+    if (!choose_plan(&n, 1, slice_bytes < kSliceReserve ? slice_bytes : kSliceReserve, 0, regions, bytes, 6, p))
+        return 0;
+    return (int)p.c;
+}
+
+// Window table access (a tuned table saved by the caller can be restored).
+hipError_t zk_msm_set_window(int log_n, int c) {
+    if (log_n < 0 || log_n > 31 || c < 0 || c == 1 || c > 17) return hipErrorInvalidValue;
+    g_window[log_n] = (uint8_t)c;
+    return hipSuccess;
+}
+int zk_msm_get_window(int log_n) {
+    if (log_n < 0 || log_n > 31) return 0;
+    return (int)default_window(log_n >= 31 ? 0x80000000u : 1u << log_n);
+}
+
+// Autotuner: times zk_msm on the caller's points / scalars (n of each,
+// device) for windows around the model's choice and records the fastest for
+// the size class ceil(log2 n).  ms (optional, 18 floats) receives the time
+// per window (0 for windows not tried).  Allocates its own scratch.
+hipError_t zk_msm_autotune(const void *points, const void *scalars, int n, int reps, hipStream_t stream, int *best,
+                           float *ms) {
+    if (n <= 0 || !points || !scalars) return hipErrorInvalidValue;
+    if (reps < 1) reps = 1;
+    const uint32_t lg = log2_exact((uint32_t)n);
+    const uint8_t saved = lg < 32 ? g_window[lg] : 0;
+    if (lg < 32) g_window[lg] = 0;
+    const int c0 = (int)default_window((uint32_t)n);
+    int lo = c0 - 2 < 4 ? 4 : c0 - 2, hi = c0 + 2 > 17 ? 17 : c0 + 2;
+    size_t need = 0;
+    for (int c = lo; c <= hi; c++) {
+        size_t b = zk_msm_scratch_bytes(n, c);
+        if (b > need) need = b;
+    }
+    void *scratch = nullptr, *out = nullptr;
+    hipError_t err = hipMalloc(&scratch, need);
+    if (err == hipSuccess) err = hipMalloc(&out, ZK_MSM_OUT_BYTES);
+    float best_ms = 0;
+    int best_c = c0;
+    if (ms)
+        for (int c = 0; c < 18; c++) ms[c] = 0;
+    for (int c = lo; c <= hi && err == hipSuccess; c++) {
+        err = zk_msm(out, points, scalars, n, c, scratch, need, stream);  // warm-up
+        double t = 1e30;
+        for (int r = 0; r < reps && err == hipSuccess; r++) {
+            auto t0 = std::chrono::steady_clock::now();
+            err = zk_msm(out, points, scalars, n, c, scratch, need, stream);
+            double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (dt < t) t = dt;
+        }
+        if (err != hipSuccess) break;
+        if (ms) ms[c] = (float)t;
+        if (best_ms == 0 || t < best_ms) {
+            best_ms = (float)t;
+            best_c = c;
+        }
+    }
+    if (scratch) (void)hipFree(scratch);
+    if (out) (void)hipFree(out);
+    if (err != hipSuccess) {
+        if (lg < 32) g_window[lg] = saved;
+        return err;
+    }
+    if (lg < 32) g_window[lg] = (uint8_t)best_c;
+    if (best) *best = best_c;
+    return hipSuccess;
+}
+
+// zk_msm_batch without any host wait: the window combination runs on the
+// device (one thread per item), so the call only enqueues kernels on stream
+// and can be captured into a HIP graph.  Results land as in zk_msm_batch once
+// the stream reaches them.  items is read during the call only.
+hipError_t zk_msm_batch_async(void *out, const zk_msm_item *items, int count, int c, void *scratch,
+                              size_t scratch_bytes, hipStream_t stream) {
+    if (count <= 0 || !items || !out || !scratch) return hipErrorInvalidValue;
+    std::vector<MsmJob> jobs(count);
+    for (int m = 0; m < count; m++) {
+        if (items[m].n <= 0) return hipErrorInvalidValue;
+        jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n,
+                         (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES};
+    }
+    uint8_t *regions[1] = {(uint8_t *)scratch};
+    size_t bytes[1] = {scratch_bytes};
+    uint8_t **slots = nullptr;
+    ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1,
+                     stream, &slots));
+    return device_collect(slots, (uint32_t)count, stream);
 }
 
 }  // extern "C"

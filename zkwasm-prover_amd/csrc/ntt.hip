@@ -15,6 +15,10 @@
 // `expand_omega_buffer` builds (one gather per element, no extra multiply);
 // the 2^(d-1) in-pass twiddles are staged once per workgroup in LDS.  The
 // n^-1 scaling of the iNTT is fused into the last pass.
+#include <string.h>
+
+#include <vector>
+
 #include "zkcore.h"
 
 using namespace zk;
@@ -25,18 +29,44 @@ constexpr uint32_t kTile = 1024;
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kMaxDeg = 8;
 
-__global__ __launch_bounds__(kThreads) void k_ntt_pass(const Fr *__restrict__ x, Fr *__restrict__ y,
-                                                       const Fr *__restrict__ omegas,
-                                                       const Fr *__restrict__ n_inv, uint32_t log_n,
-                                                       uint32_t log_p, uint32_t deg, uint32_t log_g) {
+// One pass of one NTT as a launch sees it.  A launch carries up to
+// kJobsPerLaunch jobs (one per NTT of a batch) in its kernel arguments; job j
+// owns the blocks [block0_j, block0_{j+1}).
+struct NttJob {
+    const Fr *x;
+    Fr *y;
+    const Fr *omegas;
+    const Fr *n_inv;  // null except on an iNTT's last pass
+    uint32_t block0;
+    uint8_t log_n, log_p, deg, log_g;
+};
+constexpr uint32_t kJobsPerLaunch = 48;
+struct NttPassArgs {
+    uint32_t count, pad;
+    NttJob job[kJobsPerLaunch];
+};
+
+__global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
     extern __shared__ Fr lds[];
+    // the job of this block (uniform: binary search in the kernel arguments)
+    uint32_t lo = 0, hi = a.count;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a.job[mid].block0 <= blockIdx.x) lo = mid; else hi = mid;
+    }
+    const Fr *__restrict__ x = a.job[lo].x;
+    Fr *__restrict__ y = a.job[lo].y;
+    const Fr *__restrict__ omegas = a.job[lo].omegas;
+    const Fr *__restrict__ n_inv = a.job[lo].n_inv;
+    const uint32_t log_n = a.job[lo].log_n, log_p = a.job[lo].log_p, deg = a.job[lo].deg, log_g = a.job[lo].log_g;
+    const uint32_t blk = blockIdx.x - a.job[lo].block0;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t D = 1u << deg;
     const uint32_t G = 1u << log_g;
     const uint32_t tile = D << log_g;
     const uint32_t log_t = log_n - deg;
     const uint32_t p = 1u << log_p;
-    const uint32_t index0 = blockIdx.x << log_g;
+    const uint32_t index0 = blk << log_g;
     Fr *s = lds;
     Fr *tw = lds + tile;
 
@@ -129,10 +159,15 @@ __global__ void k_coset_scale(Fr *s, const Fr *__restrict__ cp, uint32_t cn, uin
 
 namespace zk {
 
-// Radix plan: ceil(log_n / 8) passes, bits spread evenly (larger first).
+// Largest pass degree per transform size; 0 = kMaxDeg.  Set by the
+// autotuner (zk_ntt_set_max_deg), read by every plan.
+uint8_t g_max_deg[31] = {0};
+
+// Radix plan: ceil(log_n / max_deg) passes, bits spread evenly (larger first).
 int ntt_plan(uint32_t log_n, uint32_t *degs) {
     if (log_n == 0) return 0;
-    uint32_t passes = (log_n + kMaxDeg - 1) / kMaxDeg;
+    uint32_t md = log_n < 31 && g_max_deg[log_n] ? g_max_deg[log_n] : kMaxDeg;
+    uint32_t passes = (log_n + md - 1) / md;
     uint32_t rem = log_n;
     for (uint32_t i = 0; i < passes; i++) {
         uint32_t left = passes - i;
@@ -142,32 +177,77 @@ int ntt_plan(uint32_t log_n, uint32_t *degs) {
     return (int)passes;
 }
 
-// Runs the passes, ping-ponging between a and b; returns the number of
-// passes (the result is in b when odd).
+// One transform of a batch: input in a, the other buffer b.
+struct NttSpec {
+    Fr *a, *b;
+    const Fr *omegas, *n_inv;
+    uint32_t log_n;
+};
+
+// Runs a batch of NTTs: pass r of every transform that has one goes into the
+// same launch(es), kJobsPerLaunch transforms per launch.  passes[m] receives
+// transform m's pass count (its result is in b when odd).
+hipError_t ntt_run_batch(const NttSpec *specs, uint32_t count, hipStream_t stream, int *passes) {
+    std::vector<uint32_t> degs((size_t)count * 32);
+    std::vector<int> np(count);
+    std::vector<uint32_t> logp(count, 0);
+    int rounds = 0;
+    for (uint32_t m = 0; m < count; m++) {
+        np[m] = ntt_plan(specs[m].log_n, &degs[(size_t)m * 32]);
+        if (np[m] > rounds) rounds = np[m];
+        if (passes) passes[m] = np[m];
+    }
+    for (int r = 0; r < rounds; r++) {
+        NttPassArgs args;
+        memset(&args, 0, sizeof(args));
+        uint32_t blocks = 0;
+        size_t lds = 0;
+        auto flush = [&]() -> hipError_t {
+            if (!args.count) return hipSuccess;
+            k_ntt_pass<<<blocks, kThreads, lds, stream>>>(args);
+            ZK_CHECK(hipGetLastError());
+            memset(&args, 0, sizeof(args));
+            blocks = 0;
+            lds = 0;
+            return hipSuccess;
+        };
+        for (uint32_t m = 0; m < count; m++) {
+            if (r >= np[m]) continue;
+            const NttSpec &sp = specs[m];
+            const uint32_t n = 1u << sp.log_n;
+            const uint32_t deg = degs[(size_t)m * 32 + r];
+            uint32_t tile = n < kTile ? n : kTile;
+            uint32_t log_tile = log2_exact(tile);
+            uint32_t log_g = log_tile > deg ? log_tile - deg : 0;
+            uint32_t tile_elems = (1u << deg) << log_g;
+            NttJob &j = args.job[args.count++];
+            j.x = (r & 1) ? sp.b : sp.a;
+            j.y = (r & 1) ? sp.a : sp.b;
+            j.omegas = sp.omegas;
+            j.n_inv = r == np[m] - 1 ? sp.n_inv : nullptr;
+            j.block0 = blocks;
+            j.log_n = (uint8_t)sp.log_n;
+            j.log_p = (uint8_t)logp[m];
+            j.deg = (uint8_t)deg;
+            j.log_g = (uint8_t)log_g;
+            blocks += n / tile_elems;
+            size_t l = ((size_t)tile_elems + (1u << deg) / 2) * sizeof(Fr);
+            if (l > lds) lds = l;
+            logp[m] += deg;
+            if (args.count == kJobsPerLaunch) ZK_CHECK(flush());
+        }
+        ZK_CHECK(flush());
+    }
+    return hipSuccess;
+}
+
+// Runs the passes of one NTT, ping-ponging between a and b; returns the
+// number of passes (the result is in b when odd).
 hipError_t ntt_run(Fr *a, Fr *b, const Fr *omegas, const Fr *n_inv, uint32_t log_n,
                    hipStream_t stream, int *passes_out) {
-    uint32_t degs[8];
-    int passes = ntt_plan(log_n, degs);
-    Fr *src = a, *dst = b;
-    uint32_t log_p = 0;
-    const uint32_t n = 1u << log_n;
-    for (int ps = 0; ps < passes; ps++) {
-        uint32_t deg = degs[ps];
-        uint32_t tile = n < kTile ? n : kTile;
-        uint32_t log_tile = log2_exact(tile);
-        uint32_t log_g = log_tile > deg ? log_tile - deg : 0;
-        uint32_t tile_elems = (1u << deg) << log_g;
-        uint32_t blocks = n / tile_elems;
-        size_t lds = ((size_t)tile_elems + (1u << deg) / 2) * sizeof(Fr);
-        k_ntt_pass<<<blocks, kThreads, lds, stream>>>(src, dst, omegas,
-                                                      ps == passes - 1 ? n_inv : nullptr, log_n,
-                                                      log_p, deg, log_g);
-        ZK_CHECK(hipGetLastError());
-        log_p += deg;
-        Fr *t = src;
-        src = dst;
-        dst = t;
-    }
+    NttSpec sp{a, b, omegas, n_inv, log_n};
+    int passes = 0;
+    ZK_CHECK(ntt_run_batch(&sp, 1, stream, &passes));
     if (passes_out) *passes_out = passes;
     return hipSuccess;
 }
@@ -236,6 +316,91 @@ hipError_t extended_prepare(void *s, const void *coset_powers, unsigned coset_po
     if (to_coset) return coset_scale(f, (const Fr *)coset_powers, coset_powers_n, (uint32_t)extended_size, stream);
     ZK_CHECK(fill_zero(f + size, (size_t)(extended_size - size) * sizeof(Fr), stream));
     return coset_scale(f, (const Fr *)coset_powers, coset_powers_n, (uint32_t)size, stream);
+}
+
+// Batch of independent NTTs (sizes may differ): pass r of every transform
+// goes into one launch (kJobsPerLaunch transforms per launch), so a batch of
+// m transforms of at most P passes takes P launches instead of m * P.
+// Replaces a loop of `ntt` calls (src/cuda/ntt.rs:61-93 per polynomial).
+// items is a host array; swapped[m] (host) is set when result m is in tmp.
+// Asynchronous on stream, like `ntt`.
+hipError_t zk_ntt_batch(const zk_ntt_item *items, int count, unsigned char *swapped, hipStream_t stream) {
+    if (count <= 0 || !items || !swapped) return hipErrorInvalidValue;
+    std::vector<NttSpec> specs(count);
+    for (int m = 0; m < count; m++) {
+        const zk_ntt_item &it = items[m];
+        if (it.log_n < 0 || it.log_n > 30 || !it.buf || !it.omegas || (it.log_n > 0 && !it.tmp))
+            return hipErrorInvalidValue;
+        specs[m] = NttSpec{(Fr *)it.buf, (Fr *)it.tmp, (const Fr *)it.omegas, (const Fr *)it.n_inv,
+                           (uint32_t)it.log_n};
+    }
+    std::vector<int> passes(count);
+    ZK_CHECK(ntt_run_batch(specs.data(), (uint32_t)count, stream, passes.data()));
+    for (int m = 0; m < count; m++) swapped[m] = (passes[m] & 1) != 0;
+    return hipSuccess;
+}
+
+// Autotuner hook: the largest pass degree (radix 2^max_deg) for transforms
+// of 2^log_n elements; 0 restores the default (8).
+hipError_t zk_ntt_set_max_deg(int log_n, int max_deg) {
+    if (log_n < 1 || log_n > 30 || max_deg < 0 || max_deg > (int)kMaxDeg) return hipErrorInvalidValue;
+    g_max_deg[log_n] = (uint8_t)max_deg;
+    return hipSuccess;
+}
+int zk_ntt_get_max_deg(int log_n) {
+    if (log_n < 1 || log_n > 30) return 0;
+    return g_max_deg[log_n] ? g_max_deg[log_n] : (int)kMaxDeg;
+}
+
+// Autotuner: times the NTT of 2^log_n elements (buf / tmp device buffers,
+// omegas from expand_omega_buffer) for each distinct radix plan with largest
+// pass degree 4..8 and records the fastest; ms (optional, 9 floats) receives
+// the time per max degree.  The data in buf / tmp is overwritten.
+hipError_t zk_ntt_autotune(void *buf, void *tmp, const void *omegas, int log_n, int reps, hipStream_t stream,
+                           int *best, float *ms) {
+    if (log_n < 1 || log_n > 30 || !buf || !tmp || !omegas) return hipErrorInvalidValue;
+    if (reps < 1) reps = 1;
+    const uint8_t saved = g_max_deg[log_n];
+    hipEvent_t e0, e1;
+    ZK_CHECK(hipEventCreate(&e0));
+    hipError_t err = hipEventCreate(&e1);
+    if (err != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return err;
+    }
+    if (ms)
+        for (int d = 0; d <= (int)kMaxDeg; d++) ms[d] = 0;
+    float best_ms = 0;
+    int best_d = kMaxDeg;
+    uint32_t prev_passes = 0;
+    for (int d = (int)kMaxDeg; d >= 4 && err == hipSuccess; d--) {
+        g_max_deg[log_n] = (uint8_t)d;
+        uint32_t degs[32];
+        uint32_t passes = (uint32_t)ntt_plan((uint32_t)log_n, degs);
+        if (passes == prev_passes) continue;  // same plan as a larger max degree
+        prev_passes = passes;
+        int np = 0;
+        err = ntt_run((Fr *)buf, (Fr *)tmp, (const Fr *)omegas, nullptr, (uint32_t)log_n, stream, &np);
+        if (err == hipSuccess) err = hipEventRecord(e0, stream);
+        for (int r = 0; r < reps && err == hipSuccess; r++)
+            err = ntt_run((Fr *)buf, (Fr *)tmp, (const Fr *)omegas, nullptr, (uint32_t)log_n, stream, &np);
+        if (err == hipSuccess) err = hipEventRecord(e1, stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float t = 0;
+        if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+        if (err != hipSuccess) break;
+        t /= reps;
+        if (ms) ms[d] = t;
+        if (best_ms == 0 || t < best_ms) {
+            best_ms = t;
+            best_d = d;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_max_deg[log_n] = err == hipSuccess ? (uint8_t)best_d : saved;
+    if (err == hipSuccess && best) *best = best_d;
+    return err;
 }
 
 }  // extern "C"

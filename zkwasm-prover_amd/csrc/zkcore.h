@@ -32,6 +32,6 @@ hipError_t fill_zero(void *p, size_t bytes, hipStream_t stream);
 hipError_t coset_scale(Fr *s, const Fr *cp, uint32_t cn, uint32_t n, hipStream_t stream);
 hipError_t ntt_run(Fr *a, Fr *b, const Fr *omegas, const Fr *n_inv, uint32_t log_n, hipStream_t stream,
                    int *passes_out);
-int ntt_plan(uint32_t log_n, uint32_t *degs);
+int ntt_plan(uint32_t log_n, uint32_t *degs);  // degs: room for 32 pass degrees
 
 }  // namespace zk

@@ -52,6 +52,21 @@ SIGNATURES = {
     "zk_bench_dbl": [P, I32, I32, I32, P],
     "zk_bench_madd": [P, P, I32, I32, I32, I32, P],
     "zk_copy": [P, P, SZ, P],
+    "zk_rand_fr": [P, I32, ctypes.c_ulonglong, P],
+    "zk_gen_points": [P, I32, ctypes.c_ulonglong, P],
+    # MSM / NTT with caller scratch, batched forms, tuning tables
+    "zk_msm": [P, P, P, I32, I32, P, SZ, P],
+    "zk_msm_batch": [P, P, I32, I32, P, SZ, P],
+    "zk_msm_batch_async": [P, P, I32, I32, P, SZ, P],
+    "zk_graph_begin": [P],
+    "zk_graph_end": [P, P],
+    "zk_graph_launch": [P, P],
+    "zk_graph_destroy": [P],
+    "zk_ntt_batch": [P, I32, P, P],
+    "zk_msm_set_window": [I32, I32],
+    "zk_msm_autotune": [P, P, I32, I32, P, P, P],
+    "zk_ntt_set_max_deg": [I32, I32],
+    "zk_ntt_autotune": [P, P, P, I32, I32, P, P, P],
 }
 
 
