@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-proof", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-count", type=int, default=16)
     return ap.parse_args()
 
 
@@ -214,6 +217,10 @@ def main():
     if not args.no_batch:
         out["batched"] = {"msm": bench_batched_msm(L, points, max(1, args.steps // 2)),
                           "ntt": bench_batched_ntt(L, args.steps)}
+    if not args.no_proof:
+        out["proof_core"] = bench_proof_core([20, 22])
+    if not args.no_serve:
+        out["serving"] = bench_serving(args.serve_count)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)
@@ -280,6 +287,75 @@ def bench_batched_msm(L, points, reps):
     return {"workload": "64 G1 MSMs, sizes log-uniform 2^12..2^20 (%d points), shared bases" % total,
             "plain_ms": round(res["plain"], 3), "batched_ms": round(res["batched"], 3),
             "speedup": round(res["plain"] / res["batched"], 3), "identical_results": same}
+
+
+def bench_proof_core(ks):
+    """One proof's device arithmetic (prover core) for the benchmark circuit
+    family (40 advice, 6 range lookups into a 2^16 table, 12 permutation
+    columns, 2 shuffles of up to 2^18 rows) at each k: one warm-up proof, then
+    one timed proof through the batched MSM and one through one MSM call per
+    polynomial.  Witness synthesis is excluded (the reference takes the
+    advice as input)."""
+    import torch
+    from zkprover.prover import BATCHED, Circuit, Prover, ProvingKey, footprint
+    res = {}
+    for k in ks:
+        circ = Circuit.benchmark(k)
+        pk_b, job_b = footprint(circ)
+        pk = ProvingKey(circ, seed=7)
+        pr = Prover(pk)
+        try:
+            pr.prove(1, flags=BATCHED, want_pieces=False)
+            _, _, ib = pr.prove(2, flags=BATCHED, want_pieces=False)
+            _, _, ip = pr.prove(2, flags=0, want_pieces=False)
+        finally:
+            pr.close()
+            pk.close()
+            torch.cuda.synchronize()
+        res["k%d" % k] = {"batched_s": round(ib.total_ms / 1e3, 4), "plain_s": round(ip.total_ms / 1e3, 4),
+                          "speedup": round(ip.total_ms / ib.total_ms, 3),
+                          "phases_ms": ib.as_dict()["phase_ms"], "z_ok": bool(ib.z_ok and ip.z_ok),
+                          "quotient_ok": bool(ib.quotient_ok and ip.quotient_ok),
+                          "device_bytes": {"key": pk_b, "per_proof": job_b}}
+    return res
+
+
+def bench_serving(count):
+    """Proofs of k in {18, 20, 22} (weights 0.5 / 0.3 / 0.2) arriving as a
+    Poisson stream at about 0.9x the one-at-a-time capacity: the plain path
+    (one slot, FIFO) against the server (3 slots, admission by per-proof HBM
+    footprint).  Same arrivals, seeds and load for both."""
+    from zkprover.prover import Circuit
+    from zkprover.server import Server, run_poisson, summarize
+    shapes = [Circuit.benchmark(k) for k in (18, 20, 22)]
+    weights = [0.5, 0.3, 0.2]
+    out = {"workload": "Poisson arrivals of k=18/20/22 proofs (weights 0.5/0.3/0.2), %d requests" % count}
+    srv = Server(shapes, slots=1)
+    try:
+        # capacity estimate: one warm proof per shape, alone
+        t = []
+        for sh in range(3):
+            srv.wait(srv.submit(sh, 1), sh)
+            _, r = srv.wait(srv.submit(sh, 2), sh)
+            t.append(r.run_ms)
+        mean_ms = sum(w * x for w, x in zip(weights, t))
+        rate = 0.9 * 1000.0 / mean_ms
+        out["single_proof_ms"] = {"k18": round(t[0], 1), "k20": round(t[1], 1), "k22": round(t[2], 1)}
+        out["offered_rate_per_s"] = round(rate, 3)
+        recs, wall = run_poisson(srv, rate, count, weights, seed=10)
+        out["plain"] = summarize(recs, wall)
+    finally:
+        srv.close()
+    srv = Server(shapes, slots=3)
+    try:
+        for sh in range(3):
+            srv.wait(srv.submit(sh, 1), sh)
+        recs, wall = run_poisson(srv, rate, count, weights, seed=10)
+        out["served"] = summarize(recs, wall)
+        out["served"].update(srv.stats())
+    finally:
+        srv.close()
+    return out
 
 
 class _NttItem(ctypes.Structure):

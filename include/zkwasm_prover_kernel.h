@@ -156,6 +156,23 @@ hipError_t shuffle_eval_h_v2(void *res, const void *inputs, const void *tables, 
 
 /* ---- extensions: no counterpart in the reference's C ABI --------------- */
 
+/* Stream-taking forms of the reference entries that run on the legacy
+ * default stream (expand_omega_buffer, field_mul_zip, permutation_eval_h_p1,
+ * permutation_eval_h_p2, shuffle_eval_h, shuffle_eval_h_v2): same arguments
+ * plus the stream, so concurrent provers on their own streams overlap. */
+hipError_t zk_expand_omega_buffer_on(void *buf, int log_n, hipStream_t stream);
+hipError_t zk_field_mul_zip_on(void *buf, const void *coeff, int coeff_n, int n, hipStream_t stream);
+hipError_t zk_permutation_eval_h_p1_on(void *res, const void *first_set, const void *last_set, const void *l0,
+                                       const void *l_last, const void *y, int n, hipStream_t stream);
+hipError_t zk_permutation_eval_h_p2_on(void *res, const void *set, const void *l0, const void *l_last,
+                                       const void *y, int n_set, int rot, int n, hipStream_t stream);
+hipError_t zk_shuffle_eval_h_on(void *res, const void *input, const void *table, const void *z, const void *l0,
+                                const void *l_last, const void *l_active_row, const void *y, int rot, int n,
+                                hipStream_t stream);
+hipError_t zk_shuffle_eval_h_v2_on(void *res, const void *inputs, const void *tables, const void *betas,
+                                   int group_len, const void *z, const void *l0, const void *l_last,
+                                   const void *l_active_row, const void *y, int rot, int n, hipStream_t stream);
+
 /* Batch inversion in place, zeros stay zero (halo2 BatchInvert, used at
  * src/permutation.rs:73-75 and src/shuffle.rs:163-165).  temp: n Fr. */
 hipError_t zk_batch_invert(void *a, void *temp, int n, hipStream_t stream);
@@ -301,6 +318,66 @@ hipError_t zkp_vanishing_commit(void *h, void *tmp, const void *t_evals, int t_l
 /* src/eval_h.rs:546-583: h_pieces (host, 2^log_n Fr) = sum_i piece_i xn^i by
  * Horner, folded in place into the last piece of coeffs; xn: device x^n. */
 hipError_t zkp_vanishing_pieces(void *coeffs, const void *xn, int log_n, int quotient_degree, void *h_pieces);
+
+/* ---- prover core: one proof's device arithmetic ------------------------ */
+/* The PLONKish benchmark circuit: a*b + c - d = 0 under a selector on the
+ * usable rows; `lookups` advice columns range-checked against a fixed table of
+ * 2^table_log values; `perm_cols` advice columns joined by random copy
+ * cycles; `shuffles` column pairs, one a row permutation of the other on
+ * 2^shuffle_log rows; `blinding` random rows at the end.  advice must be at
+ * least 4 + lookups + perm_cols + 2 * shuffles. */
+typedef struct {
+    int k, advice, fixed, instance, lookups, table_log, perm_cols, shuffles, shuffle_log, blinding;
+} zkp_circuit;
+
+typedef struct {
+    int ncommit;         /* commitments written */
+    int z_ok;            /* every grand product closed at row u */
+    int quotient_ok;     /* h(X) has degree < 3n: all constraints hold */
+    int z_fail;          /* when !z_ok: bit 0 logup, bit 1 permutation, bit 2 shuffle */
+    double phase_ms[6];  /* witness, advice commit + iNTT, multiplicities,
+                            grand products, h evaluation, vanishing */
+    double total_ms;     /* phases 1-5 */
+} zkp_proof_info;
+
+/* Device bytes of the resident key data and of one in-flight proof. */
+size_t zkp_pk_bytes(const zkp_circuit *c);
+size_t zkp_job_bytes(const zkp_circuit *c);
+int zkp_commitment_count(const zkp_circuit *c);
+/* Key data (SRS stand-ins, fixed columns, sigma, tables): built once per
+ * circuit shape, shared by every job of it. */
+hipError_t zkp_pk_create(const zkp_circuit *c, unsigned long long seed, hipStream_t stream, void **pk);
+hipError_t zkp_pk_destroy(void *pk);
+/* Per-proof device state (witness, grand products, extended polys, h). */
+hipError_t zkp_job_create(void *pk, void **job);
+hipError_t zkp_job_destroy(void *job);
+/* One proof's arithmetic (src/lib.rs:205-1194 phase order) for seed, on
+ * stream; commitments (host) gets zkp_commitment_count affine points,
+ * h_pieces (host, 2^k Fr, may be null) the folded quotient.  flags:
+ * ZKP_BATCHED commits through zk_msm_batch (else one zk_msm per polynomial);
+ * ZKP_TAMPER_GATE breaks the gate on one usable row (a self-test: the
+ * quotient check must then fail). */
+#define ZKP_BATCHED 1
+#define ZKP_TAMPER_GATE 2
+hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commitments, void *h_pieces,
+                     zkp_proof_info *info, hipStream_t stream);
+
+/* ---- proof server: continuous admission by HBM footprint -------------- */
+/* Keys of every shape stay resident; `slots` workers (one stream each) start
+ * queued requests as soon as a slot is free and the proof's device state fits
+ * hbm_budget (0: free memory less 4 GiB), FIFO with backfill; per-proof
+ * states are pooled per shape.  flags: ZKP_BATCHED or 0.  No reference
+ * counterpart (src/lib.rs proves one circuit at a time). */
+typedef struct {
+    int shape, err, z_ok, quotient_ok, ncommit, reserved;
+    double queue_ms, run_ms, latency_ms, prove_ms;
+} zks_result;
+hipError_t zks_create(const zkp_circuit *shapes, int nshapes, int slots, size_t hbm_budget, int flags,
+                      unsigned long long key_seed, void **srv);
+int zks_submit(void *srv, int shape, unsigned long long seed);
+hipError_t zks_wait(void *srv, int id, void *commitments, zks_result *res);
+hipError_t zks_stats(void *srv, int *peak_running, size_t *peak_bytes, size_t *budget);
+hipError_t zks_destroy(void *srv);
 
 #ifdef __cplusplus
 }

@@ -182,3 +182,22 @@ def test_msm_fits_reference_buffers(lib, count):
         assert 2 <= c <= 17, (n, count)
         if n >= 1 << 12:
             assert c >= lib.zk_msm_default_window(n), (n, count, c)
+
+
+def test_prover_core_sizing_host_only(lib):
+    """footprints and commitment counts come from a host-side layout pass;
+    inconsistent circuits are rejected with 0"""
+    from zkprover.prover import Circuit, footprint
+    lib.zkp_commitment_count.restype = ctypes.c_int
+    lib.zkp_commitment_count.argtypes = [ctypes.c_void_p]
+    c20 = Circuit.benchmark(20)
+    pk20, job20 = footprint(c20)
+    pk18, job18 = footprint(Circuit.benchmark(18))
+    assert pk20 > pk18 > 0 and job20 > job18 > 0
+    # 40 advice + 6 m + 6 logup z + 6 permutation sets + 2 shuffle z + 3 pieces
+    assert lib.zkp_commitment_count(ctypes.byref(c20)) == 40 + 12 + 6 + 2 + 3
+    # per proof at k=20: ~ 40 * 3 n + 63 coset polys of 4n, well under 288 GB
+    assert 8e9 < job20 < 64e9
+    bad = Circuit.benchmark(20)
+    bad.advice = 10  # fewer than the columns the roles need
+    assert footprint(bad) == (0, 0)

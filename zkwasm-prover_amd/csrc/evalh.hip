@@ -285,27 +285,41 @@ hipError_t field_op_batch_mul_sum(void *res, const void *v, const void *rot, int
     return hipGetLastError();
 }
 
-hipError_t field_mul_zip(void *buf, const void *coeff, int coeff_n, int n) {
+hipError_t zk_field_mul_zip_on(void *buf, const void *coeff, int coeff_n, int n, hipStream_t stream) {
     if (n <= 0 || coeff_n <= 0 || !buf || !coeff) return hipErrorInvalidValue;
-    k_field_mul_zip<<<grid_for(n), kThreads, 0, 0>>>((Fr *)buf, (const Fr *)coeff, (uint32_t)coeff_n, (uint32_t)n);
+    k_field_mul_zip<<<grid_for(n), kThreads, 0, stream>>>((Fr *)buf, (const Fr *)coeff, (uint32_t)coeff_n, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t field_mul_zip(void *buf, const void *coeff, int coeff_n, int n) {
+    return zk_field_mul_zip_on(buf, coeff, coeff_n, n, 0);
+}
+
+hipError_t zk_permutation_eval_h_p1_on(void *res, const void *first_set, const void *last_set, const void *l0,
+                                 const void *l_last, const void *y, int n, hipStream_t stream) {
+    if (n <= 0 || !res || !first_set || !last_set || !l0 || !l_last || !y) return hipErrorInvalidValue;
+    k_perm_p1<<<grid_for(n), kThreads, 0, stream>>>((Fr *)res, (const Fr *)first_set, (const Fr *)last_set, (const Fr *)l0,
+                                               (const Fr *)l_last, (const Fr *)y, (uint32_t)n);
     return hipGetLastError();
 }
 
 hipError_t permutation_eval_h_p1(void *res, const void *first_set, const void *last_set, const void *l0,
                                  const void *l_last, const void *y, int n) {
-    if (n <= 0 || !res || !first_set || !last_set || !l0 || !l_last || !y) return hipErrorInvalidValue;
-    k_perm_p1<<<grid_for(n), kThreads, 0, 0>>>((Fr *)res, (const Fr *)first_set, (const Fr *)last_set, (const Fr *)l0,
-                                               (const Fr *)l_last, (const Fr *)y, (uint32_t)n);
+    return zk_permutation_eval_h_p1_on(res, first_set, last_set, l0, l_last, y, n, 0);
+}
+
+hipError_t zk_permutation_eval_h_p2_on(void *res, const void *set, const void *l0, const void *l_last, const void *y,
+                                 int n_set, int rot, int n, hipStream_t stream) {
+    (void)l_last;
+    if (bad_n(n) || !res || n_set < 0 || (n_set > 1 && !set) || !l0 || !y) return hipErrorInvalidValue;
+    k_perm_p2<<<grid_for(n), kThreads, 0, stream>>>((Fr *)res, (const Fr *const *)set, (const Fr *)l0, (const Fr *)y, n_set,
+                                               rot, (uint32_t)n);
     return hipGetLastError();
 }
 
 hipError_t permutation_eval_h_p2(void *res, const void *set, const void *l0, const void *l_last, const void *y,
                                  int n_set, int rot, int n) {
-    (void)l_last;
-    if (bad_n(n) || !res || n_set < 0 || (n_set > 1 && !set) || !l0 || !y) return hipErrorInvalidValue;
-    k_perm_p2<<<grid_for(n), kThreads, 0, 0>>>((Fr *)res, (const Fr *const *)set, (const Fr *)l0, (const Fr *)y, n_set,
-                                               rot, (uint32_t)n);
-    return hipGetLastError();
+    return zk_permutation_eval_h_p2_on(res, set, l0, l_last, y, n_set, rot, n, 0);
 }
 
 hipError_t permutation_eval_h_l(void *res, const void *beta, const void *gamma, const void *p, int n,
@@ -358,26 +372,37 @@ hipError_t logup_eval_h_inputs_product_sum(void *product, void *product_sum, con
     return hipGetLastError();
 }
 
-hipError_t shuffle_eval_h(void *res, const void *input, const void *table, const void *z, const void *l0,
-                          const void *l_last, const void *l_active_row, const void *y, int rot, int n) {
+hipError_t zk_shuffle_eval_h_on(void *res, const void *input, const void *table, const void *z, const void *l0,
+                          const void *l_last, const void *l_active_row, const void *y, int rot, int n, hipStream_t stream) {
     if (bad_n(n) || !res || !input || !table || !z || !l0 || !l_last || !l_active_row || !y) return hipErrorInvalidValue;
-    k_shuffle<<<grid_for(n), kThreads, 0, 0>>>((Fr *)res, (const Fr *)input, (const Fr *)table, (const Fr *)z,
+    k_shuffle<<<grid_for(n), kThreads, 0, stream>>>((Fr *)res, (const Fr *)input, (const Fr *)table, (const Fr *)z,
                                                (const Fr *)l0, (const Fr *)l_last, (const Fr *)l_active_row,
                                                (const Fr *)y, rot, (uint32_t)n);
+    return hipGetLastError();
+}
+
+hipError_t shuffle_eval_h(void *res, const void *input, const void *table, const void *z, const void *l0,
+                          const void *l_last, const void *l_active_row, const void *y, int rot, int n) {
+    return zk_shuffle_eval_h_on(res, input, table, z, l0, l_last, l_active_row, y, rot, n, 0);
+}
+
+hipError_t zk_shuffle_eval_h_v2_on(void *res, const void *inputs, const void *tables, const void *betas, int group_len,
+                             const void *z, const void *l0, const void *l_last, const void *l_active_row,
+                             const void *y, int rot, int n, hipStream_t stream) {
+    if (bad_n(n) || !res || !inputs || !tables || !betas || group_len <= 0 || !z || !l0 || !l_last || !l_active_row ||
+        !y)
+        return hipErrorInvalidValue;
+    k_shuffle_v2<<<grid_for(n), kThreads, 0, stream>>>((Fr *)res, (const Fr *const *)inputs, (const Fr *const *)tables,
+                                                  (const Fr *)betas, group_len, (const Fr *)z, (const Fr *)l0,
+                                                  (const Fr *)l_last, (const Fr *)l_active_row, (const Fr *)y, rot,
+                                                  (uint32_t)n);
     return hipGetLastError();
 }
 
 hipError_t shuffle_eval_h_v2(void *res, const void *inputs, const void *tables, const void *betas, int group_len,
                              const void *z, const void *l0, const void *l_last, const void *l_active_row,
                              const void *y, int rot, int n) {
-    if (bad_n(n) || !res || !inputs || !tables || !betas || group_len <= 0 || !z || !l0 || !l_last || !l_active_row ||
-        !y)
-        return hipErrorInvalidValue;
-    k_shuffle_v2<<<grid_for(n), kThreads, 0, 0>>>((Fr *)res, (const Fr *const *)inputs, (const Fr *const *)tables,
-                                                  (const Fr *)betas, group_len, (const Fr *)z, (const Fr *)l0,
-                                                  (const Fr *)l_last, (const Fr *)l_active_row, (const Fr *)y, rot,
-                                                  (uint32_t)n);
-    return hipGetLastError();
+    return zk_shuffle_eval_h_v2_on(res, inputs, tables, betas, group_len, z, l0, l_last, l_active_row, y, rot, n, 0);
 }
 
 }  // extern "C"

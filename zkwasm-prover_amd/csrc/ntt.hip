@@ -297,12 +297,18 @@ hipError_t ntt(void *buf, void *tmp, const void *pq, const void *omegas, const v
 // table for every size, so may_bit_reversed does not change the layout.
 hipError_t expand_omega_buffer(void *buf, int log_n, int may_bit_reversed) {
     (void)may_bit_reversed;
+    return zk_expand_omega_buffer_on(buf, log_n, 0);
+}
+
+// expand_omega_buffer on a given stream (the reference entry uses the legacy
+// default stream, which serialises with every other blocking stream)
+hipError_t zk_expand_omega_buffer_on(void *buf, int log_n, hipStream_t stream) {
     if (log_n < 0 || log_n > 30 || !buf) return hipErrorInvalidValue;
     uint32_t n = 1u << log_n;
     if (n <= 2) return hipSuccess;
     uint32_t chunk = n >= (1u << 16) ? 64 : 8;
     uint32_t threads = (n + chunk - 1) / chunk;
-    k_expand_omega<<<(threads + 63) / 64, 64, 0, 0>>>((Fr *)buf, n, chunk);
+    k_expand_omega<<<(threads + 63) / 64, 64, 0, stream>>>((Fr *)buf, n, chunk);
     return hipGetLastError();
 }
 

@@ -1,0 +1,25 @@
+// Launchers shared between translation units of the library (plain C++
+// declarations: host code can include this without the device headers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zk {
+
+// witness synthesis and helpers for the prover core (witness.hip)
+hipError_t wit_u32_rows_to_fr(void *col, const uint32_t *v, uint32_t rows, hipStream_t s);
+hipError_t wit_lookup_values(uint32_t *vals, uint32_t u, uint32_t tsize, uint64_t seed, hipStream_t s);
+hipError_t wit_hist(uint32_t *cnt, const uint32_t *vals, uint32_t u, hipStream_t s);
+hipError_t wit_table_rows(void *col, uint32_t u, uint32_t tsize, hipStream_t s);
+hipError_t wit_mult_rows(void *m, const uint32_t *cnt, uint32_t n, uint32_t tsize, hipStream_t s);
+hipError_t wit_gate_rows(void *d, const void *a, const void *b, const void *c, uint32_t u, hipStream_t s);
+hipError_t wit_indicator(void *out, uint32_t n, uint32_t lo, uint32_t hi, hipStream_t s);
+hipError_t wit_zero_rows(void *col, uint32_t lo, uint32_t hi, hipStream_t s);
+hipError_t wit_gather(void *dst, const void *src, const uint32_t *idx, uint32_t rows, hipStream_t s);
+hipError_t wit_cycle_values(void *const *cols_dev, const uint32_t *head, uint32_t ncols, uint32_t u, hipStream_t s);
+hipError_t wit_sigma_rows(void *const *sig_dev, const uint32_t *target, const void *deltas, const void *omegas,
+                          uint32_t ncols, uint32_t n, uint32_t u, hipStream_t s);
+hipError_t wit_fill_value(void *col, const void *v, uint32_t n, hipStream_t s);
+hipError_t wit_any_nonzero(const void *col, uint64_t lo, uint64_t hi, uint32_t *flag, hipStream_t s);
+
+}  // namespace zk

@@ -1,0 +1,103 @@
+"""Host binding of the prover core (zkp_pk_* / zkp_job_* / zkp_prove).
+
+A `Circuit` describes the PLONKish benchmark circuit; `ProvingKey` holds its
+resident shape data on the device; `Prover` owns one in-flight proof's device
+state and runs proofs on a stream.  The arithmetic is native (prover_core.cpp
+over the kernel library); this module only marshals arguments.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import HipError, check, lib
+
+
+class Circuit(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("k", "advice", "fixed", "instance", "lookups", "table_log",
+                                             "perm_cols", "shuffles", "shuffle_log", "blinding")]
+
+    @classmethod
+    def benchmark(cls, k):
+        """the 40-advice circuit family of the k=20 benchmark configuration"""
+        return cls(k=k, advice=40, fixed=10, instance=1, lookups=6, table_log=16, perm_cols=12, shuffles=2,
+                   shuffle_log=min(18, k - 2), blinding=5)
+
+
+class ProofInfo(ctypes.Structure):
+    _fields_ = [("ncommit", ctypes.c_int), ("z_ok", ctypes.c_int), ("quotient_ok", ctypes.c_int),
+                ("z_fail", ctypes.c_int), ("phase_ms", ctypes.c_double * 6), ("total_ms", ctypes.c_double)]
+
+    PHASES = ("witness", "advice_commit_intt", "multiplicities", "grand_products", "h_eval", "vanishing")
+
+    def as_dict(self):
+        return {"ncommit": self.ncommit, "z_ok": bool(self.z_ok), "quotient_ok": bool(self.quotient_ok),
+                "phase_ms": {k: round(self.phase_ms[i], 3) for i, k in enumerate(self.PHASES)},
+                "total_ms": round(self.total_ms, 3)}
+
+
+BATCHED = 1
+TAMPER_GATE = 2
+
+
+def _decl():
+    L = lib()
+    L.zkp_pk_bytes.restype = ctypes.c_size_t
+    L.zkp_pk_bytes.argtypes = [ctypes.c_void_p]
+    L.zkp_job_bytes.restype = ctypes.c_size_t
+    L.zkp_job_bytes.argtypes = [ctypes.c_void_p]
+    L.zkp_commitment_count.restype = ctypes.c_int
+    L.zkp_commitment_count.argtypes = [ctypes.c_void_p]
+    L.zkp_pk_create.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_void_p, ctypes.c_void_p]
+    L.zkp_pk_destroy.argtypes = [ctypes.c_void_p]
+    L.zkp_job_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.zkp_job_destroy.argtypes = [ctypes.c_void_p]
+    L.zkp_prove.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                            ctypes.c_void_p, ctypes.c_void_p]
+    return L
+
+
+def footprint(circuit):
+    """(key bytes, per-proof bytes) of device memory"""
+    L = _decl()
+    return L.zkp_pk_bytes(ctypes.byref(circuit)), L.zkp_job_bytes(ctypes.byref(circuit))
+
+
+class ProvingKey:
+    def __init__(self, circuit, seed=1, stream=0):
+        self.L = _decl()
+        self.circuit = circuit
+        self.handle = ctypes.c_void_p()
+        check(self.L.zkp_pk_create(ctypes.byref(circuit), seed, stream, ctypes.byref(self.handle)), "zkp_pk_create")
+        self.ncommit = self.L.zkp_commitment_count(ctypes.byref(circuit))
+
+    def close(self):
+        if self.handle:
+            check(self.L.zkp_pk_destroy(self.handle), "zkp_pk_destroy")
+            self.handle = ctypes.c_void_p()
+
+
+class Prover:
+    def __init__(self, pk):
+        self.pk = pk
+        self.L = pk.L
+        self.handle = ctypes.c_void_p()
+        check(self.L.zkp_job_create(pk.handle, ctypes.byref(self.handle)), "zkp_job_create")
+
+    def prove(self, seed, flags=BATCHED, stream=0, want_pieces=True):
+        """returns (commitments (ncommit, 8) u64 affine, h_pieces (n, 4) or None, ProofInfo)"""
+        n = 1 << self.pk.circuit.k
+        com = np.zeros((self.pk.ncommit, 8), dtype=np.uint64)
+        pieces = np.zeros((n, 4), dtype=np.uint64) if want_pieces else None
+        info = ProofInfo()
+        check(self.L.zkp_prove(self.handle, seed, flags, com.ctypes.data,
+                               pieces.ctypes.data if want_pieces else None, ctypes.byref(info), stream),
+              "zkp_prove")
+        return com, pieces, info
+
+    def close(self):
+        if self.handle:
+            check(self.L.zkp_job_destroy(self.handle), "zkp_job_destroy")
+            self.handle = ctypes.c_void_p()
+
+
+__all__ = ["Circuit", "ProofInfo", "ProvingKey", "Prover", "footprint", "BATCHED", "TAMPER_GATE", "HipError"]
