@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-proof", action="store_true")
-    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve", action="store_true")
     ap.add_argument("--serve-count", type=int, default=16)
     return ap.parse_args()
 
@@ -219,7 +219,7 @@ def main():
                           "ntt": bench_batched_ntt(L, args.steps)}
     if not args.no_proof:
         out["proof_core"] = bench_proof_core([20, 22])
-    if not args.no_serve:
+    if args.serve:
         out["serving"] = bench_serving(args.serve_count)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)

@@ -51,7 +51,9 @@ struct Request {
 
 struct Server {
     std::vector<ShapeState> shapes;
-    std::vector<Request> reqs;
+    // a deque: zks_submit appends while workers hold references to earlier
+    // requests, and push_back on a deque never moves existing elements
+    std::deque<Request> reqs;
     std::deque<int> queue;
     size_t budget = 0, used = 0, peak_used = 0;
     int flags = ZKP_BATCHED;
@@ -107,16 +109,18 @@ struct Server {
             void *job = nullptr;
             bool fresh = false;
             std::vector<void *> to_free;
+            Request *req = nullptr;  // element addresses of the deque are stable
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stop || pick(rid, job, fresh, to_free); });
                 if (rid < 0) return;  // stop with nothing admissible
                 running++;
                 if (running > peak_running) peak_running = running;
-                reqs[rid].t_start = now_ms();
+                req = &reqs[rid];
+                req->t_start = now_ms();
             }
             for (void *j : to_free) (void)zkp_job_destroy(j);
-            Request &rq = reqs[rid];
+            Request &rq = *req;
             ShapeState &sh = shapes[rq.shape];
             hipError_t e = hipSuccess;
             if (fresh) e = zkp_job_create(sh.pk, &job);

@@ -27,7 +27,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 }
 
 __global__ void k_u32_rows_to_fr(Fr *col, const uint32_t *__restrict__ v, uint32_t rows) {
-    ROWS(i, rows) fe_store<FrParams>(&col[i], fe_to_mont<FrParams>(fe_from_u32<FrParams>(v[i])));
+    ROWS(i, rows) fe_store<FrParams>(&col[i], fe_from_u32<FrParams>(v[i]));
 }
 
 // lookup input values below u: hash(seed, i) mod tsize
@@ -41,14 +41,14 @@ __global__ void k_hist(uint32_t *cnt, const uint32_t *__restrict__ vals, uint32_
 
 // col[i] = i mod tsize for i < u (Montgomery)
 __global__ void k_table_rows(Fr *col, uint32_t u, uint32_t tsize) {
-    ROWS(i, u) fe_store<FrParams>(&col[i], fe_to_mont<FrParams>(fe_from_u32<FrParams>(i % tsize)));
+    ROWS(i, u) fe_store<FrParams>(&col[i], fe_from_u32<FrParams>(i % tsize));
 }
 
 // m[i] = cnt[i] for i < tsize, 0 elsewhere
 __global__ void k_mult_rows(Fr *m, const uint32_t *__restrict__ cnt, uint32_t n, uint32_t tsize) {
     ROWS(i, n) {
         uint32_t c = i < tsize ? cnt[i] : 0u;
-        fe_store<FrParams>(&m[i], fe_to_mont<FrParams>(fe_from_u32<FrParams>(c)));
+        fe_store<FrParams>(&m[i], fe_from_u32<FrParams>(c));
     }
 }
 
