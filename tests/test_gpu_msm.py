@@ -216,3 +216,18 @@ def test_msm_batch_equals_single_calls(gpu, zo, monkeypatch):
     for m in range(len(sizes)):
         assert (host[m:m + 1] == one[m]).all()
         assert (dev[m:m + 1] == one[m]).all()
+
+
+@pytest.mark.parametrize("log_n,values", [(16, [1]), (16, [0, 1, 2, 3]), (18, [1, 2]), (14, [R - 1, 1])])
+def test_msm_heavy_buckets(gpu, zo, log_n, values):
+    """scalars from a tiny set put almost every entry into a few buckets that
+    span thousands of accumulation chunks (the multiplicities of a lookup look
+    like this); their partial sums are merged by one workgroup per bucket"""
+    n = 1 << log_n
+    rng = np.random.default_rng(log_n + len(values))
+    pts = zo.gen_bases_random(n, 70 + log_n)
+    sc = zo.to_mont([int(values[i]) for i in rng.integers(0, len(values), n)])
+    exp = zo.g1_to_affine(zo.msm(pts, sc))
+    assert (_affine_of(zo, _zk_msm(pts, sc)) == exp).all()
+    got = zo.g1_to_affine(_zk_msm_batch([pts, pts], [sc, sc]))
+    assert (got[0:1] == exp).all() and (got[1:2] == exp).all()

@@ -45,6 +45,7 @@ namespace {
 
 constexpr uint32_t kL1 = 8;          // buckets per first-level reduction segment
 constexpr uint32_t kChunk = 48;      // sorted entries per accumulation thread
+constexpr uint32_t kWalk = 32;       // chunk partials one fixup thread sums; longer buckets go to a workgroup
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMagic = 0x4d534d32u;  // "MSM2": masked sums present in the slice
 // Curve-slice layout left by msm() for batch_msm_collect:
@@ -58,10 +59,11 @@ constexpr size_t kSliceQ = 128;
 constexpr size_t kSliceReserve = 32 * 1024;
 
 // Arrays placed into caller memory, in two lifetimes: phase 1 (the sort) and
-// phase 2 (accumulation and reduction).  offsets and entries live across both;
+// phase 2 (accumulation and reduction).  offsets, entries and the long-bucket
+// list live across both;
 // the phase-1 temporaries (tmp, hist, starts, scan) are free again in phase 2.
 enum MsmArr {
-    A_ITEMS, A_SLOTS, A_OFFSETS, A_ENTRIES,                        // both phases
+    A_ITEMS, A_SLOTS, A_OFFSETS, A_ENTRIES, A_LONG,                // both phases
     A_TMP, A_HIST, A_STARTS, A_SCAN,                               // phase 1
     A_BUCKETS, A_PHEAD, A_PTAIL, A_HB, A_TB, A_WS1, A_SUM1,        // phase 2
     A_COUNT
@@ -126,6 +128,8 @@ MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c) {
     p.size[A_TB] = (size_t)p.chunks * 4;
     p.size[A_WS1] = (size_t)count * p.W * p.T1 * sizeof(G1Xyzz);
     p.size[A_SUM1] = (size_t)count * p.W * p.T1 * sizeof(G1Xyzz);
+    // [count, (bucket, first chunk, last chunk) per bucket spanning > kWalk chunks]
+    p.size[A_LONG] = (size_t)(1 + 3 * (p.chunks / kWalk + 1)) * 4;
     return p;
 }
 
@@ -239,8 +243,10 @@ struct ItemPack {
     uint8_t *slot[kItemsPerLaunch];  // the item's curve slice
     uint32_t first, k;
 };
-__global__ void k_msm_put_items(ItemPack pk, MsmItemDev *__restrict__ items, uint8_t **__restrict__ slots) {
+__global__ void k_msm_put_items(ItemPack pk, MsmItemDev *__restrict__ items, uint8_t **__restrict__ slots,
+                                uint32_t *__restrict__ long_count) {
     uint32_t j = threadIdx.x;
+    if (pk.first == 0 && j == 0) *long_count = 0;
     if (j >= pk.k) return;
     items[pk.first + j] = pk.it[j];
     slots[pk.first + j] = pk.slot[j];
@@ -509,17 +515,54 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
     tb[t] = tail_bucket;
 }
 
-// merges the partial sums of buckets that span several chunks
+// merges the partial sums of buckets that span several chunks.  A bucket b
+// whose run starts in chunk t (tail partial) ends in chunk t' = (off[b+1]-1) /
+// kChunk, and chunks t+1..t' all hold head partials of b.  Spans up to kWalk
+// chunks are summed by the thread; longer ones (few: their count is at most
+// nchunks / kWalk) are listed for k_msm_fixup_long, one workgroup each, so a
+// heavy bucket (e.g. the multiplicities of a lookup, all in a handful of
+// buckets) never becomes a serial chain.
 __global__ __launch_bounds__(256) void k_msm_fixup(G1Xyzz *__restrict__ buckets, const G1Xyzz *__restrict__ phead,
-                            const G1Xyzz *__restrict__ ptail, const uint32_t *__restrict__ hb,
-                            const uint32_t *__restrict__ tb, uint64_t nchunks) {
+                                                   const G1Xyzz *__restrict__ ptail, const uint32_t *__restrict__ tb,
+                                                   const uint32_t *__restrict__ off, uint32_t *__restrict__ longs,
+                                                   uint64_t nchunks) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nchunks) return;
     uint32_t b = tb[t];
     if (b == kNone) return;
+    const uint64_t last = (uint64_t)(off[b + 1] - 1) / kChunk;
+    if (last - t > kWalk) {
+        uint32_t i = atomicAdd(&longs[0], 1u);
+        longs[1 + 3 * i] = b;
+        longs[2 + 3 * i] = (uint32_t)t;
+        longs[3 + 3 * i] = (uint32_t)last;
+        return;
+    }
     G1Xyzz acc = xyzz_load(&ptail[t]);
-    for (uint64_t u = t + 1; u < nchunks && hb[u] == b; u++) acc = xyzz_add(acc, xyzz_load(&phead[u]));
+    for (uint64_t u = t + 1; u <= last; u++) acc = xyzz_add(acc, xyzz_load(&phead[u]));
     xyzz_store(&buckets[b], acc);
+}
+
+// one workgroup per listed long bucket: ptail[t] + sum of phead[t+1..last]
+__global__ __launch_bounds__(256) void k_msm_fixup_long(G1Xyzz *__restrict__ buckets,
+                                                        const G1Xyzz *__restrict__ phead,
+                                                        const G1Xyzz *__restrict__ ptail,
+                                                        const uint32_t *__restrict__ longs) {
+    __shared__ G1Xyzz sh[256];
+    const uint32_t count = longs[0];
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const uint32_t b = longs[1 + 3 * i], t = longs[2 + 3 * i], last = longs[3 + 3 * i];
+        G1Xyzz acc = threadIdx.x == 0 ? xyzz_load(&ptail[t]) : xyzz_id();
+        for (uint32_t u = t + 1 + threadIdx.x; u <= last; u += blockDim.x) acc = xyzz_add(acc, xyzz_load(&phead[u]));
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (uint32_t st = 128; st; st >>= 1) {
+            if (threadIdx.x < st) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + st]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) xyzz_store(&buckets[b], sh[0]);
+        __syncthreads();
+    }
 }
 
 // buckets that some chunk starts inside and no chunk owns fully are
@@ -785,6 +828,7 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     G1Xyzz *ws1 = (G1Xyzz *)p.ptr[A_WS1];
     G1Xyzz *sum1 = (G1Xyzz *)p.ptr[A_SUM1];
     uint32_t *scan_sums = (uint32_t *)p.ptr[A_SCAN];
+    uint32_t *longs = (uint32_t *)p.ptr[A_LONG];
 
     ZK_CHECK(prof_mark(prof, PH_COUNT, stream));
     // item table with prefixes, written from kernel arguments
@@ -811,7 +855,7 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
                 tile += it.ntiles;
                 dig += (uint64_t)jb.n * p.W;
             }
-            k_msm_put_items<<<1, kItemsPerLaunch, 0, stream>>>(pk, items, slots);
+            k_msm_put_items<<<1, kItemsPerLaunch, 0, stream>>>(pk, items, slots, longs);
             ZK_CHECK(hipGetLastError());
         }
     }
@@ -838,8 +882,14 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
                                                   phead, ptail, hb, tb, p.chunks);
     ZK_CHECK(hipGetLastError());
     ZK_CHECK(prof_mark(prof, PH_FIXUP, stream));
-    k_msm_fixup<<<cblocks, 256, 0, stream>>>(buckets, phead, ptail, hb, tb, p.chunks);
+    k_msm_fixup<<<cblocks, 256, 0, stream>>>(buckets, phead, ptail, tb, offsets, longs, p.chunks);
     ZK_CHECK(hipGetLastError());
+    {
+        const uint64_t max_long = p.chunks / kWalk + 1;
+        k_msm_fixup_long<<<(uint32_t)(max_long < 1024 ? max_long : 1024), 256, 0, stream>>>(buckets, phead, ptail,
+                                                                                           longs);
+        ZK_CHECK(hipGetLastError());
+    }
     ZK_CHECK(prof_mark(prof, PH_REDUCE, stream));
     const uint32_t gw = count * p.W;
     const uint32_t nseg = gw * p.T1;
