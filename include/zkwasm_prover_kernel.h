@@ -173,6 +173,16 @@ hipError_t zk_shuffle_eval_h_v2_on(void *res, const void *inputs, const void *ta
                                    int group_len, const void *z, const void *l0, const void *l_last,
                                    const void *l_active_row, const void *y, int rot, int n, hipStream_t stream);
 
+/* One permutation set's term of h on the extended coset in one pass
+ * (replaces the field_op / permutation_eval_h_l / extended-NTT sequence of
+ * src/eval_h.rs:700-840): h = h*y + l_active * (z(wX) prod_c (v_c + beta
+ * sigma_c + gamma) - z(X) prod_c (v_c + gamma + bd_c X)); v, sigma: device
+ * arrays of nc <= 16 coset columns, bd: nc Fr (beta delta^c), X: coset
+ * points, rot: one row's shift on the extended domain. */
+hipError_t zk_permutation_eval_h_set(void *h, const void *z, const void *v, const void *sigma, int nc,
+                                     const void *beta, const void *gamma, const void *bd, const void *X,
+                                     const void *l_active, const void *y, int rot, int n, hipStream_t stream);
+
 /* Batch inversion in place, zeros stay zero (halo2 BatchInvert, used at
  * src/permutation.rs:73-75 and src/shuffle.rs:163-165).  temp: n Fr. */
 hipError_t zk_batch_invert(void *a, void *temp, int n, hipStream_t stream);

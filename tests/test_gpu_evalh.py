@@ -156,3 +156,30 @@ def test_shuffle_kernels(gpu, zo, log_n):
          A.put(ll), A.put(la), A.put(y), rot, n)
     exp = zo.shuffle_eval_h_v2(exp, ins, tbs, betas, z, l0, ll, la, y, rot)
     assert (dv.from_dev(d) == exp).all()
+
+
+@pytest.mark.parametrize("nc,rot", [(1, 4), (2, 4), (3, -4)])
+def test_permutation_eval_h_set(gpu, zo, nc, rot):
+    """the fused permutation-set term equals the same expression built from the
+    oracle's element-wise field operations (the reference's field_op sequence,
+    src/eval_h.rs:700-840)"""
+    dv, call, A = _env()
+    n = 1 << 12
+    h, z, X, la = _r(zo, n, 40), _r(zo, n, 41), _r(zo, n, 42), _r(zo, n, 43)
+    beta, gamma, y = _r(zo, 1, 44), _r(zo, 1, 45), _r(zo, 1, 46)
+    vs = [_r(zo, n, 50 + c) for c in range(nc)]
+    ss = [_r(zo, n, 60 + c) for c in range(nc)]
+    bd = _r(zo, nc, 70)
+    # l = z(wX) prod (v + beta sigma + gamma), r = z prod (v + gamma + bd X)
+    l = zo.field_op(z, rot, None, None, 0, None, n, 2)
+    r = np.array(z, copy=True)
+    for c in range(nc):
+        vg = zo.field_op(vs[c], 0, None, None, 0, gamma, n, 0)
+        l = zo.field_op(l, 0, None, zo.field_op(ss[c], 0, beta, vg, 0, None, n, 0), 0, None, n, 1)
+        r = zo.field_op(r, 0, None, zo.field_op(X, 0, bd[c:c + 1], vg, 0, None, n, 0), 0, None, n, 1)
+    t = zo.field_op(zo.field_op(l, 0, None, r, 0, None, n, 3), 0, None, la, 0, None, n, 1)
+    exp = zo.field_op(h, 0, y, t, 0, None, n, 0)
+    th = dv.to_dev(h)
+    call("zk_permutation_eval_h_set", A.hold(th), A.put(z), A.put_all(vs), A.put_all(ss), nc, A.put(beta),
+         A.put(gamma), A.put(bd), A.put(X), A.put(la), A.put(y), rot, n, dv.stream_handle())
+    assert (dv.from_dev(th) == exp).all()

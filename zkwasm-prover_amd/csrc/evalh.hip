@@ -262,6 +262,38 @@ __global__ void k_shuffle_v2(Fr *res, const Fr *const *__restrict__ inputs, cons
 
 bool bad_n(int n) { return n <= 0 || (n & (n - 1)); }
 
+// One permutation set's product term on the extended coset, fused:
+// h = h*y + l_active * ( z(wX) * prod_c (v_c + beta*sigma_c + gamma)
+//                      - z(X)  * prod_c (v_c + gamma + bd_c * X) )
+// with bd_c = beta * delta^c and X the coset points.  The reference builds the
+// same term from field_op / permutation_eval_h_l and one extended NTT per
+// side and column (src/eval_h.rs:700-840); here sigma_c and X stay resident as
+// coset evaluations and one pass reads each operand once.
+constexpr int kMaxPermCols = 16;
+__global__ void k_perm_set_eval(Fr *h, const Fr *__restrict__ z, const Fr *const *__restrict__ v,
+                                const Fr *const *__restrict__ sig, int nc, const Fr *__restrict__ beta,
+                                const Fr *__restrict__ gamma, const Fr *__restrict__ bd, const Fr *__restrict__ X,
+                                const Fr *__restrict__ l_active, const Fr *__restrict__ yp, int rot, uint32_t n) {
+    __shared__ const Fr *sv[kMaxPermCols], *ss[kMaxPermCols];
+    if (threadIdx.x < (unsigned)nc) {
+        sv[threadIdx.x] = v[threadIdx.x];
+        ss[threadIdx.x] = sig[threadIdx.x];
+    }
+    __syncthreads();
+    const Fr b = ld(beta), g = ld(gamma), y = ld(yp);
+    GRID_LOOP(i, n) {
+        const uint32_t rn = (i + (uint32_t)rot) & (n - 1);
+        const Fr x = ld(&X[i]);
+        Fr l = ld(&z[rn]), r = ld(&z[i]);
+        for (int c = 0; c < nc; c++) {
+            const Fr vg = ld(&sv[c][i]) + g;
+            l = l * (vg + b * ld(&ss[c][i]));
+            r = r * (vg + ld(&bd[c]) * x);
+        }
+        st(&h[i], ld(&h[i]) * y + ld(&l_active[i]) * (l - r));
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -403,6 +435,23 @@ hipError_t shuffle_eval_h_v2(void *res, const void *inputs, const void *tables, 
                              const void *z, const void *l0, const void *l_last, const void *l_active_row,
                              const void *y, int rot, int n) {
     return zk_shuffle_eval_h_v2_on(res, inputs, tables, betas, group_len, z, l0, l_last, l_active_row, y, rot, n, 0);
+}
+
+// Extension (no reference entry; replaces the per-column field_op /
+// permutation_eval_h_l / extended-NTT sequence of src/eval_h.rs:700-840):
+// one permutation set's term folded into h in one pass.  v, sigma: device
+// arrays of nc <= 16 coset columns; bd: nc Fr (beta * delta^column);
+// X: the coset points; rot: the extended-domain shift of one row.
+hipError_t zk_permutation_eval_h_set(void *h, const void *z, const void *v, const void *sigma, int nc,
+                                     const void *beta, const void *gamma, const void *bd, const void *X,
+                                     const void *l_active, const void *y, int rot, int n, hipStream_t stream) {
+    if (bad_n(n) || !h || !z || !v || !sigma || nc <= 0 || nc > kMaxPermCols || !beta || !gamma || !bd || !X ||
+        !l_active || !y)
+        return hipErrorInvalidValue;
+    k_perm_set_eval<<<grid_for((uint32_t)n), kThreads, 0, stream>>>(
+        (Fr *)h, (const Fr *)z, (const Fr *const *)v, (const Fr *const *)sigma, nc, (const Fr *)beta,
+        (const Fr *)gamma, (const Fr *)bd, (const Fr *)X, (const Fr *)l_active, (const Fr *)y, rot, (uint32_t)n);
+    return hipGetLastError();
 }
 
 }  // extern "C"

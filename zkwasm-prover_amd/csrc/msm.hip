@@ -43,7 +43,7 @@ using namespace zk;
 
 namespace {
 
-constexpr uint32_t kL1 = 8;          // buckets per first-level reduction segment
+constexpr uint32_t kL1 = 4;          // buckets per first-level reduction segment
 constexpr uint32_t kChunk = 48;      // sorted entries per accumulation thread
 constexpr uint32_t kWalk = 32;       // chunk partials one fixup thread sums; longer buckets go to a workgroup
 constexpr uint32_t kNone = 0xffffffffu;
@@ -55,7 +55,7 @@ constexpr uint32_t kMagic = 0x4d534d32u;  // "MSM2": masked sums present in the 
 //   then            S[w]: W Jacobian window sums (device tail only)
 constexpr size_t kSliceQ = 128;
 // bytes of a slice the layout above can use: W * (nbits + 1) masked sums and
-// W window sums stay below 28800 B for every window 2 <= c <= 17
+// W window sums stay below 31200 B for every window 2 <= c <= 17 (kL1 = 4)
 constexpr size_t kSliceReserve = 32 * 1024;
 
 // Arrays placed into caller memory, in two lifetimes: phase 1 (the sort) and
@@ -298,28 +298,61 @@ __global__ __launch_bounds__(256) void k_msm_hist(const MsmItemDev *__restrict__
 }
 
 // per (tile, window): place each nonzero digit into its (item, window, bin)
-// group at starts[...] + rank; tmp entry = (low bucket bits, index | sign)
+// group; tmp entry = (low bucket bits, index | sign).  The tile is first
+// counting-sorted by bin in LDS (the bin counts are the histogram's), then
+// written out bin run by bin run, so consecutive lanes store to consecutive
+// addresses instead of 256 scattered 8-byte writes per wave.
 __global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__restrict__ items, uint32_t count,
                                                         uint32_t W, const uint32_t *__restrict__ digits,
                                                         uint32_t lo_bits, uint32_t nbins,
+                                                        const uint32_t *__restrict__ hist,
                                                         const uint32_t *__restrict__ starts, uint2 *__restrict__ tmp) {
-    __shared__ uint32_t cur[256];
+    __shared__ uint32_t gstart[256], loff[256], lcur[256];
+    __shared__ uint2 stage[kSortTile];
+    __shared__ uint32_t total;
     const uint32_t w = blockIdx.y;
     const uint32_t m = item_of(items, count, blockIdx.x, true);
     const MsmItemDev it = items[m];
     const uint32_t tl = blockIdx.x - it.tile0;
-    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) cur[k] = starts[hist_index(it, W, nbins, w, k, tl)];
+    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) {
+        const uint64_t hi = hist_index(it, W, nbins, w, k, tl);
+        gstart[k] = starts[hi];
+        lcur[k] = hist[hi];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < nbins; k++) {
+            const uint32_t c = lcur[k];
+            loff[k] = run;
+            lcur[k] = run;
+            run += c;
+        }
+        total = run;
+    }
     __syncthreads();
     const uint32_t *dw = digits + it.dig0 + (size_t)w * it.n;
     const uint32_t lo_mask = (1u << lo_bits) - 1;
-    uint32_t end = (tl + 1) * kSortTile < it.n ? (tl + 1) * kSortTile : it.n;
-    for (uint32_t i = tl * kSortTile + threadIdx.x; i < end; i += blockDim.x) {
+    const uint32_t begin = tl * kSortTile;
+    const uint32_t end = begin + kSortTile < it.n ? begin + kSortTile : it.n;
+    for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
         uint32_t d = dw[i];
         if (d) {
             uint32_t b = (d & 0x7fffffffu) - 1;
-            uint32_t pos = atomicAdd(&cur[b >> lo_bits], 1u);
-            tmp[pos] = make_uint2(b & lo_mask, i | (d & 0x80000000u));
+            uint32_t p = atomicAdd(&lcur[b >> lo_bits], 1u);
+            stage[p] = make_uint2(b & lo_mask, i | (d & 0x80000000u));
         }
+    }
+    __syncthreads();
+    const uint32_t tot = total;
+    for (uint32_t p = threadIdx.x; p < tot; p += blockDim.x) {
+        // bin of position p: the last k with loff[k] <= p (that bin is non-empty)
+        uint32_t lo = 0, hi = nbins;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (loff[mid] <= p) lo = mid; else hi = mid;
+        }
+        tmp[gstart[lo] + (p - loff[lo])] = stage[p];
     }
 }
 
@@ -870,7 +903,7 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     ZK_CHECK(launch_scan(hist, (uint32_t)p.nhist, starts, nullptr, scan_sums, stream));
     ZK_CHECK(prof_mark(prof, PH_SCATTER, stream));
     k_msm_scatter_hi<<<dim3((uint32_t)p.ntiles, p.W), 256, 0, stream>>>(items, count, p.W, digits, p.lo_bits,
-                                                                        p.nbins, starts, tmp);
+                                                                        p.nbins, hist, starts, tmp);
     ZK_CHECK(hipGetLastError());
     const uint32_t ngroups = count * p.W * p.nbins;
     k_msm_sort_lo<<<ngroups, 256, 0, stream>>>(items, p.W, p.nbins, tmp, starts, p.lo_bits, p.B, ngroups, entries,

@@ -121,7 +121,7 @@ bool make_shape(const zkp_circuit *c, Shape &s) {
     s.F = c->fixed < 2 ? 2 : c->fixed;
     s.I = c->instance < 0 ? 0 : c->instance;
     if (s.A < s.col_shuffle + 2 * s.S) return false;
-    if (2 * s.P + 2 * s.Sp + 24 > 512) return false;  // device pointer area of a job
+    if (4 * s.P + 2 * s.Sp + 24 > 512) return false;  // device pointer area of a job
     return true;
 }
 
@@ -141,8 +141,8 @@ struct Pk {
     size_t bytes = 0;
     void *g_lag, *g, *w_n, *w_n_inv, *w_e, *w_e_inv;
     uint8_t *cst;  // Fr constants
-    void *t_lag, *t_ext, *q_ext, *l0_ext, *llast_ext, *lact_ext, *ones_ext;
-    std::vector<void *> sig_lag, sig_coef;
+    void *t_lag, *t_ext, *q_ext, *l0_ext, *llast_ext, *lact_ext, *ones_ext, *x_ext;
+    std::vector<void *> sig_lag, sig_coef, sig_ext;
     uint32_t *perm_head, *perm_target;
     std::vector<uint32_t *> shuf_perm;
     void **sig_lag_dev;  // device array of P pointers
@@ -167,11 +167,14 @@ void pk_layout(Pk &pk, Arena &a) {
     pk.llast_ext = a.fr(s.N);
     pk.lact_ext = a.fr(s.N);
     pk.ones_ext = a.fr(s.N);
+    pk.x_ext = a.fr(s.N);
     pk.sig_lag.assign(s.P, nullptr);
     pk.sig_coef.assign(s.P, nullptr);
+    pk.sig_ext.assign(s.P, nullptr);
     for (int j = 0; j < s.P; j++) {
         pk.sig_lag[j] = a.fr(s.n);
         pk.sig_coef[j] = a.fr(s.n);
+        pk.sig_ext[j] = a.fr(s.N);
     }
     pk.perm_head = (uint32_t *)a.take((size_t)s.P * s.u * 4);
     pk.perm_target = (uint32_t *)a.take((size_t)s.P * s.u * 4);
@@ -324,8 +327,16 @@ hipError_t pk_build(Pk &pk, uint64_t seed, hipStream_t st) {
             PCHECK(hipMemcpyAsync(pk.sig_coef[j], pk.sig_lag[j], (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
             PCHECK(ntt(pk.sig_coef[j], tmp, nullptr, pk.w_n_inv, pk.cf(C_NINV), s.k, 8, &sw, st));
             if (sw) PCHECK(hipMemcpyAsync(pk.sig_coef[j], tmp, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
+            PCHECK(to_coset(pk, pk.sig_ext[j], pk.sig_coef[j], pk.scratch_e, &res, st));
+            if (res != pk.sig_ext[j])
+                PCHECK(hipMemcpyAsync(pk.sig_ext[j], res, (size_t)s.N * FR, hipMemcpyDeviceToDevice, st));
         }
     }
+    // the coset points X = zeta w_e^i: coset evaluations of the polynomial X
+    PCHECK(hipMemsetAsync(coef, 0, (size_t)s.n * FR, st));
+    PCHECK(hipMemcpyAsync((uint8_t *)coef + FR, pk.cf(C_ONE), FR, hipMemcpyDeviceToDevice, st));
+    PCHECK(to_coset(pk, pk.x_ext, coef, pk.scratch_e, &res, st));
+    if (res != pk.x_ext) PCHECK(hipMemcpyAsync(pk.x_ext, res, (size_t)s.N * FR, hipMemcpyDeviceToDevice, st));
     // shuffle row permutations of the first R rows
     for (int j = 0; j < s.S; j++) {
         std::vector<uint32_t> p(s.R);
@@ -357,7 +368,6 @@ struct Job {
     void *w_sum, *w_temp, *w_tab;
     std::vector<void *> ext, ext_tmp;  // coset buffers, NTT spares
     void *h, *h_tmp, *l_acc, *r_acc, *aux0, *aux1;
-    std::vector<void *> col_n;  // per-column coefficient scratch (2 * kChunk)
     uint8_t *cst;
     void **ptrs;  // device pointer arrays
     uint8_t *msm_scratch, *msm_out;
@@ -408,7 +418,6 @@ void job_layout(Job &j, Arena &a) {
     j.r_acc = a.fr(s.N);
     j.aux0 = a.fr(s.N);
     j.aux1 = a.fr(s.N);
-    cols(j.col_n, 2 * kChunk, s.n);
     j.cst = a.take((size_t)(J_COUNT_FIXED + 3 * s.P + 8) * FR);
     j.ptrs = (void **)a.take(4096);
     j.msm_bytes = msm_group_bytes(s);
@@ -696,6 +705,7 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
     // its rotations and the permutation sets; inputs / shuffles reuse the start
     void **pa = j.ptrs;
     void **pa_gate = pa + 2 * s.P + s.Sp, **pa_sets = pa_gate + 16 + 8;
+    void **pa_pv = pa_sets + s.Sp, **pa_ps = pa_pv + s.P;  // permutation columns' cosets
     int *pa_rots = (int *)(pa_gate + 16);
     if (s.P > 0) {
         std::vector<void *> host(2 * s.P + s.Sp);
@@ -796,52 +806,20 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         PCHECK(hipMemcpyAsync(pa_sets, sets.data(), (size_t)s.Sp * 8, hipMemcpyHostToDevice, st));
         PCHECK(zk_permutation_eval_h_p2_on(j.h, pa_sets, pk.l0_ext, pk.llast_ext, y, s.Sp, (int)s.u * rot1, N, st));
         PCHECK(hipStreamSynchronize(st));
-        // [gamma, beta * delta^c] per column: the identity side's first two
-        // coefficients
+        // per set: the fused product term (sigma and X resident as cosets)
+        std::vector<void *> cols(2 * s.P);
         for (int c = 0; c < s.P; c++) {
-            PCHECK(hipMemcpyAsync(j.cf(J_BDELTA + 2 * c), gamma, FR, hipMemcpyDeviceToDevice, st));
-            PCHECK(zk_fr_op(j.cf(J_BDELTA + 2 * c + 1), beta, pk.cf(C_DELTAS + c), OP_MUL, 1, st));
+            cols[c] = j.ext[E_V + c];
+            cols[s.P + c] = pk.sig_ext[c];
+            PCHECK(zk_fr_op(j.cf(J_BDELTA + c), beta, pk.cf(C_DELTAS + c), OP_MUL, 1, st));
         }
+        PCHECK(hipMemcpyAsync(pa_pv, cols.data(), cols.size() * 8, hipMemcpyHostToDevice, st));
         for (int t = 0; t < s.Sp; t++) {
             int c0 = t * kChunk, nc = s.P - c0 < kChunk ? s.P - c0 : kChunk;
-            // l_acc = z(wX), r_acc = z(X)
-            PCHECK(field_op(j.l_acc, j.ext[E_ZP + t], rot1, nullptr, nullptr, 0, nullptr, N, 2, st));
-            PCHECK(hipMemcpyAsync(j.r_acc, j.ext[E_ZP + t], (size_t)s.N * FR, hipMemcpyDeviceToDevice, st));
-            std::vector<const void *> coefs;
-            for (int c = 0; c < nc; c++) {
-                const void *v = j.adv_coef[s.col_perm + c0 + c];
-                void *ls = j.col_n[2 * c], *rs = j.col_n[2 * c + 1];
-                // sigma side: v + beta sigma + gamma (coefficients)
-                PCHECK(hipMemcpyAsync(ls, v, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
-                PCHECK(permutation_eval_h_l(ls, beta, gamma, pk.sig_coef[c0 + c], (int)s.n, st));
-                // identity side: v + gamma + beta delta^c X
-                PCHECK(hipMemcpyAsync(rs, v, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
-                PCHECK(field_op(rs, rs, 0, nullptr, j.cf(J_BDELTA + 2 * (c0 + c)), 0, nullptr, 2, 0, st));
-                coefs.push_back(ls);
-                coefs.push_back(rs);
-            }
-            // extend the 2 * nc coefficient polys: targets aux0, aux1 and the
-            // top spares, NTT scratch the bottom spares (2 * kChunk <= kExtGroup)
-            std::vector<zk_ntt_item> it(coefs.size());
-            std::vector<void *> dst(coefs.size());
-            for (size_t m = 0; m < coefs.size(); m++) {
-                dst[m] = m < 2 ? (m == 0 ? j.aux0 : j.aux1) : j.ext_tmp[kExtGroup - 1 - (m - 2)];
-                PCHECK(hipMemcpyAsync(dst[m], coefs[m], (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
-                PCHECK(extended_prepare(dst[m], pk.cf(C_FWD0), 3, (int)s.n, N, 0, st));
-            }
-            for (size_t m = 0; m < coefs.size(); m++)
-                it[m] = zk_ntt_item{dst[m], j.ext_tmp[m], pk.w_e, nullptr, s.ek, 0};
-            std::vector<unsigned char> sw(coefs.size());
-            PCHECK(zk_ntt_batch(it.data(), (int)coefs.size(), sw.data(), st));
-            for (size_t m = 0; m < coefs.size(); m++) {
-                const void *e = sw[m] ? j.ext_tmp[m] : dst[m];
-                void *acc = (m % 2 == 0) ? j.l_acc : j.r_acc;
-                PCHECK(field_op(acc, acc, 0, nullptr, e, 0, nullptr, N, 1, st));
-            }
-            PCHECK(field_op(j.l_acc, j.l_acc, 0, nullptr, j.r_acc, 0, nullptr, N, 3, st));
-            PCHECK(field_op(j.l_acc, j.l_acc, 0, nullptr, pk.lact_ext, 0, nullptr, N, 1, st));
-            PCHECK(field_op(j.h, j.h, 0, y, j.l_acc, 0, nullptr, N, 0, st));
+            PCHECK(zk_permutation_eval_h_set(j.h, j.ext[E_ZP + t], pa_pv + c0, pa_ps + c0, nc, beta, gamma,
+                                             j.cf(J_BDELTA + c0), pk.x_ext, pk.lact_ext, y, rot1, N, st));
         }
+        PCHECK(hipStreamSynchronize(st));  // the host pointer array
     }
     // logup: (t + beta), (f + beta), product-sum 1
     for (int l = 0; l < s.L; l++) {

@@ -44,6 +44,7 @@ SIGNATURES = {
     "zk_permutation_z_set": [P, P, P, I32, P, P, P, P, P, P, I32, P],
     "zk_permutation_chain": [P, I32, I32, I32, P, P],
     "zk_shuffle_z": [P, P, P, I32, P, P, I32, P],
+    "zk_permutation_eval_h_set": [P, P, P, P, I32, P, P, P, P, P, P, I32, I32, P],
     # element-wise / calibration entry points used by tests and bench
     "zk_fr_op": [P, P, P, I32, I32, P],
     "zk_fp_op": [P, P, P, I32, I32, P],
