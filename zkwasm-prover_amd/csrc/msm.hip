@@ -43,7 +43,7 @@ using namespace zk;
 
 namespace {
 
-constexpr uint32_t kL1 = 4;          // buckets per first-level reduction segment
+constexpr uint32_t kL1 = 8;          // buckets per first-level reduction segment
 constexpr uint32_t kChunk = 48;      // sorted entries per accumulation thread
 constexpr uint32_t kWalk = 32;       // chunk partials one fixup thread sums; longer buckets go to a workgroup
 constexpr uint32_t kNone = 0xffffffffu;
@@ -55,7 +55,7 @@ constexpr uint32_t kMagic = 0x4d534d32u;  // "MSM2": masked sums present in the 
 //   then            S[w]: W Jacobian window sums (device tail only)
 constexpr size_t kSliceQ = 128;
 // bytes of a slice the layout above can use: W * (nbits + 1) masked sums and
-// W window sums stay below 31200 B for every window 2 <= c <= 17 (kL1 = 4)
+// W window sums stay below 31200 B for every window 2 <= c <= 17 (kL1 >= 4)
 constexpr size_t kSliceReserve = 32 * 1024;
 
 // Arrays placed into caller memory, in two lifetimes: phase 1 (the sort) and
