@@ -296,6 +296,9 @@ hipError_t zk_msm_autotune(const void *points, const void *scalars, int n, int r
 hipError_t zk_fr_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
 hipError_t zk_fp_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
 hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
+/* the 29-bit-limb Fp of the bucket accumulation through its conversions:
+ * op 0 a*b, 1 a+b, 2 a-b, 3 a^2, 4 round trip (32-bit Montgomery in and out) */
+hipError_t zk_fp29_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
 hipError_t zk_bench_mul(void *buf, int threads, int block, int iters, hipStream_t stream);
 hipError_t zk_bench_mul_cios(void *buf, int threads, int block, int iters, hipStream_t stream);
 hipError_t zk_bench_dbl(void *buf, int threads, int block, int iters, hipStream_t stream);

@@ -245,3 +245,24 @@ def test_ntt_radix_choice_same_result(gpu, zo, max_deg):
     finally:
         L.zk_ntt_set_max_deg(log_n, 0)
     assert (got == zo.ntt(x, log_n, w)).all()
+
+
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 4])
+def test_fp29_ops_match_oracle(gpu, zo, op):
+    """the 29-bit-limb lazy Montgomery field of the bucket accumulation, through
+    its conversions from and to the 32-bit form, equals the oracle's Fp
+    arithmetic (edge values 0, 1, p-1, p-2 and 2^253 included)"""
+    from zkprover import device as dv
+    from zkprover._lib import call
+    rng = np.random.default_rng(29 + op)
+    xs = [0, 1, P - 1, P - 2, 1 << 253, P // 2] + [int.from_bytes(rng.bytes(32), "little") % P for _ in range(4090)]
+    ys = xs[::-1]
+    a = np.concatenate([zo.fp_to_mont_int(x) for x in xs])
+    b = np.concatenate([zo.fp_to_mont_int(y) for y in ys])
+    n = len(xs)
+    da, db, do = dv.to_dev(a), dv.to_dev(b), dv.empty(n)
+    call("zk_fp29_op", dv.ptr(do), dv.ptr(da), dv.ptr(db), op, n, dv.stream_handle())
+    got = dv.from_dev(do)
+    exp = {0: [x * y % P for x, y in zip(xs, ys)], 1: [(x + y) % P for x, y in zip(xs, ys)],
+           2: [(x - y) % P for x, y in zip(xs, ys)], 3: [x * x % P for x in xs], 4: xs}[op]
+    assert [zo.fp_from_mont_int(got[i]) for i in range(n)] == exp

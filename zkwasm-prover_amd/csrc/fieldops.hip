@@ -2,6 +2,7 @@
 // calibration kernels whose timings give the VALU roofline (Montgomery
 // multiplies per second, single-wave latency of a multiply, a doubling and a
 // mixed add) and the HBM copy baseline used to scale rocprof byte counters.
+#include "ff29.h"
 #include "zkcore.h"
 
 using namespace zk;
@@ -26,6 +27,25 @@ __global__ void k_fe_binop(Fe<P> *out, const Fe<P> *a, const Fe<P> *b, int op, u
     default: r = fe_from_mont<P>(x); break;
     }
     fe_store<P>(&out[i], r);
+}
+
+// The 29-bit-limb field through its conversions: op 0 a*b, 1 a+b, 2 a-b,
+// 3 a^2, 4 round trip of a; inputs and outputs in the 32-bit Montgomery form.
+__global__ void k_fp29_op(Fp *out, const Fp *a, const Fp *b, int op, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const F29 one = f29_one();
+    F29 x = f29_mul(f29_from32(fe_load<FpParams>(&a[i])), one);  // reduced, < 2p
+    F29 y = b ? f29_mul(f29_from32(fe_load<FpParams>(&b[i])), one) : x;
+    F29 r;
+    switch (op) {
+    case 0: r = f29_mul(x, y); break;
+    case 1: r = f29_add(x, y); break;
+    case 2: r = f29_sub(x, y); break;
+    case 3: r = f29_sqr(x); break;
+    default: r = x; break;
+    }
+    fe_store<FpParams>(&out[i], f29_to32(r));
 }
 
 // out[i] = a[i] + b[i] on G1, with a, b affine; op selects the formula path
@@ -172,6 +192,12 @@ hipError_t zk_fr_op(void *out, const void *a, const void *b, int op, int n, hipS
 hipError_t zk_fp_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
     if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
     k_fe_binop<FpParams><<<(n + 255) / 256, 256, 0, stream>>>((Fp *)out, (const Fp *)a, (const Fp *)b, op, n);
+    return hipGetLastError();
+}
+hipError_t zk_fp29_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
+    if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    if (!out || !a) return hipErrorInvalidValue;
+    k_fp29_op<<<(n + 255) / 256, 256, 0, stream>>>((Fp *)out, (const Fp *)a, (const Fp *)b, op, (uint32_t)n);
     return hipGetLastError();
 }
 hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {

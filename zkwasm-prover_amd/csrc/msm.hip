@@ -18,8 +18,9 @@
 //               short contiguous runs instead of 4-byte random scatters.
 //   3. accumulate  threads own equal chunks of the sorted entries (perfect
 //               load balance for any digit distribution); runs of one bucket
-//               are summed with XYZZ mixed additions; runs cut by a chunk
-//               border are left as head/tail partials.
+//               are summed with XYZZ mixed additions in 29-bit-limb lazy
+//               Montgomery arithmetic (ff29.h); runs cut by a chunk border
+//               are left as head/tail partials.
 //   4. fixup    partials of buckets that span chunks are merged.
 //   5. reduce   S_w = sum_b (b+1) B_{w,b}: per segment of L1 buckets a
 //               running sum gives (WS, SUM); then
@@ -36,6 +37,7 @@
 #include <thread>
 #include <vector>
 
+#include "ff29.h"
 #include "host_ec.h"
 #include "zkcore.h"
 
@@ -356,43 +358,95 @@ __global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__rest
     }
 }
 
-// per (item, window, bin) group: counting sort on the low bucket bits in LDS;
-// writes the final entries and the offsets of the group's buckets
+// exclusive scan of x[0..256) in LDS by 256 threads (x[k] = 0 past the used
+// range); t is a 256-entry LDS scratch
+__device__ __forceinline__ void lds_exclusive_scan256(uint32_t *x, uint32_t *t) {
+    const uint32_t i = threadIdx.x;
+    uint32_t v = x[i];
+    t[i] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < 256; d <<= 1) {
+        uint32_t add = i >= d ? t[i - d] : 0u;
+        __syncthreads();
+        t[i] += add;
+        __syncthreads();
+    }
+    x[i] = t[i] - v;
+    __syncthreads();
+}
+
+// per (item, window, bin) group: counting sort on the low bucket bits.  The
+// group histogram gives every bucket's offset; the entries are then placed
+// tile by tile (4096 at a time): each tile is counting-sorted in LDS and
+// written out bucket run by bucket run, so stores come in contiguous runs
+// rather than one 4-byte scatter per entry.  Also emits the bucket offsets.
+constexpr uint32_t kLoTile = 4096;
 __global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restrict__ items, uint32_t W, uint32_t nbins,
                                                      const uint2 *__restrict__ tmp,
                                                      const uint32_t *__restrict__ starts, uint32_t lo_bits,
                                                      uint32_t B, uint32_t ngroups, uint32_t *__restrict__ entries,
                                                      uint32_t *__restrict__ offsets) {
-    __shared__ uint32_t h[256];
-    __shared__ uint32_t cur[256];
+    __shared__ uint32_t h[256], cur[256], toff[256], tcur[256], sc[256];
+    __shared__ uint32_t stage[kLoTile];
+    __shared__ uint8_t skey[kLoTile];
     const uint32_t g = blockIdx.x;
     const uint32_t m = g / (W * nbins), r = g % (W * nbins);
     const uint32_t w = r / nbins, bin = r % nbins;
     const MsmItemDev it = items[m];
     const uint32_t nlo = 1u << lo_bits;
+    const uint32_t tid = threadIdx.x;
     const uint64_t idx = hist_index(it, W, nbins, w, bin, 0);
     const uint32_t gs = starts[idx];
     const uint32_t ge = starts[idx + it.ntiles];  // the next group's start (or the total)
-    for (uint32_t k = threadIdx.x; k < nlo; k += blockDim.x) h[k] = 0;
+    h[tid] = 0;
     __syncthreads();
-    for (uint32_t p = gs + threadIdx.x; p < ge; p += blockDim.x) atomicAdd(&h[tmp[p].x], 1u);
+    for (uint32_t p = gs + tid; p < ge; p += blockDim.x) atomicAdd(&h[tmp[p].x], 1u);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = gs;
-        for (uint32_t k = 0; k < nlo; k++) {
-            cur[k] = run;
-            run += h[k];
-        }
-    }
+    cur[tid] = h[tid];
+    __syncthreads();
+    lds_exclusive_scan256(cur, sc);
+    cur[tid] += gs;
     __syncthreads();
     const uint64_t base = ((uint64_t)m * W + w) * B + ((uint64_t)bin << lo_bits);
-    for (uint32_t k = threadIdx.x; k < nlo; k += blockDim.x) offsets[base + k] = cur[k];
-    if (g == ngroups - 1 && threadIdx.x == 0) offsets[base + nlo] = ge;  // = total: offsets[NB]
-    __syncthreads();
-    for (uint32_t p = gs + threadIdx.x; p < ge; p += blockDim.x) {
-        uint2 t = tmp[p];
-        uint32_t pos = atomicAdd(&cur[t.x], 1u);
-        entries[pos] = t.y;
+    if (tid < nlo) offsets[base + tid] = cur[tid];
+    if (g == ngroups - 1 && tid == 0) offsets[base + nlo] = ge;  // = total: offsets[NB]
+    for (uint32_t t0 = gs; t0 < ge; t0 += kLoTile) {
+        const uint32_t cnt = ge - t0 < kLoTile ? ge - t0 : kLoTile;
+        // the tile in registers: 16 entries per thread
+        uint2 v[kLoTile / 256];
+        h[tid] = 0;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kLoTile / 256; k++) {
+            const uint32_t q = k * 256 + tid;
+            if (q < cnt) {
+                v[k] = tmp[t0 + q];
+                atomicAdd(&h[v[k].x], 1u);
+            }
+        }
+        __syncthreads();
+        toff[tid] = h[tid];
+        __syncthreads();
+        lds_exclusive_scan256(toff, sc);
+        tcur[tid] = toff[tid];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kLoTile / 256; k++) {
+            const uint32_t q = k * 256 + tid;
+            if (q < cnt) {
+                const uint32_t at = atomicAdd(&tcur[v[k].x], 1u);
+                stage[at] = v[k].y;
+                skey[at] = (uint8_t)v[k].x;
+            }
+        }
+        __syncthreads();
+        for (uint32_t q = tid; q < cnt; q += blockDim.x) {
+            const uint32_t key = skey[q];
+            entries[cur[key] + (q - toff[key])] = stage[q];
+        }
+        __syncthreads();
+        cur[tid] += h[tid];
+        __syncthreads();
     }
 }
 
@@ -488,7 +542,12 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
         uint32_t b = find_bucket(off, NB, s);
         uint32_t bend = off[b + 1];
         bool first = true, before = off[b] < s;
-        G1Xyzz acc = xyzz_id();
+        // the bucket sum in 29-bit limbs (see ff29.h), converted at each store
+        X29 acc;
+        bool empty;
+        acc.x = f29_one();
+        acc.y = acc.x;
+        x29_clear(acc, empty);
         // item (and point array) of the entry being prefetched
         uint32_t m = b / WB;
         uint32_t mend = off[(m + 1) * WB];
@@ -512,21 +571,22 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
             const G1Aff qnext = more ? aff_load(&pts[enext & 0x7fffffffu]) : qcur;
             if (!aff_is_id(qcur)) {
                 Fp y = (ecur >> 31) ? fe_neg<FpParams>(qcur.y) : qcur.y;
-                acc = xyzz_add_aff(acc, qcur.x, y);
+                x29_acc_aff(acc, empty, f29_from32(qcur.x), f29_from32(y));
             }
             if (pos + 1 == bend || !more) {
                 // the run of bucket b ends at this step
+                const G1Xyzz sum = x29_to_xyzz(acc, empty);
                 if (first && before) {
-                    xyzz_store(&phead[t], acc);
+                    xyzz_store(&phead[t], sum);
                     head_bucket = b;
                 } else if (bend > e) {
-                    xyzz_store(&ptail[t], acc);
+                    xyzz_store(&ptail[t], sum);
                     tail_bucket = b;
                 } else {
-                    xyzz_store(&buckets[b], acc);
+                    xyzz_store(&buckets[b], sum);
                 }
                 first = false;
-                acc = xyzz_id();
+                x29_clear(acc, empty);
                 if (more) {
                     // next non-empty bucket starting at pos + 1
                     const uint32_t np = pos + 1;
