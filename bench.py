@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-proof", action="store_true")
+    ap.add_argument("--proof-ks", default="20,22")
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--serve-count", type=int, default=16)
     return ap.parse_args()
@@ -197,7 +198,7 @@ def main():
     roofline = {
         "bound": "valu", "kernel": "k_msm_accumulate", "achieved": round(achieved, 3),
         "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
-        "traffic": None, "avg_launch_ms": round(acc_ms, 4), "window_bits": c,
+        "traffic": _pmc_traffic("k_msm_accumulate"), "avg_launch_ms": round(acc_ms, 4), "window_bits": c,
         "algorithmic": f"{units} bucket additions x {MULS_PER_BUCKET_ADD} Fp mont-mul x "
                        f"{OPS_PER_MONT_MUL} 32-bit mad ops",
     }
@@ -218,7 +219,7 @@ def main():
         out["batched"] = {"msm": bench_batched_msm(L, points, max(1, args.steps // 2)),
                           "ntt": bench_batched_ntt(L, args.steps)}
     if not args.no_proof:
-        out["proof_core"] = bench_proof_core([20, 22])
+        out["proof_core"] = bench_proof_core([int(k) for k in args.proof_ks.split(",") if k])
     if args.serve:
         out["serving"] = bench_serving(args.serve_count)
     if not args.no_cpu:
@@ -443,6 +444,23 @@ def _root_of_unity_int(log_n):
 def _to_mont_limbs(v):
     m = (v << 256) % _R
     return [(m >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(4)]
+
+
+def _pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads
+    for a plain 256 MiB copy); None when no summary is present"""
+    path = os.path.join(ROOT, "profiles", "msm_2e22_pmc_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel:
+            try:
+                return int(float(parts[-1]))
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(n):

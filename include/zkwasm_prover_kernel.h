@@ -299,6 +299,8 @@ hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipS
 /* the 29-bit-limb Fp of the bucket accumulation through its conversions:
  * op 0 a*b, 1 a+b, 2 a-b, 3 a^2, 4 round trip (32-bit Montgomery in and out) */
 hipError_t zk_fp29_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
+/* 29-bit-limb multiply throughput (calibration): iters x 2 products per thread */
+hipError_t zk_bench_mul29(void *buf, int threads, int block, int iters, int variant, hipStream_t stream);
 hipError_t zk_bench_mul(void *buf, int threads, int block, int iters, hipStream_t stream);
 hipError_t zk_bench_mul_cios(void *buf, int threads, int block, int iters, hipStream_t stream);
 hipError_t zk_bench_dbl(void *buf, int threads, int block, int iters, hipStream_t stream);

@@ -86,6 +86,24 @@ __global__ void k_bench_mul(Fr *buf, uint32_t iters) {
     }
     fe_store<FrParams>(&buf[i], x);
 }
+// 29-bit-limb multiply throughput: variant 0 one chain per column, 1 split chains
+template <int V>
+__global__ void k_bench_mul29(Fp *buf, uint32_t iters) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const F29 one = f29_one();
+    F29 x = f29_mul(f29_from32(fe_load<FpParams>(&buf[i])), one);
+    F29 y = f29_mul(f29_from32(fe_load<FpParams>(&buf[i ^ 1])), one);
+    for (uint32_t k = 0; k < iters; k++) {
+        if (V == 0) {
+            x = f29_mul_chain(x, y);
+            y = f29_mul_chain(y, x);
+        } else {
+            x = f29_mul(x, y);
+            y = f29_mul(y, x);
+        }
+    }
+    fe_store<FpParams>(&buf[i], f29_to32(x));
+}
 __global__ void k_bench_mul_cios(Fr *buf, uint32_t iters) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Fr x = fe_load<FrParams>(&buf[i]);
@@ -208,6 +226,14 @@ hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipS
 hipError_t zk_bench_mul(void *buf, int threads, int block, int iters, hipStream_t stream) {
     if (threads % block) return hipErrorInvalidValue;
     k_bench_mul<<<threads / block, block, 0, stream>>>((Fr *)buf, iters);
+    return hipGetLastError();
+}
+hipError_t zk_bench_mul29(void *buf, int threads, int block, int iters, int variant, hipStream_t stream) {
+    if (threads % block) return hipErrorInvalidValue;
+    if (variant)
+        k_bench_mul29<1><<<threads / block, block, 0, stream>>>((Fp *)buf, iters);
+    else
+        k_bench_mul29<0><<<threads / block, block, 0, stream>>>((Fp *)buf, iters);
     return hipGetLastError();
 }
 hipError_t zk_bench_mul_cios(void *buf, int threads, int block, int iters, hipStream_t stream) {

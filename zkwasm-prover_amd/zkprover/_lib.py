@@ -50,6 +50,7 @@ SIGNATURES = {
     "zk_fp_op": [P, P, P, I32, I32, P],
     "zk_g1_op": [P, P, P, I32, I32, P],
     "zk_fp29_op": [P, P, P, I32, I32, P],
+    "zk_bench_mul29": [P, I32, I32, I32, I32, P],
     "zk_bench_mul": [P, I32, I32, I32, P],
     "zk_bench_dbl": [P, I32, I32, I32, P],
     "zk_bench_madd": [P, P, I32, I32, I32, I32, P],
