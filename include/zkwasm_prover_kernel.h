@@ -94,6 +94,31 @@ hipError_t eval_logup_z_compose(void *z, void *input, void *table, const void *m
 hipError_t eval_logup_z_pure(void *z, void *input, void *table, void *last_z, int last_z_index, int n,
                              hipStream_t stream);
 
+/* ---- lookup multiplicities m(X) (bn254_c.rs:281, :295) ----------------- */
+
+/* bn254_c.rs:281 `prepare_table_lookup`: indexes the usable rows
+ * [0, unusable_rows_start) of table (n = 2^k Fr) for calc_m.  Every buffer
+ * argument is n Fr as the reference's caller passes (src/lib.rs:407-412);
+ * start_offset receives the index (a hash table of 8n u32 slots) and must be
+ * kept for the calc_m calls of this table; the other buffers are unused.
+ * Returns hipErrorInvalidValue for k outside [1, 27] or
+ * unusable_rows_start > n. */
+hipError_t prepare_table_lookup(void *table, void *table_lowest_u32, void *sorted_table_lowest_u32, void *index,
+                                void *sorted_index, void *start_offset, void *candidate_sort_temp_storage,
+                                unsigned candidate_sort_temp_storage_bytes, unsigned k, unsigned unusable_rows_start,
+                                hipStream_t stream);
+
+/* bn254_c.rs:295 `calc_m`: for each usable input row i, the matched table row
+ * j is the largest usable row with table[j] == input[i] (Montgomery words
+ * compared); m[j] += (number of inputs matched to j).  m accumulates across
+ * calls (the caller zeroes it once per lookup).  matched_index (n Fr) is
+ * scratch; the first u32 of sorted_matched_index is set to 1 when some usable
+ * input has no usable table row equal to it (the reference asserts), else 0. */
+hipError_t calc_m(void *m, void *table, void *input, const void *sorted_index, const void *start_offset,
+                  void *matched_index, void *sorted_matched_index, void *candidate_sort_temp_storage,
+                  unsigned candidate_sort_temp_storage_bytes, unsigned k, unsigned unusable_rows_start,
+                  hipStream_t stream);
+
 /* ---- eval_h evaluator and argument kernels ---------------------------- */
 
 /* bn254_c.rs:18 `field_op`: res[i] = L op R with L = l[(i+l_rot) mod n]
@@ -154,7 +179,42 @@ hipError_t shuffle_eval_h_v2(void *res, const void *inputs, const void *tables, 
                              const void *z, const void *l0, const void *l_last, const void *l_active_row,
                              const void *y, int rot, int n);
 
+/* ---- multi-open (bn254_c.rs:200, :208, :218) ---------------------------- */
+
+/* bn254_c.rs:200 `poly_eval`: res[0] = p(x) = sum_i p[i] x^i for the n
+ * coefficients p, with x = x_table[0] (the caller's table holds x^(2^j),
+ * src/eval_poly.rs:44-53; only its first entry is read).  res (n Fr) is
+ * scratch; any n >= 1 (the reference needs a power of two and leaves res
+ * unwritten for n = 1). */
+hipError_t poly_eval(const void *p, void *res, const void *x, int n, hipStream_t stream);
+
+/* bn254_c.rs:208 `shplonk_h_x_merge`: res[i] = res[i]*v
+ * + values[i] * prod_{j < diff_points_n} (omegas[i] - diff_points[j]).
+ * Legacy default stream, as the reference. */
+hipError_t shplonk_h_x_merge(void *res, const void *v, const void *values, const void *omegas,
+                             const void *diff_points, int diff_points_n, int n);
+
+/* bn254_c.rs:218 `shplonk_h_x_div_points`: values[i] /= prod_{j < points_n}
+ * (omegas[i] - points[j]), points_n >= 1.  A zero product (omegas[i] equal to
+ * a point; the reference asserts) leaves values[i] = 0.  Legacy default
+ * stream. */
+hipError_t shplonk_h_x_div_points(void *values, const void *omegas, const void *points, int points_n, int n);
+
 /* ---- extensions: no counterpart in the reference's C ABI --------------- */
+
+/* Division by (X - z) of the n coefficients in `in` (src/multiopen.rs:145-159
+ * for GWC, :543-553 for Shplonk, done there on the host): out[i] = q_i for
+ * i <= n-2 with in(X) = (X - z) q(X) + in(z), and out[n-1] = 0 (n = 1 copies
+ * the single entry, as the reference's loop does).  out may equal in; z is a
+ * device Fr; eval (device Fr, may be NULL) receives in(z).  temp:
+ * zk_kate_divide_scratch_bytes(n) bytes of device memory. */
+size_t zk_kate_divide_scratch_bytes(int n);
+hipError_t zk_kate_divide(void *out, const void *in, const void *z, void *eval, int n, void *temp,
+                          hipStream_t stream);
+hipError_t zk_shplonk_h_x_merge_on(void *res, const void *v, const void *values, const void *omegas,
+                                   const void *diff_points, int diff_points_n, int n, hipStream_t stream);
+hipError_t zk_shplonk_h_x_div_points_on(void *values, const void *omegas, const void *points, int points_n, int n,
+                                        hipStream_t stream);
 
 /* Stream-taking forms of the reference entries that run on the legacy
  * default stream (expand_omega_buffer, field_mul_zip, permutation_eval_h_p1,

@@ -334,6 +334,54 @@ def eval_logup_z_pure(z_in, last_z, last_z_index):
     return z, last_z
 
 
+# ---- lookup multiplicity m(X) -------------------------------------------------
+def lookup_multiplicity(m, table, inputs, k, unusable_rows_start):
+    """Restates prepare_table_lookup + calc_m (cuda/bn254.cu:1637-1912) for
+    each input of `inputs` in turn; returns (new m, missing).
+
+    prepare (bn254.cu:1867-1912): key[j] = (j >= u) << (k+2) | (low k+2 bits
+    of Montgomery word 0 of table[j]) (:1645); a stable sort of (key, j)
+    (cub SortPairs, :1880-1897); start_offset[key] = the last sorted position
+    holding key (:1649-1669).  calc_m (:1794-1865): for a usable input row,
+    scan sorted positions downwards from start_offset[its key] and take the
+    first table row with equal words (:1671-1700) -- so the largest usable
+    row holding that value; rows past u never match since their keys sort
+    above every usable key.  The matched rows are then counted and
+    m[j] += count (:1702-1792).  `missing` is True when some usable input has
+    no match (the reference's device assert, :1694).  Python loops: small n.
+    """
+    n = 1 << k
+    table = np.ascontiguousarray(table, dtype=np.uint64)
+    m = np.array(m, dtype=np.uint64, copy=True, order="C")
+    mask = (1 << (k + 2)) - 1
+    rows = np.arange(n, dtype=np.uint64)
+    low = (table[:, 0] & np.uint64(0xFFFFFFFF)).astype(np.uint64)
+    key = ((rows >= np.uint64(unusable_rows_start)).astype(np.uint64) << np.uint64(k + 2)) | (low & np.uint64(mask))
+    order = np.argsort(key, kind="stable")
+    skey = key[order]
+    start_offset = {}
+    for pos in range(n):
+        start_offset[int(skey[pos])] = pos  # the last position wins
+    missing = False
+    for inp in inputs:
+        inp = np.ascontiguousarray(inp, dtype=np.uint64)
+        counts = {}
+        for i in range(unusable_rows_start):
+            pos = start_offset.get(int(inp[i, 0]) & mask, -1)
+            while pos >= 0:
+                j = int(order[pos])
+                if (inp[i] == table[j]).all():
+                    counts[j] = counts.get(j, 0) + 1
+                    break
+                pos -= 1
+            else:
+                missing = True
+        for j, c in counts.items():
+            if j < unusable_rows_start:
+                m[j] = fr_add(m[j], fr_from_u64(c))[0]
+    return m, missing
+
+
 # ---- grand products ----------------------------------------------------------
 def permutation_z_set(values, sigmas, beta, gamma, delta_start, delta, omega):
     n = len(values[0])
@@ -472,3 +520,44 @@ def vanishing_construct(h, t_evals, log_n, log_ext, ext_omega_inv, ext_divisor, 
     for i in range(q - 2, -1, -1):
         last = field_op(last, 0, xn, np.ascontiguousarray(x[i * n:(i + 1) * n]), 0, None, n, 0)
     return x, commits, last
+
+
+# ---- multi-open -----------------------------------------------------------------
+def x_table(x, k):
+    """[x, x^2, x^4, ..., x^(2^(k-1))] as src/eval_poly.rs:44-53 builds it"""
+    out = [np.ascontiguousarray(x, dtype=np.uint64).reshape(1, 4)]
+    for _ in range(1, max(k, 1)):
+        out.append(fr_mul(out[-1], out[-1]))
+    return np.concatenate(out)
+
+
+def poly_eval(p, xt):
+    """cuda/bn254.cu:1608-1633 (rounds of <= 4 bits); returns p(x) as (1, 4)"""
+    p = np.ascontiguousarray(p, dtype=np.uint64)
+    res = fr_zeros(len(p))
+    lib().zo_poly_eval(_p(res), _p(p), _p(np.ascontiguousarray(xt, dtype=np.uint64)), ctypes.c_size_t(len(p)))
+    return res[:1].copy()
+
+
+def kate_divide(poly, z):
+    """src/multiopen.rs:145-159: in-place division by (X - z)"""
+    out = np.array(poly, dtype=np.uint64, copy=True, order="C")
+    lib().zo_kate_divide(_p(out), _p(np.ascontiguousarray(z, dtype=np.uint64)), ctypes.c_size_t(len(out)))
+    return out
+
+
+def shplonk_h_x_merge(res, v, values, omegas, diff):
+    out = np.array(res, dtype=np.uint64, copy=True, order="C")
+    diff = np.ascontiguousarray(diff, dtype=np.uint64).reshape(-1, 4)
+    lib().zo_shplonk_h_x_merge(_p(out), _p(np.ascontiguousarray(v)), _p(np.ascontiguousarray(values)),
+                               _p(np.ascontiguousarray(omegas)), _p(diff) if len(diff) else None,
+                               ctypes.c_int(len(diff)), ctypes.c_size_t(len(out)))
+    return out
+
+
+def shplonk_h_x_div_points(values, omegas, points):
+    out = np.array(values, dtype=np.uint64, copy=True, order="C")
+    points = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1, 4)
+    lib().zo_shplonk_h_x_div_points(_p(out), _p(np.ascontiguousarray(omegas)), _p(points),
+                                    ctypes.c_int(len(points)), ctypes.c_size_t(len(out)))
+    return out

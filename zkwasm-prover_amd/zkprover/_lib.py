@@ -37,10 +37,18 @@ SIGNATURES = {
     "shuffle_eval_h_v2": [P, P, P, P, I32, P, P, P, P, P, I32, I32],
     "eval_logup_z_compose": [P, P, P, P, P, P, I32, I32, P],
     "eval_logup_z_pure": [P, P, P, P, I32, I32, P],
+    "prepare_table_lookup": [P, P, P, P, P, P, P, U32, U32, U32, P],
+    "calc_m": [P, P, P, P, P, P, P, P, U32, U32, U32, P],
+    "poly_eval": [P, P, P, I32, P],
+    "shplonk_h_x_merge": [P, P, P, P, P, I32, I32],
+    "shplonk_h_x_div_points": [P, P, P, I32, I32],
     "msm": [P, P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P],
     "batch_msm_collect": [P, P, P, P, P, U32, U32, U32, U32, P],
     # extensions (no reference counterpart)
     "zk_batch_invert": [P, P, I32, P],
+    "zk_kate_divide": [P, P, P, P, I32, P, P],
+    "zk_shplonk_h_x_merge_on": [P, P, P, P, P, I32, I32, P],
+    "zk_shplonk_h_x_div_points_on": [P, P, P, I32, I32, P],
     "zk_permutation_z_set": [P, P, P, I32, P, P, P, P, P, P, I32, P],
     "zk_permutation_chain": [P, I32, I32, I32, P, P],
     "zk_shuffle_z": [P, P, P, I32, P, P, I32, P],
