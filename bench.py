@@ -123,6 +123,23 @@ def main():
 
     buf_a, buf_b = u8(N * 32), u8(N * 32)
     inv_buf, inv_tmp = u8(n * 32), u8(n * 32)
+
+    # lookup multiplicity: the benchmark circuit's range table (row j holds
+    # j mod 2^16) and one input of uniform range values; buffers as the
+    # reference's caller allocates them (src/lib.rs:407-412)
+    range_vals = np.array([_to_mont_limbs(v) for v in range(1 << 16)], dtype=np.uint64)
+    lk_table = dv.to_dev(range_vals[np.arange(n) % (1 << 16)])
+    lk_input = dv.to_dev(range_vals[np.random.default_rng(5).integers(0, 1 << 16, size=n)])
+    lk_m, lk_so, lk_t0, lk_t1 = u8(n * 32), u8(n * 32), u8(n * 32), u8(n * 32)
+    lk_u = n - 6
+
+    def run_prepare_table():
+        call("prepare_table_lookup", lk_table.data_ptr(), None, None, None, None, lk_so.data_ptr(), None, 0,
+             LOG_N, lk_u, st)
+
+    def run_calc_m():
+        call("calc_m", lk_m.data_ptr(), lk_table.data_ptr(), lk_input.data_ptr(), None, lk_so.data_ptr(),
+             lk_t0.data_ptr(), lk_t1.data_ptr(), None, 0, LOG_N, lk_u, st)
     swap = ctypes.c_bool(False)
 
     def run_msm():
@@ -150,7 +167,9 @@ def main():
              ("intt_2^22", restore(buf_a, ntt_in), lambda: run_ntt(LOG_N, om_n_inv, n_inv)),
              ("coset_ntt_2^22_to_2^24", restore(buf_a, coset_in), run_coset),
              ("batch_inversion_2^22", restore(inv_buf, inv_in),
-              lambda: call("zk_batch_invert", inv_buf.data_ptr(), inv_tmp.data_ptr(), n, st))]
+              lambda: call("zk_batch_invert", inv_buf.data_ptr(), inv_tmp.data_ptr(), n, st)),
+             ("prepare_table_lookup_2^22", None, run_prepare_table),
+             ("calc_m_2^22", None, run_calc_m)]
 
     # one plain copy of 256 MiB: the calibration dispatch for the HBM byte
     # counters (FETCH_SIZE / WRITE_SIZE) when this script runs under rocprofv3
@@ -209,7 +228,8 @@ def main():
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32 limbs (exact 256-bit Montgomery)", "data": "synthetic",
         "config": {"workload": "zkWasm-size primitives: BN254 G1 MSM n=2^22; Fr NTT and iNTT 2^22; "
-                               "coset NTT 2^22->2^24; batch inversion 2^22", "msm_window_bits": c,
+                               "coset NTT 2^22->2^24; batch inversion 2^22; lookup multiplicity "
+                               "(range table 2^16 in 2^22 rows, one input)", "msm_window_bits": c,
                    "parallelism": "single GPU"},
         "primitives_ms": {k: round(v, 4) for k, v in prim_ms.items()},
         "msm_phases_ms": {k: round(v, 4) for k, v in phases.items()},

@@ -833,3 +833,80 @@ void zo_shuffle_eval_h_v2(zo_fr *res, const zo_fr *const *inputs, const zo_fr *c
         res[i] = t;
     }
 }
+
+/* ---- multi-open ------------------------------------------------------------ */
+/* cuda/bn254.cu:1608-1633 with _poly_eval :238-256: rounds of at most 4 bits;
+ * a round with step 2^deg folds each group of `count` = 2^cur entries at
+ * stride 2^deg by Horner in x_table[deg] = x^(2^deg) into its first entry.
+ * res (n entries) is the work area; res[0] = p(x) at the end (n >= 2; for
+ * n = 1 the reference writes nothing, here res[0] = p[0]). */
+void zo_poly_eval(zo_fr *res, const zo_fr *p, const zo_fr *x_table, size_t n) {
+    if (n == 1) { res[0] = p[0]; return; }
+    int deg = 0, remain = __builtin_ctzll(n);
+    size_t m = n;
+    while (m > 1) {
+        int cur = remain < 4 ? remain : 4;
+        size_t count = (size_t)1 << cur, groups = m >> cur, step = (size_t)1 << deg;
+        const zo_fr *src = deg == 0 ? p : res;
+        zo_fr x = x_table[deg];
+        for (size_t g = 0; g < groups; g++) {
+            size_t start = (g * count) << deg;
+            size_t end = start + ((count - 1) << deg);
+            zo_fr acc = src[end];
+            for (size_t i = end - step; i > start; i -= step) {
+                fr_mul(&acc, &acc, &x);
+                fr_add(&acc, &acc, &src[i]);
+            }
+            fr_mul(&acc, &acc, &x);
+            fr_add(&res[start], &acc, &src[start]);
+        }
+        m >>= cur;
+        deg += cur;
+        remain -= cur;
+    }
+}
+/* src/multiopen.rs:145-159 (GWC) and :543-553 (Shplonk): in-place division
+ * by (X - z): poly[i] = q_i for i <= n-2, poly[n-1] = 0 (the remainder, and so
+ * poly[0] before the call, is dropped). */
+void zo_kate_divide(zo_fr *poly, const zo_fr *z, size_t n) {
+    if (n < 2) return; /* the loop leaves a single entry as it is */
+    zo_fr tmp = poly[n - 1];
+    memset(&poly[n - 1], 0, sizeof(zo_fr));
+    for (size_t i = n - 2; i >= 1; i--) {
+        zo_fr t, q;
+        fr_mul(&t, &tmp, z);
+        fr_add(&q, &poly[i], &t);
+        poly[i] = tmp;
+        tmp = q;
+    }
+    poly[0] = tmp;
+}
+/* cuda/bn254.cu:1133-1151 */
+void zo_shplonk_h_x_merge(zo_fr *res, const zo_fr *v, const zo_fr *values, const zo_fr *omegas,
+                          const zo_fr *diff, int diff_n, size_t n) {
+#pragma omp parallel for OMP_THREADS if (n >= 4096)
+    for (size_t i = 0; i < n; i++) {
+        zo_fr t = values[i];
+        for (int j = 0; j < diff_n; j++) {
+            zo_fr d; fr_sub(&d, &omegas[i], &diff[j]);
+            fr_mul(&t, &t, &d);
+        }
+        zo_fr r; fr_mul(&r, &res[i], v);
+        fr_add(&res[i], &r, &t);
+    }
+}
+/* cuda/bn254.cu:1153-1170: values[i] *= 1 / prod_j (omegas[i] - points[j]);
+ * the reference asserts the product is nonzero, here a zero product gives 0 */
+void zo_shplonk_h_x_div_points(zo_fr *values, const zo_fr *omegas, const zo_fr *points,
+                               int points_n, size_t n) {
+#pragma omp parallel for OMP_THREADS if (n >= 1024)
+    for (size_t i = 0; i < n; i++) {
+        zo_fr t; fr_sub(&t, &omegas[i], &points[0]);
+        for (int j = 1; j < points_n; j++) {
+            zo_fr d; fr_sub(&d, &omegas[i], &points[j]);
+            fr_mul(&t, &t, &d);
+        }
+        zo_fr inv; zo_fr_inv(&inv, &t);
+        fr_mul(&values[i], &values[i], &inv);
+    }
+}

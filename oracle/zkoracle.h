@@ -159,6 +159,14 @@ void zo_shuffle_eval_h_v2(zo_fr *res, const zo_fr *const *inputs, const zo_fr *c
                           const zo_fr *l_last, const zo_fr *l_active_row, const zo_fr *y,
                           int rot, size_t n);
 
+/* multi-open */
+void zo_poly_eval(zo_fr *res, const zo_fr *p, const zo_fr *x_table, size_t n);
+void zo_kate_divide(zo_fr *poly, const zo_fr *z, size_t n);
+void zo_shplonk_h_x_merge(zo_fr *res, const zo_fr *v, const zo_fr *values, const zo_fr *omegas,
+                          const zo_fr *diff, int diff_n, size_t n);
+void zo_shplonk_h_x_div_points(zo_fr *values, const zo_fr *omegas, const zo_fr *points,
+                               int points_n, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

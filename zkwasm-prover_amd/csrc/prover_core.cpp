@@ -4,7 +4,7 @@
 // evaluate_h_gates_and_vanishing_construct (src/eval_h.rs:410-1196):
 //   1. advice commitments in the Lagrange basis, then iNTT to coefficients
 //      (batch_msm_and_intt_ext, src/cuda/msm.rs:73-237)
-//   2. lookup multiplicities m(X): histogram of the inputs over the table,
+//   2. lookup multiplicities m(X): prepare_table_lookup + calc_m per input,
 //      commitments, iNTT (src/lib.rs:404-530)
 //   3. grand products after beta, gamma: logup z (logup_sum_input_inv +
 //      eval_logup_z_compose, src/lib.rs:590-745), permutation z per chunk of
@@ -364,7 +364,7 @@ struct Job {
     std::vector<void *> adv_lag, adv_coef, adv_tmp;
     std::vector<void *> m_lag, m_coef, m_tmp, zl_lag, zl_coef, zl_tmp, zp_lag, zp_coef, zp_tmp, zs_lag, zs_coef,
         zs_tmp;
-    uint32_t *vals, *cnt;
+    uint32_t *vals;
     void *w_sum, *w_temp, *w_tab;
     std::vector<void *> ext, ext_tmp;  // coset buffers, NTT spares
     void *h, *h_tmp, *l_acc, *r_acc, *aux0, *aux1;
@@ -406,7 +406,6 @@ void job_layout(Job &j, Arena &a) {
     cols(j.zs_coef, s.S, s.n);
     cols(j.zs_tmp, s.S, s.n);
     j.vals = (uint32_t *)a.take((size_t)s.u * 4);
-    j.cnt = (uint32_t *)a.take((size_t)s.tsize * 4);
     j.w_sum = a.fr(s.n);
     j.w_temp = a.fr(s.n);
     j.w_tab = a.fr(s.n);
@@ -678,11 +677,16 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
     PCHECK(mark(1));
 
     // ---- 2. lookup multiplicities -------------------------------------------
+    // as src/lib.rs:404-504: index the table, then count each lookup's input
+    // (one table column serves every lookup of this circuit); the grand-product
+    // buffers are free until phase 3 and hold the index and the counters
+    if (s.L)
+        PCHECK(prepare_table_lookup(pk.t_lag, nullptr, nullptr, nullptr, nullptr, j.w_temp, nullptr, 0, s.k, s.u,
+                                    st));
     for (int l = 0; l < s.L; l++) {
-        PCHECK(zk::wit_lookup_values(j.vals, s.u, s.tsize, seed * 7919ull + l, st));
-        PCHECK(hipMemsetAsync(j.cnt, 0, (size_t)s.tsize * 4, st));
-        PCHECK(zk::wit_hist(j.cnt, j.vals, s.u, st));
-        PCHECK(zk::wit_mult_rows(j.m_lag[l], j.cnt, s.n, s.tsize, st));
+        PCHECK(zk::wit_zero_rows(j.m_lag[l], 0, s.n, st));
+        PCHECK(calc_m(j.m_lag[l], pk.t_lag, j.adv_lag[s.col_lookup + l], nullptr, j.w_temp, j.w_sum, j.w_tab,
+                      nullptr, 0, s.k, s.u, st));
     }
     if (s.L) {
         std::vector<const void *> polys(j.m_lag.begin(), j.m_lag.end());

@@ -35,21 +35,9 @@ __global__ void k_lookup_values(uint32_t *vals, uint32_t u, uint32_t tsize, uint
     ROWS(i, u) vals[i] = (uint32_t)(mix64(seed ^ ((uint64_t)i * 0x9e3779b97f4a7c15ull)) % tsize);
 }
 
-__global__ void k_hist(uint32_t *cnt, const uint32_t *__restrict__ vals, uint32_t u) {
-    ROWS(i, u) atomicAdd(&cnt[vals[i]], 1u);
-}
-
 // col[i] = i mod tsize for i < u (Montgomery)
 __global__ void k_table_rows(Fr *col, uint32_t u, uint32_t tsize) {
     ROWS(i, u) fe_store<FrParams>(&col[i], fe_from_u32<FrParams>(i % tsize));
-}
-
-// m[i] = cnt[i] for i < tsize, 0 elsewhere
-__global__ void k_mult_rows(Fr *m, const uint32_t *__restrict__ cnt, uint32_t n, uint32_t tsize) {
-    ROWS(i, n) {
-        uint32_t c = i < tsize ? cnt[i] : 0u;
-        fe_store<FrParams>(&m[i], fe_from_u32<FrParams>(c));
-    }
 }
 
 __global__ void k_gate_rows(Fr *d, const Fr *__restrict__ a, const Fr *__restrict__ b, const Fr *__restrict__ c,
@@ -125,18 +113,9 @@ hipError_t wit_lookup_values(uint32_t *vals, uint32_t u, uint32_t tsize, uint64_
     k_lookup_values<<<blocks_for(u), kT, 0, s>>>(vals, u, tsize, seed);
     return hipGetLastError();
 }
-hipError_t wit_hist(uint32_t *cnt, const uint32_t *vals, uint32_t u, hipStream_t s) {
-    if (!u) return hipSuccess;
-    k_hist<<<blocks_for(u), kT, 0, s>>>(cnt, vals, u);
-    return hipGetLastError();
-}
 hipError_t wit_table_rows(void *col, uint32_t u, uint32_t tsize, hipStream_t s) {
     if (!u || !tsize) return hipErrorInvalidValue;
     k_table_rows<<<blocks_for(u), kT, 0, s>>>((Fr *)col, u, tsize);
-    return hipGetLastError();
-}
-hipError_t wit_mult_rows(void *m, const uint32_t *cnt, uint32_t n, uint32_t tsize, hipStream_t s) {
-    k_mult_rows<<<blocks_for(n), kT, 0, s>>>((Fr *)m, cnt, n, tsize);
     return hipGetLastError();
 }
 hipError_t wit_gate_rows(void *d, const void *a, const void *b, const void *c, uint32_t u, hipStream_t s) {

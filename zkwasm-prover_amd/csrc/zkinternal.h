@@ -9,9 +9,7 @@ namespace zk {
 // witness synthesis and helpers for the prover core (witness.hip)
 hipError_t wit_u32_rows_to_fr(void *col, const uint32_t *v, uint32_t rows, hipStream_t s);
 hipError_t wit_lookup_values(uint32_t *vals, uint32_t u, uint32_t tsize, uint64_t seed, hipStream_t s);
-hipError_t wit_hist(uint32_t *cnt, const uint32_t *vals, uint32_t u, hipStream_t s);
 hipError_t wit_table_rows(void *col, uint32_t u, uint32_t tsize, hipStream_t s);
-hipError_t wit_mult_rows(void *m, const uint32_t *cnt, uint32_t n, uint32_t tsize, hipStream_t s);
 hipError_t wit_gate_rows(void *d, const void *a, const void *b, const void *c, uint32_t u, hipStream_t s);
 hipError_t wit_indicator(void *out, uint32_t n, uint32_t lo, uint32_t hi, hipStream_t s);
 hipError_t wit_zero_rows(void *col, uint32_t lo, uint32_t hi, hipStream_t s);
