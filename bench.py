@@ -315,10 +315,11 @@ def bench_proof_core(ks):
     family (40 advice, 6 range lookups into a 2^16 table, 12 permutation
     columns, 2 shuffles of up to 2^18 rows) at each k: one warm-up proof, then
     one timed proof through the batched MSM and one through one MSM call per
-    polynomial.  Witness synthesis is excluded (the reference takes the
-    advice as input)."""
+    polynomial (both opened with GWC), and one batched proof opened with
+    Shplonk.  Witness synthesis is excluded (the reference takes the advice as
+    input)."""
     import torch
-    from zkprover.prover import BATCHED, Circuit, Prover, ProvingKey, footprint
+    from zkprover.prover import BATCHED, SHPLONK, Circuit, Prover, ProvingKey, footprint
     res = {}
     for k in ks:
         circ = Circuit.benchmark(k)
@@ -329,14 +330,19 @@ def bench_proof_core(ks):
             pr.prove(1, flags=BATCHED, want_pieces=False)
             _, _, ib = pr.prove(2, flags=BATCHED, want_pieces=False)
             _, _, ip = pr.prove(2, flags=0, want_pieces=False)
+            _, _, ish = pr.prove(2, flags=BATCHED | SHPLONK, want_pieces=False)
         finally:
             pr.close()
             pk.close()
             torch.cuda.synchronize()
         res["k%d" % k] = {"batched_s": round(ib.total_ms / 1e3, 4), "plain_s": round(ip.total_ms / 1e3, 4),
                           "speedup": round(ip.total_ms / ib.total_ms, 3),
-                          "phases_ms": ib.as_dict()["phase_ms"], "z_ok": bool(ib.z_ok and ip.z_ok),
-                          "quotient_ok": bool(ib.quotient_ok and ip.quotient_ok),
+                          "shplonk_batched_s": round(ish.total_ms / 1e3, 4),
+                          "phases_ms": ib.as_dict()["phase_ms"],
+                          "shplonk_open_ms": round(ish.phase_ms[6], 3),
+                          "z_ok": bool(ib.z_ok and ip.z_ok and ish.z_ok),
+                          "quotient_ok": bool(ib.quotient_ok and ip.quotient_ok and ish.quotient_ok),
+                          "open_ok": bool(ib.open_ok and ip.open_ok and ish.open_ok),
                           "device_bytes": {"key": pk_b, "per_proof": job_b}}
     return res
 

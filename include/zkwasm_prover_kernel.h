@@ -410,9 +410,12 @@ typedef struct {
     int z_ok;            /* every grand product closed at row u */
     int quotient_ok;     /* h(X) has degree < 3n: all constraints hold */
     int z_fail;          /* when !z_ok: bit 0 logup, bit 1 permutation, bit 2 shuffle */
-    double phase_ms[6];  /* witness, advice commit + iNTT, multiplicities,
-                            grand products, h evaluation, vanishing */
-    double total_ms;     /* phases 1-5 */
+    int open_ok;         /* GWC: each batch evaluates at its point to the
+                            combination of the evaluations; Shplonk: L(u) = 0 */
+    int nevals;          /* polynomial evaluations of the opening */
+    double phase_ms[7];  /* witness, advice commit + iNTT, multiplicities,
+                            grand products, h evaluation, vanishing, opening */
+    double total_ms;     /* phases 1-6 */
 } zkp_proof_info;
 
 /* Device bytes of the resident key data and of one in-flight proof. */
@@ -426,14 +429,21 @@ hipError_t zkp_pk_destroy(void *pk);
 /* Per-proof device state (witness, grand products, extended polys, h). */
 hipError_t zkp_job_create(void *pk, void **job);
 hipError_t zkp_job_destroy(void *job);
-/* One proof's arithmetic (src/lib.rs:205-1194 phase order) for seed, on
- * stream; commitments (host) gets zkp_commitment_count affine points,
+/* One proof's arithmetic (src/lib.rs:205-1194 phase order, then the
+ * multi-open of src/multiopen.rs) for seed, on stream; commitments (host,
+ * room for zkp_commitment_count affine points) gets info.ncommit points,
  * h_pieces (host, 2^k Fr, may be null) the folded quotient.  flags:
  * ZKP_BATCHED commits through zk_msm_batch (else one zk_msm per polynomial);
  * ZKP_TAMPER_GATE breaks the gate on one usable row (a self-test: the
- * quotient check must then fail). */
+ * quotient check must then fail); ZKP_SHPLONK opens with Shplonk
+ * (create_proof_from_advices_with_shplonk), else GWC (_with_gwc);
+ * ZKP_NO_OPEN stops after the quotient commitments; ZKP_TAMPER_EVAL moves
+ * one opening evaluation off by one (a self-test: open_ok must then be 0). */
 #define ZKP_BATCHED 1
 #define ZKP_TAMPER_GATE 2
+#define ZKP_SHPLONK 4
+#define ZKP_NO_OPEN 8
+#define ZKP_TAMPER_EVAL 16
 hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commitments, void *h_pieces,
                      zkp_proof_info *info, hipStream_t stream);
 

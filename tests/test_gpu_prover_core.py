@@ -9,7 +9,12 @@ fixes, independent of any restatement:
     all right;
   * a witness with one broken gate row fails that check (the check has teeth);
   * the batched-MSM and the per-polynomial paths give identical bytes, and a
-    proof is a deterministic function of its seed.
+    proof is a deterministic function of its seed;
+  * the multi-open: with GWC each batch polynomial evaluates at its point to
+    the v-combination of the evaluations (the division by X - z leaves that
+    remainder); with Shplonk L(u) = 0, which holds only if every evaluation,
+    the interpolations, h(X) and the divisions are right; one evaluation off
+    by one fails either check.
 The commitments themselves are MSMs whose parity is covered by the MSM tests.
 """
 import numpy as np
@@ -37,9 +42,15 @@ def core(gpu):
 def test_prover_core_constraints_hold(core, zo):
     pk, pr = core
     com, pieces, info = pr.prove(11)
-    assert info.ncommit == pk.ncommit == 30 + 2 * 3 + 3 + 2 + 3
+    # advice, m, logup z, permutation z (3 sets), shuffle z, quotient pieces,
+    # GWC quotients at x, x w, x w^u
+    assert info.ncommit == pk.ncommit == 30 + 2 * 3 + 3 + 2 + 3 + 3
     assert info.z_ok == 1, "grand products that did not close: %d" % info.z_fail
     assert info.quotient_ok == 1
+    assert info.open_ok == 1
+    # set {x}: 30 advice, q, t, 6 sigmas, 3 m, h; {x, xw}: 3 logup z, 2
+    # shuffle z, last permutation z; {x, xw, xw^u}: 2 permutation z
+    assert info.nevals == 42 + 2 * 6 + 3 * 2
     assert all(zo.g1a_is_on_curve(com[i:i + 1]) for i in range(len(com)))
     assert not (com == 0).all(axis=1).any()
     assert not (pieces == 0).all()
@@ -51,6 +62,26 @@ def test_prover_core_tamper_detected(core):
     _, _, info = pr.prove(12, flags=BATCHED | TAMPER_GATE)
     assert info.z_ok == 1          # the gate is in no grand product
     assert info.quotient_ok == 0   # but h is no longer a polynomial
+
+
+def test_prover_core_shplonk(core, zo):
+    from zkprover.prover import BATCHED, SHPLONK
+    pk, pr = core
+    g, _, gi = pr.prove(13, flags=BATCHED)
+    s, _, si = pr.prove(13, flags=BATCHED | SHPLONK)
+    assert si.open_ok == 1 and si.quotient_ok == 1 and si.z_ok == 1
+    assert si.ncommit == gi.ncommit - 1  # h(X) and L(X) against three GWC quotients
+    base = gi.ncommit - 3
+    assert (s[:base] == g[:base]).all()  # the phases before the opening are shared
+    assert all(zo.g1a_is_on_curve(s[i:i + 1]) for i in range(base, si.ncommit))
+
+
+@pytest.mark.parametrize("shplonk", [False, True])
+def test_prover_core_opening_tamper_detected(core, shplonk):
+    from zkprover.prover import BATCHED, SHPLONK, TAMPER_EVAL
+    pk, pr = core
+    _, _, info = pr.prove(14, flags=BATCHED | TAMPER_EVAL | (SHPLONK if shplonk else 0))
+    assert info.quotient_ok == 1 and info.open_ok == 0
 
 
 def test_prover_core_batched_equals_plain_and_deterministic(core):
@@ -72,7 +103,10 @@ def test_prover_core_k14(gpu):
     pr = Prover(pk)
     try:
         _, _, info = pr.prove(31)
-        assert info.z_ok == 1 and info.quotient_ok == 1
+        assert info.z_ok == 1 and info.quotient_ok == 1 and info.open_ok == 1
+        from zkprover.prover import BATCHED, SHPLONK
+        _, _, info = pr.prove(32, flags=BATCHED | SHPLONK)
+        assert info.z_ok == 1 and info.quotient_ok == 1 and info.open_ok == 1
     finally:
         pr.close()
         pk.close()

@@ -37,6 +37,7 @@
 #include <chrono>
 #include <vector>
 
+#include "host_fr.h"
 #include "zkinternal.h"
 #include "zkwasm_prover_kernel.h"
 
@@ -67,6 +68,8 @@ constexpr uint64_t kDelta[4] = {0x870e56bbe533e9a2ull, 0x5b5f898e5e963f25ull, 0x
                                 0x09226b6e22c6f0caull};
 
 enum FrOp { OP_ADD = 0, OP_MUL = 1, OP_SUB = 2, OP_INV = 3, OP_SQR = 4, OP_NEG = 5, OP_TO_MONT = 6 };
+// field_op's operations
+enum FieldOp { FO_ADD = 0, FO_MUL = 1, FO_LEFT = 2, FO_SUB = 3 };
 
 uint64_t splitmix(uint64_t &s) {
     uint64_t z = (s += 0x9e3779b97f4a7c15ull);
@@ -142,6 +145,7 @@ struct Pk {
     void *g_lag, *g, *w_n, *w_n_inv, *w_e, *w_e_inv;
     uint8_t *cst;  // Fr constants
     void *t_lag, *t_ext, *q_ext, *l0_ext, *llast_ext, *lact_ext, *ones_ext, *x_ext;
+    void *t_coef, *q_coef;  // fixed columns opened in the multi-open step
     std::vector<void *> sig_lag, sig_coef, sig_ext;
     uint32_t *perm_head, *perm_target;
     std::vector<uint32_t *> shuf_perm;
@@ -162,6 +166,8 @@ void pk_layout(Pk &pk, Arena &a) {
     pk.cst = a.take((size_t)(C_COUNT_FIXED + s.P + 8) * FR);
     pk.t_lag = a.fr(s.n);
     pk.t_ext = a.fr(s.N);
+    pk.t_coef = a.fr(s.n);
+    pk.q_coef = a.fr(s.n);
     pk.q_ext = a.fr(s.N);
     pk.l0_ext = a.fr(s.N);
     pk.llast_ext = a.fr(s.N);
@@ -274,19 +280,21 @@ hipError_t pk_build(Pk &pk, uint64_t seed, hipStream_t st) {
     // l_active; a column of ones
     void *coef = pk.scratch_n[1], *tmp = pk.scratch_n[2], *res = nullptr;
     bool sw = false;
-    auto lag_to_coset = [&](const void *lag, void *ext) -> hipError_t {
+    auto lag_to_coset = [&](const void *lag, void *ext, void *coef_out = nullptr) -> hipError_t {
         PCHECK(hipMemcpyAsync(coef, lag, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
         PCHECK(ntt(coef, tmp, nullptr, pk.w_n_inv, pk.cf(C_NINV), s.k, 8, &sw, st));
+        if (coef_out)
+            PCHECK(hipMemcpyAsync(coef_out, sw ? tmp : coef, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
         PCHECK(to_coset(pk, ext, sw ? tmp : coef, pk.scratch_e, &res, st));
         if (res != ext) PCHECK(hipMemcpyAsync(ext, res, (size_t)s.N * FR, hipMemcpyDeviceToDevice, st));
         return hipSuccess;
     };
     PCHECK(zk_rand_fr(pk.t_lag, (int)s.n, seed ^ 0x3333ull, st));
     PCHECK(zk::wit_table_rows(pk.t_lag, s.u, s.tsize, st));
-    PCHECK(lag_to_coset(pk.t_lag, pk.t_ext));
+    PCHECK(lag_to_coset(pk.t_lag, pk.t_ext, pk.t_coef));
     void *lag = pk.scratch_n[0];
     PCHECK(zk::wit_indicator(lag, s.n, 0, s.u, st));  // q = l_active on this circuit
-    PCHECK(lag_to_coset(lag, pk.q_ext));
+    PCHECK(lag_to_coset(lag, pk.q_ext, pk.q_coef));
     PCHECK(hipMemcpyAsync(pk.lact_ext, pk.q_ext, (size_t)s.N * FR, hipMemcpyDeviceToDevice, st));
     PCHECK(zk::wit_indicator(lag, s.n, 0, 1, st));
     PCHECK(lag_to_coset(lag, pk.l0_ext));
@@ -353,8 +361,30 @@ hipError_t pk_build(Pk &pk, uint64_t seed, hipStream_t st) {
     return hipStreamSynchronize(st);
 }
 
+// ---- multi-open query layout -------------------------------------------------
+// Polynomials grouped by the points they are opened at (points: x, x w,
+// x w^u): set 0 at {x}: advice, q, t, sigmas, m, h; set 1 at {x, x w}:
+// logup z, shuffle z, the last permutation z; set 2 at {x, x w, x w^u}: the
+// other permutation z (u = the last usable row, the halo2 "last" rotation).
+int open_set_size(const Shape &s, int set) {
+    if (set == 0) return s.A + 2 + s.P + s.L + 1;
+    if (set == 1) return s.L + s.S + (s.Sp > 0 ? 1 : 0);
+    return s.Sp > 1 ? s.Sp - 1 : 0;
+}
+int open_eval_count(const Shape &s) { return open_set_size(s, 0) + 2 * open_set_size(s, 1) + 3 * open_set_size(s, 2); }
+// GWC commits one quotient per distinct point, Shplonk two polynomials
+int open_gwc_points(const Shape &s) {
+    return open_set_size(s, 2) ? 3 : open_set_size(s, 1) ? 2 : 1;
+}
+constexpr int kOpenCst = 64;
+
 // ---- per-proof state ---------------------------------------------------------
-enum JConst { J_BETA, J_GAMMA, J_Y, J_X, J_XN, J_LASTZ, J_BDELTA, J_COUNT_FIXED = J_BDELTA };
+// beta, gamma, y, x; x^n; logup carry; opening challenges v, y', u; opening
+// points x w, x w^u; then beta delta^j per permutation column
+enum JConst {
+    J_BETA, J_GAMMA, J_Y, J_X, J_XN, J_LASTZ, J_V, J_SY, J_U, J_XW, J_XWU, J_BDELTA,
+    J_COUNT_FIXED = J_BDELTA
+};
 
 struct Job {
     const Pk *pk;
@@ -373,6 +403,8 @@ struct Job {
     uint8_t *msm_scratch, *msm_out;
     size_t msm_bytes;
     uint32_t *flag;
+    void *evals;     // opening evaluations, one Fr per (polynomial, point)
+    void *open_cst;  // opening scalars staged from the host
     void *cf(int c) const { return cst + (size_t)c * FR; }
 };
 
@@ -423,6 +455,8 @@ void job_layout(Job &j, Arena &a) {
     j.msm_scratch = a.take(j.msm_bytes);
     j.msm_out = a.take((size_t)kMsmGroup * ZK_MSM_OUT_BYTES);
     j.flag = (uint32_t *)a.take(256);
+    j.evals = a.fr((size_t)open_eval_count(s));
+    j.open_cst = a.fr(kOpenCst);
 }
 
 // commitments of polys (device Lagrange or coefficient values of n) against
@@ -534,7 +568,212 @@ hipError_t witness(Job &j, hipStream_t st) {
     }
     // challenges
     PCHECK(zk_rand_fr(j.cf(J_BETA), 4, seed ^ 0xc4a11e46e5ull, st));  // beta, gamma, y, x
+    PCHECK(zk_rand_fr(j.cf(J_V), 3, seed ^ 0x0be11a7e5ull, st));      // v, y', u
     return hipStreamSynchronize(st);
+}
+
+// coefficients (ascending) of the polynomial of degree < m through
+// (pts[k], vals[k]) (halo2's lagrange_interpolate)
+std::vector<zkh::Fr> lagrange(const zkh::Fr *pts, const zkh::Fr *vals, int m) {
+    using namespace zkh;
+    std::vector<Fr> res(m, fr_zero());
+    for (int k = 0; k < m; k++) {
+        std::vector<Fr> num(1, fr_one());
+        Fr den = fr_one();
+        for (int l = 0; l < m; l++) {
+            if (l == k) continue;
+            std::vector<Fr> nx(num.size() + 1, fr_zero());
+            for (size_t i = 0; i < num.size(); i++) {
+                nx[i + 1] = fr_add(nx[i + 1], num[i]);
+                nx[i] = fr_sub(nx[i], fr_mul(pts[l], num[i]));
+            }
+            num.swap(nx);
+            den = fr_mul(den, fr_sub(pts[k], pts[l]));
+        }
+        const Fr c = fr_mul(vals[k], fr_inv(den));
+        for (int i = 0; i < m; i++) res[i] = fr_add(res[i], fr_mul(num[i], c));
+    }
+    return res;
+}
+
+// Multi-open of every query (src/multiopen.rs): evaluations of each
+// polynomial at its points (batch_poly_eval, src/eval_poly.rs:18), then GWC
+// (one quotient per point, :68-177) or Shplonk (h(X) and L(X), :283-561) with
+// the divisions by (X - z) on the device.  The challenges come from the proof
+// seed.  open_ok: GWC, every batch polynomial evaluates at its point to the
+// v-combination of the evaluations; Shplonk, L(u) = 0 -- each holds only if
+// every evaluation and every division is right.
+hipError_t open_phase(Job &j, const void *hpoly, bool shplonk, bool batched, bool tamper, uint8_t *out, int &ncom,
+                      int &open_ok, int &nevals, hipStream_t st) {
+    using namespace zkh;
+    const Pk &pk = *j.pk;
+    const Shape &s = pk.sh;
+    const int n = (int)s.n;
+    const size_t nb = (size_t)s.n * FR;
+    // n-size work slots in the extended buffers (free after the vanishing step)
+    std::vector<void *> slot;
+    for (void *b : {j.l_acc, j.r_acc, j.aux0, j.aux1})
+        for (int q = 0; q < (1 << kExtBits); q++) slot.push_back((uint8_t *)b + (size_t)q * nb);
+    void *ev_scratch = slot[15], *kate_tmp = slot[14];
+    uint8_t *oc = (uint8_t *)j.open_cst;
+    auto ocs = [&](int i) -> void * { return oc + (size_t)i * FR; };
+    std::vector<const void *> sets[3];
+    for (auto p : j.adv_coef) sets[0].push_back(p);
+    sets[0].push_back(pk.q_coef);
+    sets[0].push_back(pk.t_coef);
+    for (auto p : pk.sig_coef) sets[0].push_back(p);
+    for (auto p : j.m_coef) sets[0].push_back(p);
+    sets[0].push_back(hpoly);
+    for (auto p : j.zl_coef) sets[1].push_back(p);
+    for (auto p : j.zs_coef) sets[1].push_back(p);
+    if (s.Sp > 0) sets[1].push_back(j.zp_coef[s.Sp - 1]);
+    for (int t = 0; t + 1 < s.Sp; t++) sets[2].push_back(j.zp_coef[t]);
+    const int nsets = sets[2].empty() ? (sets[1].empty() ? 1 : 2) : 3;  // = number of distinct points
+
+    // points x, x w, x w^u with w^u = (w^-1)^(blinding + 1)
+    PCHECK(zk_fr_op(j.cf(J_XW), j.cf(J_X), pk.cf(C_WN), OP_MUL, 1, st));
+    PCHECK(hipMemcpyAsync(j.cf(J_XWU), j.cf(J_X), FR, hipMemcpyDeviceToDevice, st));
+    for (int i = 0; i <= s.blind; i++) PCHECK(zk_fr_op(j.cf(J_XWU), j.cf(J_XWU), pk.cf(C_WNINV), OP_MUL, 1, st));
+    const void *pts_dev[3] = {j.cf(J_X), j.cf(J_XW), j.cf(J_XWU)};
+
+    // evaluations: set by set, polynomial by polynomial, point by point
+    int e = 0;
+    for (int set = 0; set < 3; set++)
+        for (auto p : sets[set])
+            for (int q = 0; q <= set; q++) {
+                PCHECK(poly_eval(p, ev_scratch, pts_dev[q], n, st));
+                PCHECK(hipMemcpyAsync((uint8_t *)j.evals + (size_t)e * FR, ev_scratch, FR, hipMemcpyDeviceToDevice,
+                                      st));
+                e++;
+            }
+    nevals = e;
+    if (tamper)  // self-test of the opening check: one evaluation off by one
+        PCHECK(zk_fr_op((uint8_t *)j.evals + (size_t)(e - 1) * FR, (uint8_t *)j.evals + (size_t)(e - 1) * FR,
+                        pk.cf(C_ONE), OP_ADD, 1, st));
+    std::vector<Fr> ev(e);
+    Fr ch[3], P[3];
+    PCHECK(hipMemcpyAsync(ev.data(), j.evals, (size_t)e * FR, hipMemcpyDeviceToHost, st));
+    PCHECK(hipMemcpyAsync(ch, j.cf(J_V), 3 * FR, hipMemcpyDeviceToHost, st));  // v, y', u
+    for (int q = 0; q < 3; q++) PCHECK(hipMemcpyAsync(&P[q], pts_dev[q], FR, hipMemcpyDeviceToHost, st));
+    PCHECK(hipStreamSynchronize(st));
+    const Fr v = ch[0], y = ch[1], u = ch[2];
+    // Horner in c over the evaluations of `set` at point q, in query order
+    auto combine = [&](int set, int q, const Fr &c) {
+        int first = 0;
+        for (int t = 0; t < set; t++) first += (int)sets[t].size() * (t + 1);
+        Fr acc = fr_zero();
+        for (size_t i = 0; i < sets[set].size(); i++) acc = fr_add(fr_mul(acc, c), ev[first + i * (set + 1) + q]);
+        return acc;
+    };
+
+    if (!shplonk) {
+        // one batch per point: every query at point q, sets q.. in order
+        std::vector<const void *> W;
+        std::vector<Fr> expect(nsets);
+        for (int q = 0; q < nsets; q++) {
+            void *B = slot[q];
+            bool first = true;
+            Fr acc = fr_zero();
+            for (int set = q; set < 3; set++) {
+                Fr cs = combine(set, q, v);
+                // the set's block of the Horner sum: acc v^len + (set's own sum)
+                for (size_t i = 0; i < sets[set].size(); i++) acc = fr_mul(acc, v);
+                acc = fr_add(acc, cs);
+                for (auto p : sets[set]) {
+                    if (first)
+                        PCHECK(hipMemcpyAsync(B, p, nb, hipMemcpyDeviceToDevice, st));
+                    else
+                        PCHECK(field_op(B, B, 0, j.cf(J_V), p, 0, nullptr, n, FO_ADD, st));
+                    first = false;
+                }
+            }
+            expect[q] = acc;
+            PCHECK(zk_kate_divide(B, B, pts_dev[q], ocs(q), n, kate_tmp, st));
+            W.push_back(B);
+        }
+        PCHECK(commit(j, pk.g, W, out + (size_t)ncom * 64, batched, st));
+        ncom += nsets;
+        std::vector<Fr> got(nsets);
+        PCHECK(hipMemcpyAsync(got.data(), oc, (size_t)nsets * FR, hipMemcpyDeviceToHost, st));
+        PCHECK(hipStreamSynchronize(st));
+        open_ok = 1;
+        for (int q = 0; q < nsets; q++) open_ok &= fr_eq(got[q], expect[q]) ? 1 : 0;
+        return hipSuccess;
+    }
+
+    // Shplonk.  V_s = y-combination of set s; r_s interpolates its evaluations
+    void *V[3] = {slot[0], slot[1], slot[2]};
+    void *hx = slot[3], *pb = slot[4], *tmp = slot[5], *fz = slot[6];
+    std::vector<Fr> r[3];
+    for (int set = 0; set < nsets; set++) {
+        bool first = true;
+        for (auto p : sets[set]) {
+            if (first)
+                PCHECK(hipMemcpyAsync(V[set], p, nb, hipMemcpyDeviceToDevice, st));
+            else
+                PCHECK(field_op(V[set], V[set], 0, j.cf(J_SY), p, 0, nullptr, n, FO_ADD, st));
+            first = false;
+        }
+        Fr ec[3];
+        for (int q = 0; q <= set; q++) ec[q] = combine(set, q, y);
+        r[set] = lagrange(P, ec, set + 1);
+    }
+    // host staging of the scalars the device reads: [0, 9) r coefficients of
+    // the three sets, then per set u-evaluation and z_s, then Z_T(u), 1/z_0
+    std::vector<Fr> stage(kOpenCst - 32, fr_zero());
+    for (int set = 0; set < nsets; set++)
+        for (int c = 0; c <= set; c++) stage[3 * set + c] = r[set][c];
+    Fr zt = fr_one(), z0inv = fr_one();
+    for (int q = 0; q < nsets; q++) zt = fr_mul(zt, fr_sub(u, P[q]));
+    for (int set = 0; set < nsets; set++) {
+        Fr ru = fr_zero();  // r_s(u)
+        for (int c = set; c >= 0; c--) ru = fr_add(fr_mul(ru, u), r[set][c]);
+        Fr zs = fr_one();   // prod over the points outside the set
+        for (int q = set + 1; q < nsets; q++) zs = fr_mul(zs, fr_sub(u, P[q]));
+        stage[9 + 2 * set] = ru;
+        stage[10 + 2 * set] = zs;
+        if (set == 0) z0inv = fr_inv(zs);
+    }
+    stage[15] = zt;
+    stage[16] = z0inv;
+    PCHECK(hipMemcpyAsync(ocs(32), stage.data(), stage.size() * FR, hipMemcpyHostToDevice, st));
+    // point arrays on the device: [8, 11) the points in order
+    for (int q = 0; q < nsets; q++) PCHECK(hipMemcpyAsync(ocs(8 + q), pts_dev[q], FR, hipMemcpyDeviceToDevice, st));
+
+    PCHECK(hipMemsetAsync(hx, 0, nb, st));
+    for (int set = 0; set < nsets; set++) {
+        PCHECK(hipMemcpyAsync(pb, V[set], nb, hipMemcpyDeviceToDevice, st));
+        PCHECK(field_op(pb, pb, 0, nullptr, ocs(32 + 3 * set), 0, nullptr, set + 1, FO_SUB, st));
+        bool sw = false;
+        PCHECK(ntt(pb, tmp, nullptr, pk.w_n, nullptr, s.k, 8, &sw, st));
+        // the points outside the set are ocs(8 + set + 1 ..)
+        PCHECK(zk_shplonk_h_x_merge_on(hx, j.cf(J_V), sw ? tmp : pb, pk.w_n, ocs(8 + set + 1), nsets - set - 1, n,
+                                       st));
+    }
+    PCHECK(zk_shplonk_h_x_div_points_on(hx, pk.w_n, ocs(8), nsets, n, st));
+    {
+        bool sw = false;
+        PCHECK(ntt(hx, tmp, nullptr, pk.w_n_inv, pk.cf(C_NINV), s.k, 8, &sw, st));
+        if (sw) PCHECK(hipMemcpyAsync(hx, tmp, nb, hipMemcpyDeviceToDevice, st));
+    }
+    PCHECK(commit(j, pk.g, std::vector<const void *>{hx}, out + (size_t)ncom * 64, batched, st));
+    ncom += 1;
+    // L(X) = (sum_s v^. z_s (V_s - r_s(u)) - Z_T(u) h(X)) / z_0, then / (X - u)
+    PCHECK(hipMemsetAsync(fz, 0, nb, st));
+    for (int set = 0; set < nsets; set++) {
+        PCHECK(zk_fr_op(V[set], V[set], ocs(32 + 9 + 2 * set), OP_SUB, 1, st));
+        PCHECK(field_op(fz, fz, 0, j.cf(J_V), V[set], 0, ocs(32 + 10 + 2 * set), n, FO_ADD, st));
+    }
+    PCHECK(field_op(fz, fz, 0, nullptr, hx, 0, ocs(32 + 15), n, FO_SUB, st));
+    PCHECK(field_op(fz, fz, 0, ocs(32 + 16), nullptr, 0, nullptr, n, FO_LEFT, st));
+    PCHECK(zk_kate_divide(fz, fz, j.cf(J_U), ocs(0), n, kate_tmp, st));
+    PCHECK(commit(j, pk.g, std::vector<const void *>{fz}, out + (size_t)ncom * 64, batched, st));
+    ncom += 1;
+    Fr lu;
+    PCHECK(hipMemcpyAsync(&lu, ocs(0), FR, hipMemcpyDeviceToHost, st));
+    PCHECK(hipStreamSynchronize(st));
+    open_ok = fr_is_zero(lu) ? 1 : 0;
+    return hipSuccess;
 }
 
 }  // namespace
@@ -597,12 +836,14 @@ hipError_t zkp_pk_destroy(void *pk) {
     return e;
 }
 
-// Number of commitments a proof writes: advice, m, logup z, permutation z,
-// shuffle z, quotient pieces.
+// Commitments a proof writes at most: advice, m, logup z, permutation z,
+// shuffle z, quotient pieces, then the opening (GWC: one per distinct point;
+// Shplonk: h and L).
 int zkp_commitment_count(const zkp_circuit *c) {
     Shape s;
     if (!make_shape(c, s)) return 0;
-    return s.A + 2 * s.L + s.Sp + s.S + kQuot;
+    const int open = open_gwc_points(s) > 2 ? open_gwc_points(s) : 2;
+    return s.A + 2 * s.L + s.Sp + s.S + kQuot + open;
 }
 
 hipError_t zkp_job_create(void *pk, void **job_out) {
@@ -842,6 +1083,7 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
     PCHECK(mark(4));
 
     // ---- 5. vanishing -----------------------------------------------------------
+    const void *hpoly = nullptr;  // h(X) folded in x^n: opened at x in phase 6
     {
         PCHECK(zk_field_mul_zip_on(j.h, pk.cf(C_TEV), 4, N, st));
         bool sw = false;
@@ -866,8 +1108,19 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
             PCHECK(field_op(last, last, 0, j.cf(J_XN), (uint8_t *)c + (size_t)i * s.n * FR, 0, nullptr, (int)s.n, 0,
                             st));
         if (h_pieces) PCHECK(hipMemcpyAsync(h_pieces, last, (size_t)s.n * FR, hipMemcpyDeviceToHost, st));
+        hpoly = last;
     }
     PCHECK(mark(5));
+
+    // ---- 6. multi-open ------------------------------------------------------------
+    if (!(flags & ZKP_NO_OPEN)) {
+        int ok = 0, ne = 0;
+        PCHECK(open_phase(j, hpoly, (flags & ZKP_SHPLONK) != 0, batched != 0, (flags & ZKP_TAMPER_EVAL) != 0, out,
+                          ncom, ok, ne, st));
+        inf.open_ok = ok;
+        inf.nevals = ne;
+    }
+    PCHECK(mark(6));
     inf.total_ms = tp - t_start;
     inf.ncommit = ncom;
     if (info) *info = inf;

@@ -156,7 +156,7 @@ hipError_t zks_create(const zkp_circuit *shapes, int nshapes, int slots, size_t 
                       unsigned long long key_seed, void **srv_out) {
     if (!shapes || nshapes <= 0 || slots <= 0 || slots > 32 || !srv_out) return hipErrorInvalidValue;
     Server *s = new Server();
-    s->flags = flags & ZKP_BATCHED;
+    s->flags = flags & (ZKP_BATCHED | ZKP_SHPLONK | ZKP_NO_OPEN);
     hipError_t e = hipGetDevice(&s->device);
     s->shapes.resize(nshapes);
     for (int i = 0; i < nshapes && e == hipSuccess; i++) {

@@ -25,18 +25,24 @@ class Circuit(ctypes.Structure):
 
 class ProofInfo(ctypes.Structure):
     _fields_ = [("ncommit", ctypes.c_int), ("z_ok", ctypes.c_int), ("quotient_ok", ctypes.c_int),
-                ("z_fail", ctypes.c_int), ("phase_ms", ctypes.c_double * 6), ("total_ms", ctypes.c_double)]
+                ("z_fail", ctypes.c_int), ("open_ok", ctypes.c_int), ("nevals", ctypes.c_int),
+                ("phase_ms", ctypes.c_double * 7), ("total_ms", ctypes.c_double)]
 
-    PHASES = ("witness", "advice_commit_intt", "multiplicities", "grand_products", "h_eval", "vanishing")
+    PHASES = ("witness", "advice_commit_intt", "multiplicities", "grand_products", "h_eval", "vanishing",
+              "multiopen")
 
     def as_dict(self):
         return {"ncommit": self.ncommit, "z_ok": bool(self.z_ok), "quotient_ok": bool(self.quotient_ok),
+                "open_ok": bool(self.open_ok), "nevals": self.nevals,
                 "phase_ms": {k: round(self.phase_ms[i], 3) for i, k in enumerate(self.PHASES)},
                 "total_ms": round(self.total_ms, 3)}
 
 
 BATCHED = 1
 TAMPER_GATE = 2
+SHPLONK = 4
+NO_OPEN = 8
+TAMPER_EVAL = 16
 
 
 def _decl():
@@ -100,4 +106,5 @@ class Prover:
             self.handle = ctypes.c_void_p()
 
 
-__all__ = ["Circuit", "ProofInfo", "ProvingKey", "Prover", "footprint", "BATCHED", "TAMPER_GATE", "HipError"]
+__all__ = ["Circuit", "ProofInfo", "ProvingKey", "Prover", "footprint", "BATCHED", "TAMPER_GATE", "SHPLONK",
+           "NO_OPEN", "TAMPER_EVAL", "HipError"]
