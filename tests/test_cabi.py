@@ -195,7 +195,8 @@ def test_prover_core_sizing_host_only(lib):
     pk18, job18 = footprint(Circuit.benchmark(18))
     assert pk20 > pk18 > 0 and job20 > job18 > 0
     # 40 advice + 6 m + 6 logup z + 6 permutation sets + 2 shuffle z + 3 pieces
-    assert lib.zkp_commitment_count(ctypes.byref(c20)) == 40 + 12 + 6 + 2 + 3
+    # + 3 GWC quotients (points x, x w, x w^u)
+    assert lib.zkp_commitment_count(ctypes.byref(c20)) == 40 + 12 + 6 + 2 + 3 + 3
     # per proof at k=20: ~ 40 * 3 n + 63 coset polys of 4n, well under 288 GB
     assert 8e9 < job20 < 64e9
     bad = Circuit.benchmark(20)
