@@ -103,6 +103,40 @@ __device__ __forceinline__ F29 f29_mul(const F29 &a, const F29 &b) {
     return r;
 }
 
+// (a * b + c * d) * 2^-261 mod p, < 2p, with one reduction for both
+// products (the sum of the two a*b-type terms and the m*p terms of a column
+// is at most 27 values < 2^58); valid for a * b + c * d < 2^261 * p, e.g. all
+// four < 2p
+__device__ __forceinline__ F29 f29_mul2(const F29 &a, const F29 &b, const F29 &c, const F29 &d) {
+    uint32_t m[9];
+    F29 r;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+        const int lo = k < 9 ? 0 : k - 8, hi = k < 9 ? k : 8;
+        uint64_t t0 = 0, t1 = 0;
+#pragma unroll
+        for (int i = lo; i <= hi; i++) {
+            t0 += (uint64_t)a.l[i] * b.l[k - i];
+            t1 += (uint64_t)c.l[i] * d.l[k - i];
+        }
+        uint64_t u = 0;
+#pragma unroll
+        for (int i = lo; i <= hi; i++)
+            if (i < k) u += (uint64_t)m[i] * kP29[k - i];
+        acc += t0 + t1 + u;
+        if (k < 9) {
+            m[k] = ((uint32_t)acc * kPInv29) & kM29;
+            acc += (uint64_t)m[k] * kP29[0];
+        } else {
+            r.l[k - 9] = (uint32_t)acc & kM29;
+        }
+        acc >>= 29;
+    }
+    r.l[8] = (uint32_t)acc;
+    return r;
+}
+
 // a^2, cross products once and doubled
 __device__ __forceinline__ F29 f29_sqr(const F29 &a) {
     uint32_t m[9];
@@ -301,7 +335,9 @@ __device__ __forceinline__ void x29_acc_aff(X29 &p, bool &empty, const F29 &x2, 
     F29 PPP = f29_mul(P, PP);
     F29 Q = f29_mul(p.x, PP);
     F29 X3 = f29_sub(f29_sub(f29_sqr(R), PPP), f29_dbl(Q));
-    F29 Y3 = f29_sub(f29_mul(R, f29_sub(Q, X3)), f29_mul(p.y, PPP));
+    // R (Q - X3) - Y1 PPP as R (Q - X3) + (2p - Y1) PPP: one reduction
+    const F29 zero = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    F29 Y3 = f29_mul2(R, f29_sub(Q, X3), f29_sub(zero, p.y), PPP);
     p.zz = f29_mul(p.zz, PP);
     p.zzz = f29_mul(p.zzz, PPP);
     p.x = X3;

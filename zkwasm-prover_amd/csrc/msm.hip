@@ -682,18 +682,29 @@ __global__ __launch_bounds__(64) void k_msm_reduce_l1(const G1Xyzz *__restrict__
     xyzz_store(&sum1[g], run);
 }
 
-// grid (nbits + 1, items * W): q < nbits -> sum of sum1[g][s] over s with
-// bit q set; q == nbits -> sum of ws1[g][s] over all s.  Written into the
-// item's curve slice at Q[w][q].
+// grid (nbits + 1, items * W, parts): q < nbits -> sum of sum1[g][s] over s
+// with bit q set; q == nbits -> sum of ws1[g][s] over all s.  With one part
+// the sum goes into the item's curve slice at Q[w][q]; with several, block z
+// sums the z-th slice of s into qpart[(g * (nbits + 1) + q) * parts + z] and
+// k_msm_masked_merge adds the parts (a single MSM has only (nbits+1) * W
+// sums, too few workgroups to fill the device).
+__device__ __forceinline__ G1Xyzz *slot_q(uint8_t *const *slots, uint32_t g, uint32_t W, uint32_t nbits,
+                                          uint32_t bit) {
+    G1Xyzz *q = reinterpret_cast<G1Xyzz *>(slots[g / W] + kSliceQ);
+    return &q[(size_t)(g % W) * (nbits + 1) + bit];
+}
+
 __global__ __launch_bounds__(256) void k_msm_masked_sums(const G1Xyzz *__restrict__ ws1,
                                                          const G1Xyzz *__restrict__ sum1, uint32_t T1,
                                                          uint32_t nbits, uint32_t W,
-                                                         uint8_t *const *__restrict__ slots) {
+                                                         uint8_t *const *__restrict__ slots,
+                                                         G1Xyzz *__restrict__ qpart) {
     __shared__ G1Xyzz sh[256];
-    uint32_t bit = blockIdx.x, g = blockIdx.y;
+    const uint32_t bit = blockIdx.x, g = blockIdx.y, parts = gridDim.z, part = blockIdx.z;
+    const uint32_t span = T1 / parts, s0 = part * span;
     const G1Xyzz *src = (bit == nbits ? ws1 : sum1) + (size_t)g * T1;
     G1Xyzz acc = xyzz_id();
-    for (uint32_t s = threadIdx.x; s < T1; s += blockDim.x) {
+    for (uint32_t s = s0 + threadIdx.x; s < s0 + span; s += blockDim.x) {
         if (bit == nbits || ((s >> bit) & 1u)) acc = xyzz_add(acc, xyzz_load(&src[s]));
     }
     sh[threadIdx.x] = acc;
@@ -703,9 +714,31 @@ __global__ __launch_bounds__(256) void k_msm_masked_sums(const G1Xyzz *__restric
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        G1Xyzz *q = reinterpret_cast<G1Xyzz *>(slots[g / W] + kSliceQ);
-        xyzz_store(&q[(size_t)(g % W) * (nbits + 1) + bit], sh[0]);
+        if (parts == 1)
+            xyzz_store(slot_q(slots, g, W, nbits, bit), sh[0]);
+        else
+            xyzz_store(&qpart[((size_t)g * (nbits + 1) + bit) * parts + part], sh[0]);
     }
+}
+
+// the parts of each (item window, bit) sum: a workgroup takes 256 / parts
+// sums, one thread per part, and adds them as a tree in LDS
+__global__ __launch_bounds__(256) void k_msm_masked_merge(const G1Xyzz *__restrict__ qpart, uint32_t parts,
+                                                          uint32_t nbits, uint32_t W, uint32_t gw,
+                                                          uint8_t *const *__restrict__ slots) {
+    __shared__ G1Xyzz sh[256];
+    const uint32_t per = 256 / parts, z = threadIdx.x % parts;
+    const uint64_t total = (uint64_t)gw * (nbits + 1);
+    const uint64_t i = (uint64_t)blockIdx.x * per + threadIdx.x / parts;
+    const bool live = i < total;
+    sh[threadIdx.x] = live ? xyzz_load(&qpart[i * parts + z]) : xyzz_id();
+    __syncthreads();
+    for (uint32_t st = parts / 2; st; st >>= 1) {
+        if (z < st) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (live && z == 0) xyzz_store(slot_q(slots, (uint32_t)(i / (nbits + 1)), W, nbits, (uint32_t)(i % (nbits + 1))),
+                                   sh[threadIdx.x]);
 }
 
 struct SliceHeader {
@@ -988,8 +1021,26 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     const uint32_t nseg = gw * p.T1;
     k_msm_reduce_l1<<<(nseg + 63) / 64, 64, 0, stream>>>(buckets, offsets, gw, p.B, p.T1, ws1, sum1);
     ZK_CHECK(hipGetLastError());
-    k_msm_masked_sums<<<dim3(p.nbits + 1, gw), 256, 0, stream>>>(ws1, sum1, p.T1, p.nbits, p.W, slots);
-    ZK_CHECK(hipGetLastError());
+    {
+        // split each masked sum over `parts` workgroups when there are too
+        // few sums to give every SIMD a wave (the sums are latency-bound
+        // chains of point additions; more parts than that only adds tree
+        // levels); the parts go to the chunk partials' array (dead after the
+        // fixup)
+        const uint64_t sums = (uint64_t)gw * (p.nbits + 1);
+        uint32_t parts = 1;
+        while (sums * parts * 2 <= 1024 && parts * 2 * 256 <= p.T1 && parts < 256 && sums * parts * 2 <= p.chunks)
+            parts *= 2;
+        k_msm_masked_sums<<<dim3(p.nbits + 1, gw, parts), 256, 0, stream>>>(ws1, sum1, p.T1, p.nbits, p.W, slots,
+                                                                          phead);
+        ZK_CHECK(hipGetLastError());
+        if (parts > 1) {
+            const uint32_t per = 256 / parts;
+            k_msm_masked_merge<<<(uint32_t)((sums + per - 1) / per), 256, 0, stream>>>(phead, parts, p.nbits, p.W,
+                                                                                      gw, slots);
+            ZK_CHECK(hipGetLastError());
+        }
+    }
     ZK_CHECK(prof_mark(prof, PH_HEADER, stream));
     k_slice_header<<<(count + 63) / 64, 64, 0, stream>>>(slots, count, c, p.W, p.nbits);
     ZK_CHECK(hipGetLastError());
