@@ -133,6 +133,17 @@ def main():
     lk_m, lk_so, lk_t0, lk_t1 = u8(n * 32), u8(n * 32), u8(n * 32), u8(n * 32)
     lk_u = n - 6
 
+    # one eval_h constraint kernel at the extended size of a k=22 proof: the
+    # logup term over 2^24 rows (ten input columns, res read and written)
+    eh = [u8(N * 32) for _ in range(10)]
+    for i, t in enumerate(eh):
+        call("zk_rand_fr", t.data_ptr(), N, 40 + i, st)
+    eh_y = u8(32)
+    call("zk_rand_fr", eh_y.data_ptr(), 1, 50, st)
+
+    def run_logup_eval_h():
+        call("logup_eval_h", *[t.data_ptr() for t in eh], eh_y.data_ptr(), 4, N, st)
+
     def run_prepare_table():
         call("prepare_table_lookup", lk_table.data_ptr(), None, None, None, None, lk_so.data_ptr(), None, 0,
              LOG_N, lk_u, st)
@@ -169,7 +180,8 @@ def main():
              ("batch_inversion_2^22", restore(inv_buf, inv_in),
               lambda: call("zk_batch_invert", inv_buf.data_ptr(), inv_tmp.data_ptr(), n, st)),
              ("prepare_table_lookup_2^22", None, run_prepare_table),
-             ("calc_m_2^22", None, run_calc_m)]
+             ("calc_m_2^22", None, run_calc_m),
+             ("logup_eval_h_2^24", None, run_logup_eval_h)]
 
     # one plain copy of 256 MiB: the calibration dispatch for the HBM byte
     # counters (FETCH_SIZE / WRITE_SIZE) when this script runs under rocprofv3
@@ -222,6 +234,22 @@ def main():
                        f"{OPS_PER_MONT_MUL} 32-bit mad ops",
     }
 
+    # the other hot kernels against both ceilings: algorithmic bytes (one read
+    # of every input and one write of every output) over HBM peak, canonical
+    # Montgomery products (136 mad ops each) over VALU peak
+    def fracs(ms, nbytes, muls):
+        return {"time_ms": round(ms, 4), "bytes": nbytes, "mont_muls": muls,
+                "hbm_frac": round(nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                "valu_frac": round(muls * OPS_PER_MONT_MUL / (ms * 1e-3) / (VALU_PEAK_TOPS * 1e12), 4)}
+    secondary = {
+        "ntt_2^22": fracs(prim_ms["ntt_2^22"], 2 * n * 32, n // 2 * LOG_N),
+        "intt_2^22": fracs(prim_ms["intt_2^22"], 2 * n * 32, n // 2 * LOG_N + n),
+        "coset_ntt_2^22_to_2^24": fracs(prim_ms["coset_ntt_2^22_to_2^24"], n * 32 + N * 32,
+                                        N // 2 * (LOG_N + EXT) + n),
+        "batch_inversion_2^22": fracs(prim_ms["batch_inversion_2^22"], 2 * n * 32, 3 * n),
+        "logup_eval_h_2^24": fracs(prim_ms["logup_eval_h_2^24"], 11 * N * 32, 9 * N),
+    }
+
     out = {
         "metric": METRIC, "value": round(prim_ms["msm_2^22"], 3), "unit": "ms (G1 MSM, n=2^22)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 3),
@@ -229,11 +257,13 @@ def main():
         "dtype": "u32 limbs (exact 256-bit Montgomery)", "data": "synthetic",
         "config": {"workload": "zkWasm-size primitives: BN254 G1 MSM n=2^22; Fr NTT and iNTT 2^22; "
                                "coset NTT 2^22->2^24; batch inversion 2^22; lookup multiplicity "
-                               "(range table 2^16 in 2^22 rows, one input)", "msm_window_bits": c,
+                               "(range table 2^16 in 2^22 rows, one input); logup eval_h term over 2^24 rows",
+                   "msm_window_bits": c,
                    "parallelism": "single GPU"},
         "primitives_ms": {k: round(v, 4) for k, v in prim_ms.items()},
         "msm_phases_ms": {k: round(v, 4) for k, v in phases.items()},
         "roofline": roofline,
+        "secondary_rooflines": secondary,
     }
     if not args.no_batch:
         out["batched"] = {"msm": bench_batched_msm(L, points, max(1, args.steps // 2)),
