@@ -344,6 +344,81 @@ __device__ __forceinline__ void x29_acc_aff(X29 &p, bool &empty, const F29 &x2, 
     p.y = Y3;
 }
 
+// p += q for two XYZZ points (add-2008-s); the flags mark the identity.
+// Every coordinate is < 2p, as F29 results are.
+__device__ __forceinline__ void x29_add(X29 &p, bool &pe, const X29 &q, bool qe) {
+    if (qe) return;
+    if (pe) {
+        p = q;
+        pe = false;
+        return;
+    }
+    F29 U1 = f29_mul(p.x, q.zz);
+    F29 U2 = f29_mul(q.x, p.zz);
+    F29 S1 = f29_mul(p.y, q.zzz);
+    F29 S2 = f29_mul(q.y, p.zzz);
+    F29 P = f29_sub(U2, U1);
+    F29 R = f29_sub(S2, S1);
+    if (f29_is_zero(P)) {
+        if (f29_is_zero(R))
+            p = x29_dbl(p);
+        else
+            x29_clear(p, pe);
+        return;
+    }
+    F29 PP = f29_sqr(P);
+    F29 PPP = f29_mul(P, PP);
+    F29 Q = f29_mul(U1, PP);
+    F29 X3 = f29_sub(f29_sub(f29_sqr(R), PPP), f29_dbl(Q));
+    const F29 zero = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    F29 Y3 = f29_mul2(R, f29_sub(Q, X3), f29_sub(zero, S1), PPP);
+    p.zz = f29_mul(f29_mul(p.zz, q.zz), PP);
+    p.zzz = f29_mul(f29_mul(p.zzz, q.zzz), PPP);
+    p.x = X3;
+    p.y = Y3;
+}
+
+// Storage form of an X29 point between kernels: the 36 limbs, 144 B,
+// loaded and stored as nine 16-byte words; zz = 0 marks the identity.
+struct alignas(16) X29S {
+    uint4 q[9];
+};
+__device__ __forceinline__ void x29_store(X29S *d, const X29 &p, bool empty) {
+    uint32_t w[36];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        w[i] = p.x.l[i];
+        w[9 + i] = p.y.l[i];
+        w[18 + i] = empty ? 0u : p.zz.l[i];
+        w[27 + i] = p.zzz.l[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) d->q[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+__device__ __forceinline__ X29 x29_load(const X29S *s, bool &empty) {
+    uint32_t w[36];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const uint4 v = s->q[k];
+        w[4 * k] = v.x;
+        w[4 * k + 1] = v.y;
+        w[4 * k + 2] = v.z;
+        w[4 * k + 3] = v.w;
+    }
+    X29 p;
+    uint32_t z = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        p.x.l[i] = w[i];
+        p.y.l[i] = w[9 + i];
+        p.zz.l[i] = w[18 + i];
+        p.zzz.l[i] = w[27 + i];
+        z |= w[18 + i];
+    }
+    empty = z == 0;
+    return p;
+}
+
 // back to the library's XYZZ (canonical 32-bit Montgomery coordinates)
 __device__ __forceinline__ G1Xyzz x29_to_xyzz(const X29 &p, bool empty) {
     if (empty) return xyzz_id();

@@ -123,13 +123,13 @@ MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c) {
     p.size[A_HIST] = (size_t)p.nhist * 4;
     p.size[A_STARTS] = (size_t)(p.nhist + 1) * 4;
     p.size[A_SCAN] = ((size_t)(p.nhist + 1 + 1023) / 1024 + 1) * 4;
-    p.size[A_BUCKETS] = (size_t)p.NB * sizeof(G1Xyzz);
-    p.size[A_PHEAD] = (size_t)p.chunks * sizeof(G1Xyzz);
-    p.size[A_PTAIL] = (size_t)p.chunks * sizeof(G1Xyzz);
+    p.size[A_BUCKETS] = (size_t)p.NB * sizeof(X29S);
+    p.size[A_PHEAD] = (size_t)p.chunks * sizeof(X29S);
+    p.size[A_PTAIL] = (size_t)p.chunks * sizeof(X29S);
     p.size[A_HB] = (size_t)p.chunks * 4;
     p.size[A_TB] = (size_t)p.chunks * 4;
-    p.size[A_WS1] = (size_t)count * p.W * p.T1 * sizeof(G1Xyzz);
-    p.size[A_SUM1] = (size_t)count * p.W * p.T1 * sizeof(G1Xyzz);
+    p.size[A_WS1] = (size_t)count * p.W * p.T1 * sizeof(X29S);
+    p.size[A_SUM1] = (size_t)count * p.W * p.T1 * sizeof(X29S);
     // [count, (bucket, first chunk, last chunk) per bucket spanning > kWalk chunks]
     p.size[A_LONG] = (size_t)(1 + 3 * (p.chunks / kWalk + 1)) * 4;
     return p;
@@ -529,8 +529,8 @@ __device__ __forceinline__ uint32_t find_bucket(const uint32_t *__restrict__ off
 __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restrict__ entries,
                                                         const uint32_t *__restrict__ off, uint32_t NB, uint32_t WB,
                                                         const MsmItemDev *__restrict__ items,
-                                                        G1Xyzz *__restrict__ buckets, G1Xyzz *__restrict__ phead,
-                                                        G1Xyzz *__restrict__ ptail, uint32_t *__restrict__ hb,
+                                                        X29S *__restrict__ buckets, X29S *__restrict__ phead,
+                                                        X29S *__restrict__ ptail, uint32_t *__restrict__ hb,
                                                         uint32_t *__restrict__ tb, uint64_t nchunks) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nchunks) return;
@@ -574,16 +574,16 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
                 x29_acc_aff(acc, empty, f29_from32(qcur.x), f29_from32(y));
             }
             if (pos + 1 == bend || !more) {
-                // the run of bucket b ends at this step
-                const G1Xyzz sum = x29_to_xyzz(acc, empty);
+                // the run of bucket b ends at this step (stored in the 29-bit
+                // form: the fixup and the reduction continue in it)
                 if (first && before) {
-                    xyzz_store(&phead[t], sum);
+                    x29_store(&phead[t], acc, empty);
                     head_bucket = b;
                 } else if (bend > e) {
-                    xyzz_store(&ptail[t], sum);
+                    x29_store(&ptail[t], acc, empty);
                     tail_bucket = b;
                 } else {
-                    xyzz_store(&buckets[b], sum);
+                    x29_store(&buckets[b], acc, empty);
                 }
                 first = false;
                 x29_clear(acc, empty);
@@ -615,8 +615,8 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
 // nchunks / kWalk) are listed for k_msm_fixup_long, one workgroup each, so a
 // heavy bucket (e.g. the multiplicities of a lookup, all in a handful of
 // buckets) never becomes a serial chain.
-__global__ __launch_bounds__(256) void k_msm_fixup(G1Xyzz *__restrict__ buckets, const G1Xyzz *__restrict__ phead,
-                                                   const G1Xyzz *__restrict__ ptail, const uint32_t *__restrict__ tb,
+__global__ __launch_bounds__(256) void k_msm_fixup(X29S *__restrict__ buckets, const X29S *__restrict__ phead,
+                                                   const X29S *__restrict__ ptail, const uint32_t *__restrict__ tb,
                                                    const uint32_t *__restrict__ off, uint32_t *__restrict__ longs,
                                                    uint64_t nchunks) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -631,29 +631,46 @@ __global__ __launch_bounds__(256) void k_msm_fixup(G1Xyzz *__restrict__ buckets,
         longs[3 + 3 * i] = (uint32_t)last;
         return;
     }
-    G1Xyzz acc = xyzz_load(&ptail[t]);
-    for (uint64_t u = t + 1; u <= last; u++) acc = xyzz_add(acc, xyzz_load(&phead[u]));
-    xyzz_store(&buckets[b], acc);
+    bool ae, qe;
+    X29 acc = x29_load(&ptail[t], ae);
+    for (uint64_t u = t + 1; u <= last; u++) {
+        const X29 q = x29_load(&phead[u], qe);
+        x29_add(acc, ae, q, qe);
+    }
+    x29_store(&buckets[b], acc, ae);
 }
 
 // one workgroup per listed long bucket: ptail[t] + sum of phead[t+1..last]
-__global__ __launch_bounds__(256) void k_msm_fixup_long(G1Xyzz *__restrict__ buckets,
-                                                        const G1Xyzz *__restrict__ phead,
-                                                        const G1Xyzz *__restrict__ ptail,
+__global__ __launch_bounds__(256) void k_msm_fixup_long(X29S *__restrict__ buckets,
+                                                        const X29S *__restrict__ phead,
+                                                        const X29S *__restrict__ ptail,
                                                         const uint32_t *__restrict__ longs) {
-    __shared__ G1Xyzz sh[256];
+    __shared__ X29S sh[256];
     const uint32_t count = longs[0];
     for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
         const uint32_t b = longs[1 + 3 * i], t = longs[2 + 3 * i], last = longs[3 + 3 * i];
-        G1Xyzz acc = threadIdx.x == 0 ? xyzz_load(&ptail[t]) : xyzz_id();
-        for (uint32_t u = t + 1 + threadIdx.x; u <= last; u += blockDim.x) acc = xyzz_add(acc, xyzz_load(&phead[u]));
-        sh[threadIdx.x] = acc;
+        bool ae = true, qe;
+        X29 acc;
+        if (threadIdx.x == 0) acc = x29_load(&ptail[t], ae);
+        for (uint32_t u = t + 1 + threadIdx.x; u <= last; u += blockDim.x) {
+            const X29 q = x29_load(&phead[u], qe);
+            x29_add(acc, ae, q, qe);
+        }
+        x29_store(&sh[threadIdx.x], acc, ae);
         __syncthreads();
         for (uint32_t st = 128; st; st >>= 1) {
-            if (threadIdx.x < st) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + st]);
+            if (threadIdx.x < st) {
+                X29 l = x29_load(&sh[threadIdx.x], ae);
+                const X29 r = x29_load(&sh[threadIdx.x + st], qe);
+                x29_add(l, ae, r, qe);
+                x29_store(&sh[threadIdx.x], l, ae);
+            }
             __syncthreads();
         }
-        if (threadIdx.x == 0) xyzz_store(&buckets[b], sh[0]);
+        if (threadIdx.x == 0) {
+            const X29 r = x29_load(&sh[0], ae);
+            x29_store(&buckets[b], r, ae);
+        }
         __syncthreads();
     }
 }
@@ -664,22 +681,26 @@ __global__ __launch_bounds__(256) void k_msm_fixup_long(G1Xyzz *__restrict__ buc
 
 // ---- reduction -------------------------------------------------------------
 // one thread per (item window, segment of kL1 buckets); W = items * windows
-__global__ __launch_bounds__(64) void k_msm_reduce_l1(const G1Xyzz *__restrict__ buckets, const uint32_t *__restrict__ off,
-                                uint32_t W, uint32_t B, uint32_t T1, G1Xyzz *__restrict__ ws1,
-                                G1Xyzz *__restrict__ sum1) {
+__global__ __launch_bounds__(64) void k_msm_reduce_l1(const X29S *__restrict__ buckets, const uint32_t *__restrict__ off,
+                                uint32_t W, uint32_t B, uint32_t T1, X29S *__restrict__ ws1,
+                                X29S *__restrict__ sum1) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= W * T1) return;
     uint32_t w = g / T1, s = g % T1;
-    G1Xyzz run = xyzz_id(), acc = xyzz_id();
+    X29 run, acc;
+    bool re = true, ae = true, be;
     uint32_t lo = s * kL1;
     uint32_t hi = lo + kL1 < B ? lo + kL1 : B;
     for (uint32_t b = hi; b-- > lo;) {
         uint32_t idx = w * B + b;
-        if (off[idx + 1] != off[idx]) run = xyzz_add(run, xyzz_load(&buckets[idx]));
-        acc = xyzz_add(acc, run);
+        if (off[idx + 1] != off[idx]) {
+            const X29 q = x29_load(&buckets[idx], be);
+            x29_add(run, re, q, be);
+        }
+        x29_add(acc, ae, run, re);
     }
-    xyzz_store(&ws1[g], acc);
-    xyzz_store(&sum1[g], run);
+    x29_store(&ws1[g], acc, ae);
+    x29_store(&sum1[g], run, re);
 }
 
 // grid (nbits + 1, items * W, parts): q < nbits -> sum of sum1[g][s] over s
@@ -694,51 +715,78 @@ __device__ __forceinline__ G1Xyzz *slot_q(uint8_t *const *slots, uint32_t g, uin
     return &q[(size_t)(g % W) * (nbits + 1) + bit];
 }
 
-__global__ __launch_bounds__(256) void k_msm_masked_sums(const G1Xyzz *__restrict__ ws1,
-                                                         const G1Xyzz *__restrict__ sum1, uint32_t T1,
-                                                         uint32_t nbits, uint32_t W,
-                                                         uint8_t *const *__restrict__ slots,
-                                                         G1Xyzz *__restrict__ qpart) {
-    __shared__ G1Xyzz sh[256];
-    const uint32_t bit = blockIdx.x, g = blockIdx.y, parts = gridDim.z, part = blockIdx.z;
-    const uint32_t span = T1 / parts, s0 = part * span;
-    const G1Xyzz *src = (bit == nbits ? ws1 : sum1) + (size_t)g * T1;
-    G1Xyzz acc = xyzz_id();
-    for (uint32_t s = s0 + threadIdx.x; s < s0 + span; s += blockDim.x) {
-        if (bit == nbits || ((s >> bit) & 1u)) acc = xyzz_add(acc, xyzz_load(&src[s]));
-    }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
+// LDS tree over the 256 threads' X29 values in sh; the sum ends in sh[0]
+__device__ __forceinline__ void x29_block_tree(X29S *sh) {
+    bool ae, qe;
     for (uint32_t st = 128; st; st >>= 1) {
-        if (threadIdx.x < st) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + st]);
+        if (threadIdx.x < st) {
+            X29 l = x29_load(&sh[threadIdx.x], ae);
+            const X29 r = x29_load(&sh[threadIdx.x + st], qe);
+            x29_add(l, ae, r, qe);
+            x29_store(&sh[threadIdx.x], l, ae);
+        }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void k_msm_masked_sums(const X29S *__restrict__ ws1,
+                                                         const X29S *__restrict__ sum1, uint32_t T1,
+                                                         uint32_t nbits, uint32_t W,
+                                                         uint8_t *const *__restrict__ slots,
+                                                         X29S *__restrict__ qpart) {
+    __shared__ X29S sh[256];
+    const uint32_t bit = blockIdx.x, g = blockIdx.y, parts = gridDim.z, part = blockIdx.z;
+    const uint32_t span = T1 / parts, s0 = part * span;
+    const X29S *src = (bit == nbits ? ws1 : sum1) + (size_t)g * T1;
+    X29 acc;
+    bool ae = true, qe;
+    for (uint32_t s = s0 + threadIdx.x; s < s0 + span; s += blockDim.x) {
+        if (bit == nbits || ((s >> bit) & 1u)) {
+            const X29 q = x29_load(&src[s], qe);
+            x29_add(acc, ae, q, qe);
+        }
+    }
+    x29_store(&sh[threadIdx.x], acc, ae);
+    __syncthreads();
+    x29_block_tree(sh);
     if (threadIdx.x == 0) {
+        const X29 r = x29_load(&sh[0], ae);
         if (parts == 1)
-            xyzz_store(slot_q(slots, g, W, nbits, bit), sh[0]);
+            xyzz_store(slot_q(slots, g, W, nbits, bit), x29_to_xyzz(r, ae));
         else
-            xyzz_store(&qpart[((size_t)g * (nbits + 1) + bit) * parts + part], sh[0]);
+            x29_store(&qpart[((size_t)g * (nbits + 1) + bit) * parts + part], r, ae);
     }
 }
 
 // the parts of each (item window, bit) sum: a workgroup takes 256 / parts
 // sums, one thread per part, and adds them as a tree in LDS
-__global__ __launch_bounds__(256) void k_msm_masked_merge(const G1Xyzz *__restrict__ qpart, uint32_t parts,
+__global__ __launch_bounds__(256) void k_msm_masked_merge(const X29S *__restrict__ qpart, uint32_t parts,
                                                           uint32_t nbits, uint32_t W, uint32_t gw,
                                                           uint8_t *const *__restrict__ slots) {
-    __shared__ G1Xyzz sh[256];
+    __shared__ X29S sh[256];
     const uint32_t per = 256 / parts, z = threadIdx.x % parts;
     const uint64_t total = (uint64_t)gw * (nbits + 1);
     const uint64_t i = (uint64_t)blockIdx.x * per + threadIdx.x / parts;
     const bool live = i < total;
-    sh[threadIdx.x] = live ? xyzz_load(&qpart[i * parts + z]) : xyzz_id();
+    bool ae = true, qe;
+    X29 v;
+    if (live) v = x29_load(&qpart[i * parts + z], ae);
+    x29_store(&sh[threadIdx.x], v, ae);
     __syncthreads();
     for (uint32_t st = parts / 2; st; st >>= 1) {
-        if (z < st) sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + st]);
+        if (z < st) {
+            X29 l = x29_load(&sh[threadIdx.x], ae);
+            const X29 r = x29_load(&sh[threadIdx.x + st], qe);
+            x29_add(l, ae, r, qe);
+            x29_store(&sh[threadIdx.x], l, ae);
+        }
         __syncthreads();
     }
-    if (live && z == 0) xyzz_store(slot_q(slots, (uint32_t)(i / (nbits + 1)), W, nbits, (uint32_t)(i % (nbits + 1))),
-                                   sh[threadIdx.x]);
+    if (live && z == 0) {
+        const X29 r = x29_load(&sh[threadIdx.x], ae);
+        xyzz_store(slot_q(slots, (uint32_t)(i / (nbits + 1)), W, nbits, (uint32_t)(i % (nbits + 1))),
+                   x29_to_xyzz(r, ae));
+    }
 }
 
 struct SliceHeader {
@@ -946,13 +994,13 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     uint32_t *entries = (uint32_t *)p.ptr[A_ENTRIES];
     uint32_t *digits = entries;  // dead before the final entries are written
     uint2 *tmp = (uint2 *)p.ptr[A_TMP];  // dead after k_msm_sort_lo
-    G1Xyzz *buckets = (G1Xyzz *)p.ptr[A_BUCKETS];
-    G1Xyzz *phead = (G1Xyzz *)p.ptr[A_PHEAD];
-    G1Xyzz *ptail = (G1Xyzz *)p.ptr[A_PTAIL];
+    X29S *buckets = (X29S *)p.ptr[A_BUCKETS];
+    X29S *phead = (X29S *)p.ptr[A_PHEAD];
+    X29S *ptail = (X29S *)p.ptr[A_PTAIL];
     uint32_t *hb = (uint32_t *)p.ptr[A_HB];
     uint32_t *tb = (uint32_t *)p.ptr[A_TB];
-    G1Xyzz *ws1 = (G1Xyzz *)p.ptr[A_WS1];
-    G1Xyzz *sum1 = (G1Xyzz *)p.ptr[A_SUM1];
+    X29S *ws1 = (X29S *)p.ptr[A_WS1];
+    X29S *sum1 = (X29S *)p.ptr[A_SUM1];
     uint32_t *scan_sums = (uint32_t *)p.ptr[A_SCAN];
     uint32_t *longs = (uint32_t *)p.ptr[A_LONG];
 
