@@ -231,3 +231,41 @@ def test_msm_heavy_buckets(gpu, zo, log_n, values):
     assert (_affine_of(zo, _zk_msm(pts, sc)) == exp).all()
     got = zo.g1_to_affine(_zk_msm_batch([pts, pts], [sc, sc]))
     assert (got[0:1] == exp).all() and (got[1:2] == exp).all()
+
+
+def _zk_msm_dev(dp, ds, n, c=0):
+    """zk_msm over device arrays (torch tensors / pointer offsets in bytes)"""
+    import torch
+    from zkprover import device as dv
+    from zkprover._lib import call, lib
+    need = lib().zk_msm_scratch_bytes
+    need.restype = ctypes.c_size_t
+    need.argtypes = [ctypes.c_int, ctypes.c_int]
+    nbytes = need(n, c)
+    scratch = torch.empty(nbytes, dtype=torch.uint8, device=dv.device())
+    out = torch.zeros(OUT_BYTES, dtype=torch.uint8, device=dv.device())
+    lib().zk_msm.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                      ctypes.c_size_t, ctypes.c_void_p]
+    call("zk_msm", out.data_ptr(), dp, ds, n, c, scratch.data_ptr(), nbytes, dv.stream_handle())
+    return out[:96].cpu().numpy().view(np.uint64).reshape(1, 12).copy()
+
+
+def test_msm_two_entry_layouts_agree(gpu, zo):
+    """above 2^23 scalars the sort passes (index, low bits) as two words,
+    below as one packed word: an MSM of 2^23 + 3 points (two-word layout)
+    equals the sum of the MSMs of its two halves (one-word layout)"""
+    import torch
+    from zkprover import device as dv
+    from zkprover._lib import call, lib
+    n = (1 << 23) + 3
+    lib().zk_gen_points.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_void_p]
+    lib().zk_rand_fr.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_void_p]
+    pts = torch.empty(n * 64, dtype=torch.uint8, device=dv.device())
+    sc = torch.empty(n * 32, dtype=torch.uint8, device=dv.device())
+    call("zk_gen_points", pts.data_ptr(), n, 91, dv.stream_handle())
+    call("zk_rand_fr", sc.data_ptr(), n, 92, dv.stream_handle())
+    h = 1 << 22
+    full = _zk_msm_dev(pts.data_ptr(), sc.data_ptr(), n)
+    a = _zk_msm_dev(pts.data_ptr(), sc.data_ptr(), h)
+    b = _zk_msm_dev(pts.data_ptr() + h * 64, sc.data_ptr() + h * 32, n - h)
+    assert (zo.g1_to_affine(full) == zo.g1_to_affine(zo.g1_add(a, b))).all()

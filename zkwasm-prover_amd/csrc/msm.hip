@@ -74,6 +74,7 @@ constexpr int kFirstPhase2 = A_BUCKETS;
 
 constexpr uint32_t kSortTile = 4096;  // scalars per histogram / scatter tile
 constexpr uint32_t kDigitBlock = 256;  // scalars per digit-kernel block
+constexpr uint32_t kPackedMaxN = 1u << 23;  // items up to this size sort with one-word entries
 
 // One MSM of a batch as the kernels see it (built on the device).
 struct MsmItemDev {
@@ -87,6 +88,7 @@ struct MsmPlan {
     uint32_t count, c, W, B, T1, nbits;
     uint32_t hi_bits, lo_bits, nbins;
     uint64_t ntot, nblk, ntiles, nhist, NB, max_entries, chunks;
+    bool packed;  // every item has at most kPackedMaxN scalars: one-word sort entries
     size_t size[A_COUNT];
     uint8_t *ptr[A_COUNT];
 };
@@ -105,7 +107,9 @@ MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c) {
     p.hi_bits = c - 1 > 8 ? c - 1 - 8 : 0;
     p.lo_bits = (c - 1) - p.hi_bits;
     p.nbins = 1u << p.hi_bits;
+    p.packed = true;
     for (uint32_t m = 0; m < count; m++) {
+        if (ns[m] > kPackedMaxN) p.packed = false;
         p.ntot += ns[m];
         p.nblk += (ns[m] + kDigitBlock - 1) / kDigitBlock;
         p.ntiles += (ns[m] + kSortTile - 1) / kSortTile;
@@ -119,7 +123,7 @@ MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c) {
     p.size[A_OFFSETS] = (size_t)(p.NB + 1) * 4;
     // digits (window-major per item, ntot * W u32) share this array with the entries
     p.size[A_ENTRIES] = (size_t)p.max_entries * 4;
-    p.size[A_TMP] = (size_t)p.max_entries * 8;
+    p.size[A_TMP] = (size_t)p.max_entries * (p.packed ? 4 : 8);
     p.size[A_HIST] = (size_t)p.nhist * 4;
     p.size[A_STARTS] = (size_t)(p.nhist + 1) * 4;
     p.size[A_SCAN] = ((size_t)(p.nhist + 1 + 1023) / 1024 + 1) * 4;
@@ -299,65 +303,6 @@ __global__ __launch_bounds__(256) void k_msm_hist(const MsmItemDev *__restrict__
     for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) hist[hist_index(it, W, nbins, w, k, tl)] = h[k];
 }
 
-// per (tile, window): place each nonzero digit into its (item, window, bin)
-// group; tmp entry = (low bucket bits, index | sign).  The tile is first
-// counting-sorted by bin in LDS (the bin counts are the histogram's), then
-// written out bin run by bin run, so consecutive lanes store to consecutive
-// addresses instead of 256 scattered 8-byte writes per wave.
-__global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__restrict__ items, uint32_t count,
-                                                        uint32_t W, const uint32_t *__restrict__ digits,
-                                                        uint32_t lo_bits, uint32_t nbins,
-                                                        const uint32_t *__restrict__ hist,
-                                                        const uint32_t *__restrict__ starts, uint2 *__restrict__ tmp) {
-    __shared__ uint32_t gstart[256], loff[256], lcur[256];
-    __shared__ uint2 stage[kSortTile];
-    __shared__ uint32_t total;
-    const uint32_t w = blockIdx.y;
-    const uint32_t m = item_of(items, count, blockIdx.x, true);
-    const MsmItemDev it = items[m];
-    const uint32_t tl = blockIdx.x - it.tile0;
-    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) {
-        const uint64_t hi = hist_index(it, W, nbins, w, k, tl);
-        gstart[k] = starts[hi];
-        lcur[k] = hist[hi];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (uint32_t k = 0; k < nbins; k++) {
-            const uint32_t c = lcur[k];
-            loff[k] = run;
-            lcur[k] = run;
-            run += c;
-        }
-        total = run;
-    }
-    __syncthreads();
-    const uint32_t *dw = digits + it.dig0 + (size_t)w * it.n;
-    const uint32_t lo_mask = (1u << lo_bits) - 1;
-    const uint32_t begin = tl * kSortTile;
-    const uint32_t end = begin + kSortTile < it.n ? begin + kSortTile : it.n;
-    for (uint32_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-        uint32_t d = dw[i];
-        if (d) {
-            uint32_t b = (d & 0x7fffffffu) - 1;
-            uint32_t p = atomicAdd(&lcur[b >> lo_bits], 1u);
-            stage[p] = make_uint2(b & lo_mask, i | (d & 0x80000000u));
-        }
-    }
-    __syncthreads();
-    const uint32_t tot = total;
-    for (uint32_t p = threadIdx.x; p < tot; p += blockDim.x) {
-        // bin of position p: the last k with loff[k] <= p (that bin is non-empty)
-        uint32_t lo = 0, hi = nbins;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (loff[mid] <= p) lo = mid; else hi = mid;
-        }
-        tmp[gstart[lo] + (p - loff[lo])] = stage[p];
-    }
-}
-
 // exclusive scan of x[0..256) in LDS by 256 threads (x[k] = 0 past the used
 // range); t is a 256-entry LDS scratch
 __device__ __forceinline__ void lds_exclusive_scan256(uint32_t *x, uint32_t *t) {
@@ -375,17 +320,91 @@ __device__ __forceinline__ void lds_exclusive_scan256(uint32_t *x, uint32_t *t) 
     __syncthreads();
 }
 
+// Entries between the two sort levels: (low bucket bits, scalar index |
+// sign << 31), as a pair of words, or packed into one word when every item
+// has at most 2^23 scalars (sign | low bits << 23 | index), which halves the
+// traffic of the pass between the two levels.
+template <bool kPacked> struct TmpEnt;
+template <> struct TmpEnt<false> {
+    using T = uint2;
+    static __device__ __forceinline__ T make(uint32_t lo, uint32_t v) { return make_uint2(lo, v); }
+    static __device__ __forceinline__ uint32_t key(const T &e) { return e.x; }
+    static __device__ __forceinline__ uint32_t val(const T &e) { return e.y; }
+};
+template <> struct TmpEnt<true> {
+    using T = uint32_t;
+    static __device__ __forceinline__ T make(uint32_t lo, uint32_t v) {
+        return (v & 0x80000000u) | (lo << 23) | (v & 0x7fffffu);
+    }
+    static __device__ __forceinline__ uint32_t key(const T &e) { return (e >> 23) & 0xffu; }
+    static __device__ __forceinline__ uint32_t val(const T &e) { return e & 0x807fffffu; }
+};
+// per (tile, window): place each nonzero digit into its (item, window, bin)
+// group; tmp entry = (low bucket bits, index | sign).  The tile is first
+// counting-sorted by bin in LDS (the bin counts are the histogram's), then
+// written out bin run by bin run, so consecutive lanes store to consecutive
+// addresses instead of 256 scattered 8-byte writes per wave.
+template <bool kPacked>
+__global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__restrict__ items, uint32_t count,
+                                                        uint32_t W, const uint32_t *__restrict__ digits,
+                                                        uint32_t lo_bits, uint32_t nbins,
+                                                        const uint32_t *__restrict__ hist,
+                                                        const uint32_t *__restrict__ starts,
+                                                        typename TmpEnt<kPacked>::T *__restrict__ tmp) {
+    using E = TmpEnt<kPacked>;
+    __shared__ uint32_t gstart[256], loff[256], lcur[256], sc[256];
+    __shared__ typename E::T stage[kSortTile];
+    __shared__ uint8_t sbin[kSortTile];
+    const uint32_t w = blockIdx.y;
+    const uint32_t m = item_of(items, count, blockIdx.x, true);
+    const MsmItemDev it = items[m];
+    const uint32_t tl = blockIdx.x - it.tile0;
+    const uint32_t tid = threadIdx.x;
+    uint32_t cnt = 0;
+    if (tid < nbins) {
+        const uint64_t hi = hist_index(it, W, nbins, w, tid, tl);
+        gstart[tid] = starts[hi];
+        cnt = hist[hi];
+    }
+    loff[tid] = cnt;  // bins past nbins count 0
+    __syncthreads();
+    lds_exclusive_scan256(loff, sc);
+    lcur[tid] = loff[tid];
+    __syncthreads();
+    const uint32_t *dw = digits + it.dig0 + (size_t)w * it.n;
+    const uint32_t lo_mask = (1u << lo_bits) - 1;
+    const uint32_t begin = tl * kSortTile;
+    const uint32_t end = begin + kSortTile < it.n ? begin + kSortTile : it.n;
+    for (uint32_t i = begin + tid; i < end; i += blockDim.x) {
+        uint32_t d = dw[i];
+        if (d) {
+            uint32_t b = (d & 0x7fffffffu) - 1;
+            uint32_t p = atomicAdd(&lcur[b >> lo_bits], 1u);
+            stage[p] = E::make(b & lo_mask, i | (d & 0x80000000u));
+            sbin[p] = (uint8_t)(b >> lo_bits);
+        }
+    }
+    __syncthreads();
+    const uint32_t tot = lcur[nbins - 1];  // the last bin's cursor ends at the tile's total
+    for (uint32_t p = tid; p < tot; p += blockDim.x) {
+        const uint32_t k = sbin[p];
+        tmp[gstart[k] + (p - loff[k])] = stage[p];
+    }
+}
+
 // per (item, window, bin) group: counting sort on the low bucket bits.  The
 // group histogram gives every bucket's offset; the entries are then placed
 // tile by tile (4096 at a time): each tile is counting-sorted in LDS and
 // written out bucket run by bucket run, so stores come in contiguous runs
 // rather than one 4-byte scatter per entry.  Also emits the bucket offsets.
 constexpr uint32_t kLoTile = 4096;
+template <bool kPacked>
 __global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restrict__ items, uint32_t W, uint32_t nbins,
-                                                     const uint2 *__restrict__ tmp,
+                                                     const typename TmpEnt<kPacked>::T *__restrict__ tmp,
                                                      const uint32_t *__restrict__ starts, uint32_t lo_bits,
                                                      uint32_t B, uint32_t ngroups, uint32_t *__restrict__ entries,
                                                      uint32_t *__restrict__ offsets) {
+    using E = TmpEnt<kPacked>;
     __shared__ uint32_t h[256], cur[256], toff[256], tcur[256], sc[256];
     __shared__ uint32_t stage[kLoTile];
     __shared__ uint8_t skey[kLoTile];
@@ -400,7 +419,7 @@ __global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restric
     const uint32_t ge = starts[idx + it.ntiles];  // the next group's start (or the total)
     h[tid] = 0;
     __syncthreads();
-    for (uint32_t p = gs + tid; p < ge; p += blockDim.x) atomicAdd(&h[tmp[p].x], 1u);
+    for (uint32_t p = gs + tid; p < ge; p += blockDim.x) atomicAdd(&h[E::key(tmp[p])], 1u);
     __syncthreads();
     cur[tid] = h[tid];
     __syncthreads();
@@ -413,7 +432,7 @@ __global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restric
     for (uint32_t t0 = gs; t0 < ge; t0 += kLoTile) {
         const uint32_t cnt = ge - t0 < kLoTile ? ge - t0 : kLoTile;
         // the tile in registers: 16 entries per thread
-        uint2 v[kLoTile / 256];
+        typename E::T v[kLoTile / 256];
         h[tid] = 0;
         __syncthreads();
 #pragma unroll
@@ -421,7 +440,7 @@ __global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restric
             const uint32_t q = k * 256 + tid;
             if (q < cnt) {
                 v[k] = tmp[t0 + q];
-                atomicAdd(&h[v[k].x], 1u);
+                atomicAdd(&h[E::key(v[k])], 1u);
             }
         }
         __syncthreads();
@@ -434,9 +453,9 @@ __global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restric
         for (uint32_t k = 0; k < kLoTile / 256; k++) {
             const uint32_t q = k * 256 + tid;
             if (q < cnt) {
-                const uint32_t at = atomicAdd(&tcur[v[k].x], 1u);
-                stage[at] = v[k].y;
-                skey[at] = (uint8_t)v[k].x;
+                const uint32_t at = atomicAdd(&tcur[E::key(v[k])], 1u);
+                stage[at] = E::val(v[k]);
+                skey[at] = (uint8_t)E::key(v[k]);
             }
         }
         __syncthreads();
@@ -993,7 +1012,7 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     uint32_t *starts = (uint32_t *)p.ptr[A_STARTS];  // dead after k_msm_sort_lo
     uint32_t *entries = (uint32_t *)p.ptr[A_ENTRIES];
     uint32_t *digits = entries;  // dead before the final entries are written
-    uint2 *tmp = (uint2 *)p.ptr[A_TMP];  // dead after k_msm_sort_lo
+    void *tmp = p.ptr[A_TMP];  // dead after k_msm_sort_lo
     X29S *buckets = (X29S *)p.ptr[A_BUCKETS];
     X29S *phead = (X29S *)p.ptr[A_PHEAD];
     X29S *ptail = (X29S *)p.ptr[A_PTAIL];
@@ -1043,12 +1062,20 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     ZK_CHECK(prof_mark(prof, PH_SCAN, stream));
     ZK_CHECK(launch_scan(hist, (uint32_t)p.nhist, starts, nullptr, scan_sums, stream));
     ZK_CHECK(prof_mark(prof, PH_SCATTER, stream));
-    k_msm_scatter_hi<<<dim3((uint32_t)p.ntiles, p.W), 256, 0, stream>>>(items, count, p.W, digits, p.lo_bits,
-                                                                        p.nbins, hist, starts, tmp);
-    ZK_CHECK(hipGetLastError());
     const uint32_t ngroups = count * p.W * p.nbins;
-    k_msm_sort_lo<<<ngroups, 256, 0, stream>>>(items, p.W, p.nbins, tmp, starts, p.lo_bits, p.B, ngroups, entries,
-                                                offsets);
+    if (p.packed) {
+        k_msm_scatter_hi<true><<<dim3((uint32_t)p.ntiles, p.W), 256, 0, stream>>>(
+            items, count, p.W, digits, p.lo_bits, p.nbins, hist, starts, (uint32_t *)tmp);
+        ZK_CHECK(hipGetLastError());
+        k_msm_sort_lo<true><<<ngroups, 256, 0, stream>>>(items, p.W, p.nbins, (const uint32_t *)tmp, starts,
+                                                          p.lo_bits, p.B, ngroups, entries, offsets);
+    } else {
+        k_msm_scatter_hi<false><<<dim3((uint32_t)p.ntiles, p.W), 256, 0, stream>>>(
+            items, count, p.W, digits, p.lo_bits, p.nbins, hist, starts, (uint2 *)tmp);
+        ZK_CHECK(hipGetLastError());
+        k_msm_sort_lo<false><<<ngroups, 256, 0, stream>>>(items, p.W, p.nbins, (const uint2 *)tmp, starts,
+                                                           p.lo_bits, p.B, ngroups, entries, offsets);
+    }
     ZK_CHECK(hipGetLastError());
     ZK_CHECK(prof_mark(prof, PH_ACCUMULATE, stream));
     const uint32_t cblocks = (uint32_t)((p.chunks + 255) / 256);
