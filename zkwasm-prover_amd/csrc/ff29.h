@@ -30,6 +30,8 @@ constexpr uint32_t kP29[9] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u
                               0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
 constexpr uint32_t kP2x29[9] = {0x10f9fa8eu, 0x0208c16du, 0x18e5469eu, 0x05aa45a1u, 0x0b0bb2f0u,
                                 0x05b68181u, 0x014dc282u, 0x1cb84c68u, 0x0060c89cu};
+constexpr uint32_t kP4x29[9] = {0x01f3f51cu, 0x041182dbu, 0x11ca8d3cu, 0x0b548b43u, 0x161765e0u,
+                                0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
 constexpr uint32_t kPInv29 = 0x04866389u;
 constexpr uint32_t kPInv5 = 9u;
 constexpr uint32_t kOne29[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u,
@@ -207,6 +209,45 @@ __device__ __forceinline__ F29 f29_sub(const F29 &a, const F29 &b) {
 
 __device__ __forceinline__ F29 f29_dbl(const F29 &a) { return f29_add(a, a); }
 
+// Unreduced forms for values that only feed products (any operand < 8p keeps
+// a * b + c * d below 2^261 p) or one final reduction:
+// a + b, and a - b + 2p (b < 2p + a) or a - b + 4p (b < 4p + a)
+__device__ __forceinline__ F29 f29_add_nr(const F29 &a, const F29 &b) {
+    F29 r;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        uint32_t s = a.l[i] + b.l[i] + c;
+        r.l[i] = s & kM29;
+        c = s >> 29;
+    }
+    return r;
+}
+template <int K>  // K = 2 or 4: adds K p
+__device__ __forceinline__ F29 f29_sub_nr(const F29 &a, const F29 &b) {
+    F29 r;
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        int32_t s = (int32_t)a.l[i] - (int32_t)b.l[i] + (int32_t)(K == 4 ? kP4x29[i] : kP2x29[i]) + br;
+        r.l[i] = (uint32_t)s & kM29;
+        br = s >> 29;
+    }
+    return r;
+}
+// a < 8p -> < 2p
+__device__ __forceinline__ F29 f29_reduce8(const F29 &a) {
+    F29 d;
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        int32_t s = (int32_t)a.l[i] - (int32_t)kP4x29[i] + br;
+        d.l[i] = (uint32_t)s & kM29;
+        br = s >> 29;
+    }
+    return f29_sub2p_if(br < 0 ? a : d);
+}
+
 // a == 0 mod p for a < 2p
 __device__ __forceinline__ bool f29_is_zero(const F29 &a) {
     uint32_t z = 0, q = 0;
@@ -322,9 +363,9 @@ __device__ __forceinline__ void x29_acc_aff(X29 &p, bool &empty, const F29 &x2, 
         return;
     }
     F29 P = f29_sub(U2, p.x);
-    F29 R = f29_sub(S2, p.y);
+    F29 R = f29_sub_nr<2>(S2, p.y);  // < 4p: only multiplied
     if (f29_is_zero(P)) {
-        if (f29_is_zero(R)) {
+        if (f29_is_zero(f29_sub2p_if(R))) {
             p = x29_dbl(p);  // the point equals the accumulator
         } else {
             x29_clear(p, empty);
@@ -334,10 +375,12 @@ __device__ __forceinline__ void x29_acc_aff(X29 &p, bool &empty, const F29 &x2, 
     F29 PP = f29_sqr(P);
     F29 PPP = f29_mul(P, PP);
     F29 Q = f29_mul(p.x, PP);
-    F29 X3 = f29_sub(f29_sub(f29_sqr(R), PPP), f29_dbl(Q));
-    // R (Q - X3) - Y1 PPP as R (Q - X3) + (2p - Y1) PPP: one reduction
+    // X3 = R^2 - PPP - 2Q: (R^2 - PPP + 2p) - 2Q + 4p < 8p, reduced once
+    F29 X3 = f29_reduce8(f29_sub_nr<4>(f29_sub_nr<2>(f29_sqr(R), PPP), f29_add_nr(Q, Q)));
+    // R (Q - X3) - Y1 PPP as R (Q - X3 + 2p) + (2p - Y1) PPP: one reduction,
+    // operands < 4p, < 4p, <= 2p, < 2p
     const F29 zero = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
-    F29 Y3 = f29_mul2(R, f29_sub(Q, X3), f29_sub(zero, p.y), PPP);
+    F29 Y3 = f29_mul2(R, f29_sub_nr<2>(Q, X3), f29_sub_nr<2>(zero, p.y), PPP);
     p.zz = f29_mul(p.zz, PP);
     p.zzz = f29_mul(p.zzz, PPP);
     p.x = X3;
