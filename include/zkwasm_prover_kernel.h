@@ -394,6 +394,21 @@ hipError_t zkp_vanishing_commit(void *h, void *tmp, const void *t_evals, int t_l
  * Horner, folded in place into the last piece of coeffs; xn: device x^n. */
 hipError_t zkp_vanishing_pieces(void *coeffs, const void *xn, int log_n, int quotient_degree, void *h_pieces);
 
+/* ---- Fiat-Shamir transcript (host) -------------------------------------- */
+/* halo2_proofs' Blake2bWrite, which the reference's create_proof drives
+ * (src/lib.rs:205-1194; not part of its CUDA ABI): Blake2b-512 personalised
+ * "Halo2-Transcript"; a point absorbs 0x01 || x || y and a scalar 0x02 || s
+ * (canonical little-endian 32-byte values); a challenge absorbs 0x00, and is
+ * the 64-byte digest of the state so far read little-endian mod r
+ * (from_bytes_wide), returned in Montgomery form.  Host functions: no device
+ * is touched. */
+int zk_blake2b(void *out, int outlen, const void *in, size_t inlen, const void *personal16);
+void *zk_transcript_new(void);
+void zk_transcript_free(void *t);
+int zk_transcript_common_point(void *t, const void *x, const void *y);
+int zk_transcript_common_scalar(void *t, const void *s);
+int zk_transcript_squeeze(void *t, void *mont_out, void *digest64);
+
 /* ---- prover core: one proof's device arithmetic ------------------------ */
 /* The PLONKish benchmark circuit: a*b + c - d = 0 under a selector on the
  * usable rows; `lookups` advice columns range-checked against a fixed table of
@@ -416,6 +431,9 @@ typedef struct {
     double phase_ms[7];  /* witness, advice commit + iNTT, multiplicities,
                             grand products, h evaluation, vanishing, opening */
     double total_ms;     /* phases 1-6 */
+    unsigned char transcript[32]; /* a last challenge's digest: fingerprint of
+                                     every commitment and evaluation */
+    unsigned long long challenges[4][4]; /* beta, gamma, y, x (Montgomery) */
 } zkp_proof_info;
 
 /* Device bytes of the resident key data and of one in-flight proof. */

@@ -84,6 +84,51 @@ def test_prover_core_opening_tamper_detected(core, shplonk):
     assert info.quotient_ok == 1 and info.open_ok == 0
 
 
+def _transcript_model(zo, com, A, L, nz):
+    """beta, gamma, y, x as halo2's Blake2bWrite derives them from the
+    commitments (a model written with hashlib): advice, theta, m, beta,
+    gamma, the grand products, y, the quotient pieces, x"""
+    import hashlib
+    R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+    h = hashlib.blake2b(digest_size=64, person=b"Halo2-Transcript")
+
+    def point(i):
+        x, y = zo.g1a_to_ints(com[i:i + 1])
+        h.update(b"\x01" + x.to_bytes(32, "little") + y.to_bytes(32, "little"))
+
+    def challenge():
+        h.update(b"\x00")
+        return int.from_bytes(h.copy().digest(), "little") % R
+
+    i = 0
+    for _ in range(A):
+        point(i); i += 1
+    challenge()  # theta
+    for _ in range(L):
+        point(i); i += 1
+    beta, gamma = challenge(), challenge()
+    for _ in range(nz):
+        point(i); i += 1
+    y = challenge()
+    for _ in range(3):
+        point(i); i += 1
+    return [beta, gamma, y, challenge()]
+
+
+def test_prover_core_transcript(core, zo):
+    """the challenges are the transcript's: recomputed from the returned
+    commitments with a hashlib model of halo2's Blake2bWrite"""
+    from zkprover.prover import BATCHED, SHPLONK
+    pk, pr = core
+    com, _, info = pr.prove(17)
+    exp = _transcript_model(zo, com, 30, 3, 3 + 3 + 2)
+    got = [zo.from_mont(np.array(info.challenges[c], dtype=np.uint64).reshape(1, 4))[0] for c in range(4)]
+    assert got == exp
+    com2, _, info2 = pr.prove(17, flags=BATCHED | SHPLONK)
+    assert bytes(info.transcript) != bytes(info2.transcript)  # the openings differ
+    assert [list(info2.challenges[c]) for c in range(4)] == [list(info.challenges[c]) for c in range(4)]
+
+
 def test_prover_core_batched_equals_plain_and_deterministic(core):
     from zkprover.prover import BATCHED
     pk, pr = core
@@ -91,6 +136,8 @@ def test_prover_core_batched_equals_plain_and_deterministic(core):
     b = pr.prove(21, flags=0)
     c = pr.prove(22, flags=BATCHED)
     assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
+    assert bytes(a[2].transcript) == bytes(b[2].transcript)
+    assert bytes(a[2].transcript) != bytes(c[2].transcript)
     assert a[2].quotient_ok and b[2].quotient_ok and c[2].quotient_ok
     assert not (a[0] == c[0]).all()
 

@@ -25,8 +25,11 @@
 // table column (values i mod 2^table_log); `perm_cols` advice columns joined
 // by random copy cycles; `shuffles` pairs of advice columns, the second a
 // row permutation of the first on 2^shuffle_log rows.  Rows from
-// u = n - (blinding + 1) on are random blinding rows.  Challenges come from
-// the proof seed (no transcript).  Because the witness satisfies every
+// u = n - (blinding + 1) on are random blinding rows.  Challenges are
+// Fiat-Shamir: squeezed from a Blake2b transcript that absorbs every
+// commitment and evaluation in the reference's order (theta after the advice,
+// beta and gamma after m, y after the grand products, x after the quotient
+// pieces, then the opening's).  Because the witness satisfies every
 // constraint, h(X) is a polynomial of degree < 3n: the coefficients
 // [3n, 4n) of the extended iNTT must vanish, which checks the whole
 // composition at any size.
@@ -37,6 +40,7 @@
 #include <chrono>
 #include <vector>
 
+#include "host_ec.h"
 #include "host_fr.h"
 #include "zkinternal.h"
 #include "zkwasm_prover_kernel.h"
@@ -566,9 +570,6 @@ hipError_t witness(Job &j, hipStream_t st) {
         PCHECK(zk::wit_zero_rows(in, s.R, s.u, st));
         PCHECK(zk::wit_zero_rows(sh, s.R, s.u, st));
     }
-    // challenges
-    PCHECK(zk_rand_fr(j.cf(J_BETA), 4, seed ^ 0xc4a11e46e5ull, st));  // beta, gamma, y, x
-    PCHECK(zk_rand_fr(j.cf(J_V), 3, seed ^ 0x0be11a7e5ull, st));      // v, y', u
     return hipStreamSynchronize(st);
 }
 
@@ -596,15 +597,55 @@ std::vector<zkh::Fr> lagrange(const zkh::Fr *pts, const zkh::Fr *vals, int m) {
     return res;
 }
 
+// ---- Fiat-Shamir (halo2's Blake2bWrite, csrc/transcript.cpp) -------------------
+struct Transcript {
+    void *t = zk_transcript_new();
+    ~Transcript() { zk_transcript_free(t); }
+    // affine commitments as commit() writes them (Montgomery x, y; zeros for
+    // the identity, which halo2 cannot absorb: absorbed here as x = y = 0)
+    void points(const uint8_t *aff, int count) {
+        for (int i = 0; i < count; i++) {
+            zkh::Fp x, y;
+            memcpy(&x, aff + (size_t)i * 64, 32);
+            memcpy(&y, aff + (size_t)i * 64 + 32, 32);
+            const zkh::Fp one_raw{{1, 0, 0, 0}};
+            if (!zkh::fp_is_zero(x) || !zkh::fp_is_zero(y)) {
+                x = zkh::fp_mul(x, one_raw);  // canonical
+                y = zkh::fp_mul(y, one_raw);
+            }
+            zk_transcript_common_point(t, &x, &y);
+        }
+    }
+    void scalars(const zkh::Fr *v, int count) {
+        const zkh::Fr one_raw{{1, 0, 0, 0}};
+        for (int i = 0; i < count; i++) {
+            const zkh::Fr c = zkh::fr_mul(v[i], one_raw);
+            zk_transcript_common_scalar(t, &c);
+        }
+    }
+    zkh::Fr challenge() {
+        zkh::Fr c;
+        zk_transcript_squeeze(t, &c, nullptr);
+        return c;
+    }
+};
+
+// challenges to the job's constant slots (synchronous: the host values are
+// temporaries)
+hipError_t put_challenges(Job &j, int first, const zkh::Fr *v, int count, hipStream_t st) {
+    PCHECK(hipMemcpyAsync(j.cf(first), v, (size_t)count * FR, hipMemcpyHostToDevice, st));
+    return hipStreamSynchronize(st);
+}
+
 // Multi-open of every query (src/multiopen.rs): evaluations of each
 // polynomial at its points (batch_poly_eval, src/eval_poly.rs:18), then GWC
 // (one quotient per point, :68-177) or Shplonk (h(X) and L(X), :283-561) with
-// the divisions by (X - z) on the device.  The challenges come from the proof
-// seed.  open_ok: GWC, every batch polynomial evaluates at its point to the
+// the divisions by (X - z) on the device.  The challenges come from the
+// transcript.  open_ok: GWC, every batch polynomial evaluates at its point to the
 // v-combination of the evaluations; Shplonk, L(u) = 0 -- each holds only if
 // every evaluation and every division is right.
-hipError_t open_phase(Job &j, const void *hpoly, bool shplonk, bool batched, bool tamper, uint8_t *out, int &ncom,
-                      int &open_ok, int &nevals, hipStream_t st) {
+hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, bool batched, bool tamper,
+                      uint8_t *out, int &ncom, int &open_ok, int &nevals, hipStream_t st) {
     using namespace zkh;
     const Pk &pk = *j.pk;
     const Shape &s = pk.sh;
@@ -651,12 +692,23 @@ hipError_t open_phase(Job &j, const void *hpoly, bool shplonk, bool batched, boo
         PCHECK(zk_fr_op((uint8_t *)j.evals + (size_t)(e - 1) * FR, (uint8_t *)j.evals + (size_t)(e - 1) * FR,
                         pk.cf(C_ONE), OP_ADD, 1, st));
     std::vector<Fr> ev(e);
-    Fr ch[3], P[3];
+    Fr P[3];
     PCHECK(hipMemcpyAsync(ev.data(), j.evals, (size_t)e * FR, hipMemcpyDeviceToHost, st));
-    PCHECK(hipMemcpyAsync(ch, j.cf(J_V), 3 * FR, hipMemcpyDeviceToHost, st));  // v, y', u
     for (int q = 0; q < 3; q++) PCHECK(hipMemcpyAsync(&P[q], pts_dev[q], FR, hipMemcpyDeviceToHost, st));
     PCHECK(hipStreamSynchronize(st));
-    const Fr v = ch[0], y = ch[1], u = ch[2];
+    // the evaluations enter the transcript; GWC draws v, Shplonk y' then v
+    // (src/multiopen.rs:85, :302-305); Shplonk's u follows h's commitment
+    tr.scalars(ev.data(), e);
+    Fr v, y = fr_zero();
+    if (shplonk) {
+        y = tr.challenge();
+        v = tr.challenge();
+        const Fr yv[2] = {v, y};  // J_V, J_SY
+        PCHECK(put_challenges(j, J_V, yv, 2, st));
+    } else {
+        v = tr.challenge();
+        PCHECK(put_challenges(j, J_V, &v, 1, st));
+    }
     // Horner in c over the evaluations of `set` at point q, in query order
     auto combine = [&](int set, int q, const Fr &c) {
         int first = 0;
@@ -692,6 +744,7 @@ hipError_t open_phase(Job &j, const void *hpoly, bool shplonk, bool batched, boo
             W.push_back(B);
         }
         PCHECK(commit(j, pk.g, W, out + (size_t)ncom * 64, batched, st));
+        tr.points(out + (size_t)ncom * 64, nsets);
         ncom += nsets;
         std::vector<Fr> got(nsets);
         PCHECK(hipMemcpyAsync(got.data(), oc, (size_t)nsets * FR, hipMemcpyDeviceToHost, st));
@@ -719,26 +772,15 @@ hipError_t open_phase(Job &j, const void *hpoly, bool shplonk, bool batched, boo
         r[set] = lagrange(P, ec, set + 1);
     }
     // host staging of the scalars the device reads: [0, 9) r coefficients of
-    // the three sets, then per set u-evaluation and z_s, then Z_T(u), 1/z_0
+    // the three sets, then (once u is drawn) per set u-evaluation and z_s,
+    // then Z_T(u), 1/z_0
     std::vector<Fr> stage(kOpenCst - 32, fr_zero());
     for (int set = 0; set < nsets; set++)
         for (int c = 0; c <= set; c++) stage[3 * set + c] = r[set][c];
-    Fr zt = fr_one(), z0inv = fr_one();
-    for (int q = 0; q < nsets; q++) zt = fr_mul(zt, fr_sub(u, P[q]));
-    for (int set = 0; set < nsets; set++) {
-        Fr ru = fr_zero();  // r_s(u)
-        for (int c = set; c >= 0; c--) ru = fr_add(fr_mul(ru, u), r[set][c]);
-        Fr zs = fr_one();   // prod over the points outside the set
-        for (int q = set + 1; q < nsets; q++) zs = fr_mul(zs, fr_sub(u, P[q]));
-        stage[9 + 2 * set] = ru;
-        stage[10 + 2 * set] = zs;
-        if (set == 0) z0inv = fr_inv(zs);
-    }
-    stage[15] = zt;
-    stage[16] = z0inv;
-    PCHECK(hipMemcpyAsync(ocs(32), stage.data(), stage.size() * FR, hipMemcpyHostToDevice, st));
+    PCHECK(hipMemcpyAsync(ocs(32), stage.data(), 9 * FR, hipMemcpyHostToDevice, st));
     // point arrays on the device: [8, 11) the points in order
     for (int q = 0; q < nsets; q++) PCHECK(hipMemcpyAsync(ocs(8 + q), pts_dev[q], FR, hipMemcpyDeviceToDevice, st));
+    PCHECK(hipStreamSynchronize(st));  // the staged coefficients
 
     PCHECK(hipMemsetAsync(hx, 0, nb, st));
     for (int set = 0; set < nsets; set++) {
@@ -757,7 +799,25 @@ hipError_t open_phase(Job &j, const void *hpoly, bool shplonk, bool batched, boo
         if (sw) PCHECK(hipMemcpyAsync(hx, tmp, nb, hipMemcpyDeviceToDevice, st));
     }
     PCHECK(commit(j, pk.g, std::vector<const void *>{hx}, out + (size_t)ncom * 64, batched, st));
+    tr.points(out + (size_t)ncom * 64, 1);
     ncom += 1;
+    const Fr u = tr.challenge();
+    PCHECK(put_challenges(j, J_U, &u, 1, st));
+    Fr zt = fr_one(), z0inv = fr_one();
+    for (int q = 0; q < nsets; q++) zt = fr_mul(zt, fr_sub(u, P[q]));
+    for (int set = 0; set < nsets; set++) {
+        Fr ru = fr_zero();  // r_s(u)
+        for (int c = set; c >= 0; c--) ru = fr_add(fr_mul(ru, u), r[set][c]);
+        Fr zs = fr_one();   // prod over the points outside the set
+        for (int q = set + 1; q < nsets; q++) zs = fr_mul(zs, fr_sub(u, P[q]));
+        stage[9 + 2 * set] = ru;
+        stage[10 + 2 * set] = zs;
+        if (set == 0) z0inv = fr_inv(zs);
+    }
+    stage[15] = zt;
+    stage[16] = z0inv;
+    PCHECK(hipMemcpyAsync(ocs(32 + 9), stage.data() + 9, (stage.size() - 9) * FR, hipMemcpyHostToDevice, st));
+    PCHECK(hipStreamSynchronize(st));  // the staged scalars
     // L(X) = (sum_s v^. z_s (V_s - r_s(u)) - Z_T(u) h(X)) / z_0, then / (X - u)
     PCHECK(hipMemsetAsync(fz, 0, nb, st));
     for (int set = 0; set < nsets; set++) {
@@ -768,6 +828,7 @@ hipError_t open_phase(Job &j, const void *hpoly, bool shplonk, bool batched, boo
     PCHECK(field_op(fz, fz, 0, ocs(32 + 16), nullptr, 0, nullptr, n, FO_LEFT, st));
     PCHECK(zk_kate_divide(fz, fz, j.cf(J_U), ocs(0), n, kate_tmp, st));
     PCHECK(commit(j, pk.g, std::vector<const void *>{fz}, out + (size_t)ncom * 64, batched, st));
+    tr.points(out + (size_t)ncom * 64, 1);
     ncom += 1;
     Fr lu;
     PCHECK(hipMemcpyAsync(&lu, ocs(0), FR, hipMemcpyDeviceToHost, st));
@@ -898,6 +959,10 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         return hipSuccess;
     };
 
+    // Fiat-Shamir: commitments and evaluations are absorbed in the order the
+    // reference writes them; every challenge is squeezed from the transcript
+    Transcript tr;
+
     // ---- 0. witness (the caller's advice in the reference) -----------------
     PCHECK(witness(j, st));
     if (flags & ZKP_TAMPER_GATE) {
@@ -912,7 +977,9 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
     {
         std::vector<const void *> polys(j.adv_lag.begin(), j.adv_lag.end());
         PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
+        tr.points(out + (size_t)ncom * 64, s.A);
         ncom += s.A;
+        (void)tr.challenge();  // theta (src/lib.rs:395): this circuit's lookups are single columns
         PCHECK(intt_all(j, j.adv_lag, j.adv_coef, j.adv_tmp, st));
     }
     PCHECK(mark(1));
@@ -932,8 +999,15 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
     if (s.L) {
         std::vector<const void *> polys(j.m_lag.begin(), j.m_lag.end());
         PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
+        tr.points(out + (size_t)ncom * 64, s.L);
         ncom += s.L;
         PCHECK(intt_all(j, j.m_lag, j.m_coef, j.m_tmp, st));
+    }
+    {
+        const zkh::Fr bg[2] = {tr.challenge(), tr.challenge()};  // beta, gamma (src/lib.rs:531-532)
+        PCHECK(put_challenges(j, J_BETA, bg, 2, st));
+        memcpy(inf.challenges[0], &bg[0], FR);
+        memcpy(inf.challenges[1], &bg[1], FR);
     }
     PCHECK(mark(2));
 
@@ -975,12 +1049,18 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         PCHECK(hipStreamSynchronize(st));
     }
     {
+        // permutation, lookup, shuffle: the reference's transcript order
+        // (src/lib.rs:968-980)
         std::vector<const void *> polys;
-        for (auto p : j.zl_lag) polys.push_back(p);
         for (auto p : j.zp_lag) polys.push_back(p);
+        for (auto p : j.zl_lag) polys.push_back(p);
         for (auto p : j.zs_lag) polys.push_back(p);
         if (!polys.empty()) PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
+        tr.points(out + (size_t)ncom * 64, (int)polys.size());
         ncom += (int)polys.size();
+        const zkh::Fr yc = tr.challenge();  // y (src/lib.rs:988)
+        PCHECK(put_challenges(j, J_Y, &yc, 1, st));
+        memcpy(inf.challenges[2], &yc, FR);
         // closure of every grand product at row u: logup 0, permutation (last
         // set) 1, shuffle 1
         int bad = 0;  // bit 0: a logup sum, bit 1: the permutation, bit 2: a shuffle
@@ -1100,7 +1180,11 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         std::vector<const void *> pieces;
         for (int i = 0; i < kQuot; i++) pieces.push_back((uint8_t *)c + (size_t)i * s.n * FR);
         PCHECK(commit(j, pk.g, pieces, out + (size_t)ncom * 64, batched != 0, st));
+        tr.points(out + (size_t)ncom * 64, kQuot);
         ncom += kQuot;
+        const zkh::Fr xc = tr.challenge();  // x
+        PCHECK(put_challenges(j, J_X, &xc, 1, st));
+        memcpy(inf.challenges[3], &xc, FR);
         // x^n, then h_pieces = sum_i piece_i x^(n i) by Horner into the last piece
         PCHECK(square_times(j.cf(J_XN), j.cf(J_X), s.k, st));
         uint8_t *last = (uint8_t *)c + (size_t)(kQuot - 1) * s.n * FR;
@@ -1115,12 +1199,17 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
     // ---- 6. multi-open ------------------------------------------------------------
     if (!(flags & ZKP_NO_OPEN)) {
         int ok = 0, ne = 0;
-        PCHECK(open_phase(j, hpoly, (flags & ZKP_SHPLONK) != 0, batched != 0, (flags & ZKP_TAMPER_EVAL) != 0, out,
-                          ncom, ok, ne, st));
+        PCHECK(open_phase(j, tr, hpoly, (flags & ZKP_SHPLONK) != 0, batched != 0, (flags & ZKP_TAMPER_EVAL) != 0,
+                          out, ncom, ok, ne, st));
         inf.open_ok = ok;
         inf.nevals = ne;
     }
     PCHECK(mark(6));
+    {
+        uint8_t d[64];
+        zk_transcript_squeeze(tr.t, nullptr, d);
+        memcpy(inf.transcript, d, sizeof(inf.transcript));
+    }
     inf.total_ms = tp - t_start;
     inf.ncommit = ncom;
     if (info) *info = inf;

@@ -26,7 +26,8 @@ class Circuit(ctypes.Structure):
 class ProofInfo(ctypes.Structure):
     _fields_ = [("ncommit", ctypes.c_int), ("z_ok", ctypes.c_int), ("quotient_ok", ctypes.c_int),
                 ("z_fail", ctypes.c_int), ("open_ok", ctypes.c_int), ("nevals", ctypes.c_int),
-                ("phase_ms", ctypes.c_double * 7), ("total_ms", ctypes.c_double)]
+                ("phase_ms", ctypes.c_double * 7), ("total_ms", ctypes.c_double),
+                ("transcript", ctypes.c_ubyte * 32), ("challenges", (ctypes.c_ulonglong * 4) * 4)]
 
     PHASES = ("witness", "advice_commit_intt", "multiplicities", "grand_products", "h_eval", "vanishing",
               "multiopen")
