@@ -370,6 +370,8 @@ hipError_t zk_copy(void *dst, const void *src, size_t bytes, hipStream_t stream)
 /* seeded synthetic inputs (bench): uniform Fr (Montgomery) and points k_i * G */
 hipError_t zk_rand_fr(void *out, int n, unsigned long long seed, hipStream_t stream);
 hipError_t zk_gen_points(void *out, int n, unsigned long long seed, hipStream_t stream);
+/* out[i] = s[i] G (affine, G = (1, 2)) for n Montgomery scalars s */
+hipError_t zk_g1_mul_scalars(void *out, const void *scalars, int n, hipStream_t stream);
 
 /* ---- host-side mirror of the reference's MSM driver -------------------- */
 /* src/cuda/msm.rs:274-283 window policy */
@@ -444,6 +446,15 @@ int zkp_commitment_count(const zkp_circuit *c);
  * circuit shape, shared by every job of it. */
 hipError_t zkp_pk_create(const zkp_circuit *c, unsigned long long seed, hipStream_t stream, void **pk);
 hipError_t zkp_pk_destroy(void *pk);
+/* The key's SRS is a KZG setup for a secret tau drawn from the key seed:
+ * g[i] = tau^i G (coefficient basis), g_lag[i] = L_i(tau) G (Lagrange basis).
+ * zkp_pk_srs_secret copies tau (Montgomery Fr) -- a synthetic setup whose
+ * trapdoor tests use to check openings without pairings.
+ * zkp_pk_fixed_commitments writes the verifying key's commitments (affine,
+ * 64 B each) of q, t and the perm_cols sigma columns; returns their count
+ * (2 + perm_cols) or -1 if out is too small. */
+hipError_t zkp_pk_srs_secret(void *pk, void *tau_out);
+int zkp_pk_fixed_commitments(void *pk, void *out, int max);
 /* Per-proof device state (witness, grand products, extended polys, h). */
 hipError_t zkp_job_create(void *pk, void **job);
 hipError_t zkp_job_destroy(void *job);
@@ -462,6 +473,13 @@ hipError_t zkp_job_destroy(void *job);
 #define ZKP_SHPLONK 4
 #define ZKP_NO_OPEN 8
 #define ZKP_TAMPER_EVAL 16
+/* The last proof's opening: its evaluations (host, Montgomery Fr, in query
+ * order: the {x} set -- advice, q, t, sigmas, m, h -- then the {x, xw} set --
+ * logup z, shuffle z, last permutation z -- at x then xw, then the other
+ * permutation z at x, xw, xw^u) and the opening challenges [v, y', u] (y', u
+ * zero for GWC).  Returns the number of evaluations, or -1 if max is too
+ * small. */
+int zkp_proof_openings(void *job, void *evals, int max, void *challenges3);
 hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commitments, void *h_pieces,
                      zkp_proof_info *info, hipStream_t stream);
 

@@ -163,10 +163,8 @@ __global__ void k_rand_fr(Fr *out, uint32_t n, uint64_t seed) {
 }
 
 // out[i] = k_i * G (affine, Montgomery) with k_i = rand_canonical(seed, i)
-__global__ __launch_bounds__(64) void k_gen_points(G1Aff *out, uint32_t n, uint64_t seed) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Fr k = rand_canonical(seed, i);
+// k G (canonical k < 2^254, G = (1, 2)) as an affine point, by double-and-add
+__device__ G1Aff g_times(const Fr &k) {
     Fp gx = fe_from_u32<FpParams>(1), gy = fe_from_u32<FpParams>(2);
     G1Xyzz acc = xyzz_id();
     for (int b = 253; b >= 0; b--) {
@@ -182,6 +180,22 @@ __global__ __launch_bounds__(64) void k_gen_points(G1Aff *out, uint32_t n, uint6
         r.x = j.x;
         r.y = j.y;
     }
+    return r;
+}
+
+__global__ __launch_bounds__(64) void k_gen_points(G1Aff *out, uint32_t n, uint64_t seed) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const G1Aff r = g_times(rand_canonical(seed, i));
+    fe_store<FpParams>(&out[i].x, r.x);
+    fe_store<FpParams>(&out[i].y, r.y);
+}
+
+// out[i] = s[i] G for Montgomery scalars s
+__global__ __launch_bounds__(64) void k_g_mul_scalars(G1Aff *out, const Fr *__restrict__ s, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const G1Aff r = g_times(fe_from_mont<FrParams>(fe_load<FrParams>(&s[i])));
     fe_store<FpParams>(&out[i].x, r.x);
     fe_store<FpParams>(&out[i].y, r.y);
 }
@@ -199,6 +213,12 @@ hipError_t zk_rand_fr(void *out, int n, unsigned long long seed, hipStream_t str
 hipError_t zk_gen_points(void *out, int n, unsigned long long seed, hipStream_t stream) {
     if (n <= 0 || !out) return n == 0 ? hipSuccess : hipErrorInvalidValue;
     k_gen_points<<<(n + 63) / 64, 64, 0, stream>>>((G1Aff *)out, (uint32_t)n, seed);
+    return hipGetLastError();
+}
+
+hipError_t zk_g1_mul_scalars(void *out, const void *scalars, int n, hipStream_t stream) {
+    if (n <= 0 || !out || !scalars) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    k_g_mul_scalars<<<(n + 63) / 64, 64, 0, stream>>>((G1Aff *)out, (const Fr *)scalars, (uint32_t)n);
     return hipGetLastError();
 }
 

@@ -138,7 +138,8 @@ enum Const {
     C_FWD0, C_FWD1,  // [zeta, zeta^2]: into the coset
     C_INV0, C_INV1,  // [zeta^2, zeta]: out of the coset
     C_TEV,           // 4 values of 1 / Z_H on the extended coset
-    C_DELTAS = C_TEV + 4,
+    C_TAU = C_TEV + 4,  // the setup's secret (test access only)
+    C_DELTAS,
     C_COUNT_FIXED = C_DELTAS  // + P
 };
 
@@ -150,6 +151,7 @@ struct Pk {
     uint8_t *cst;  // Fr constants
     void *t_lag, *t_ext, *q_ext, *l0_ext, *llast_ext, *lact_ext, *ones_ext, *x_ext;
     void *t_coef, *q_coef;  // fixed columns opened in the multi-open step
+    std::vector<uint8_t> vk;  // commitments of q, t, sigmas (affine, host)
     std::vector<void *> sig_lag, sig_coef, sig_ext;
     uint32_t *perm_head, *perm_target;
     std::vector<uint32_t *> shuf_perm;
@@ -277,9 +279,22 @@ hipError_t pk_build(Pk &pk, uint64_t seed, hipStream_t st) {
             PCHECK(zk_fr_op(acc, acc, wn4, OP_MUL, 1, st));
         }
     }
-    // SRS stand-ins: seeded multiples of G
-    PCHECK(zk_gen_points(pk.g_lag, (int)s.n, seed ^ 0x1111ull, st));
-    PCHECK(zk_gen_points(pk.g, (int)s.n, seed ^ 0x2222ull, st));
+    // KZG setup for a seeded secret tau: g[i] = tau^i G, g_lag[i] = L_i(tau) G
+    // with L_i(tau) = w^i (tau^n - 1) / (n (tau - w^i))
+    {
+        void *tau = pk.cf(C_TAU), *sc = pk.scratch_n[0], *tmp = pk.scratch_n[1];
+        PCHECK(zk_rand_fr(tau, 1, seed ^ 0x7a7a7a7aull, st));
+        PCHECK(omega_table(sc, tau, s.k, pk.cf(C_ONE), st));  // tau^i
+        PCHECK(zk_g1_mul_scalars(pk.g, sc, (int)s.n, st));
+        void *c = (uint8_t *)pk.scratch_n[2];  // (tau^n - 1) / n
+        PCHECK(square_times(c, tau, s.k, st));
+        PCHECK(zk_fr_op(c, c, pk.cf(C_ONE), OP_SUB, 1, st));
+        PCHECK(zk_fr_op(c, c, pk.cf(C_NINV), OP_MUL, 1, st));
+        PCHECK(field_op(sc, nullptr, 0, tau, pk.w_n, 0, nullptr, (int)s.n, 3, st));  // tau - w^i
+        PCHECK(zk_batch_invert(sc, tmp, (int)s.n, st));
+        PCHECK(field_op(sc, sc, 0, c, pk.w_n, 0, nullptr, (int)s.n, 1, st));
+        PCHECK(zk_g1_mul_scalars(pk.g_lag, sc, (int)s.n, st));
+    }
     // fixed columns: table (Lagrange) and its coset; selector q; l0, l_last,
     // l_active; a column of ones
     void *coef = pk.scratch_n[1], *tmp = pk.scratch_n[2], *res = nullptr;
@@ -362,6 +377,14 @@ hipError_t pk_build(Pk &pk, uint64_t seed, hipStream_t st) {
         }
         PCHECK(hipMemcpy(pk.shuf_perm[j], p.data(), (size_t)s.R * 4, hipMemcpyHostToDevice));
     }
+    // the verifying key: commitments of the fixed columns the proof opens
+    {
+        std::vector<const void *> fixed = {pk.q_coef, pk.t_coef};
+        for (auto c : pk.sig_coef) fixed.push_back(c);
+        PCHECK(hipStreamSynchronize(st));
+        pk.vk.assign(fixed.size() * 64, 0);
+        PCHECK(zkp_batch_msm(pk.vk.data(), pk.g, fixed.data(), (int)fixed.size(), (int)s.n, 0));
+    }
     return hipStreamSynchronize(st);
 }
 
@@ -409,6 +432,8 @@ struct Job {
     uint32_t *flag;
     void *evals;     // opening evaluations, one Fr per (polynomial, point)
     void *open_cst;  // opening scalars staged from the host
+    std::vector<zkh::Fr> last_evals;  // the last proof's opening (host copies)
+    zkh::Fr last_open_ch[3] = {};     // v, y', u
     void *cf(int c) const { return cst + (size_t)c * FR; }
 };
 
@@ -699,6 +724,7 @@ hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, b
     // the evaluations enter the transcript; GWC draws v, Shplonk y' then v
     // (src/multiopen.rs:85, :302-305); Shplonk's u follows h's commitment
     tr.scalars(ev.data(), e);
+    j.last_evals = ev;
     Fr v, y = fr_zero();
     if (shplonk) {
         y = tr.challenge();
@@ -709,6 +735,9 @@ hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, b
         v = tr.challenge();
         PCHECK(put_challenges(j, J_V, &v, 1, st));
     }
+    j.last_open_ch[0] = v;
+    j.last_open_ch[1] = y;
+    j.last_open_ch[2] = fr_zero();
     // Horner in c over the evaluations of `set` at point q, in query order
     auto combine = [&](int set, int q, const Fr &c) {
         int first = 0;
@@ -803,6 +832,7 @@ hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, b
     ncom += 1;
     const Fr u = tr.challenge();
     PCHECK(put_challenges(j, J_U, &u, 1, st));
+    j.last_open_ch[2] = u;
     Fr zt = fr_one(), z0inv = fr_one();
     for (int q = 0; q < nsets; q++) zt = fr_mul(zt, fr_sub(u, P[q]));
     for (int set = 0; set < nsets; set++) {
@@ -887,6 +917,32 @@ hipError_t zkp_pk_create(const zkp_circuit *c, unsigned long long seed, hipStrea
     }
     *pk_out = pk;
     return hipSuccess;
+}
+
+hipError_t zkp_pk_srs_secret(void *pk, void *tau_out) {
+    if (!pk || !tau_out) return hipErrorInvalidValue;
+    return hipMemcpy(tau_out, ((Pk *)pk)->cf(C_TAU), FR, hipMemcpyDeviceToHost);
+}
+
+int zkp_pk_fixed_commitments(void *pk, void *out, int max) {
+    if (!pk) return -1;
+    const Pk &p = *(const Pk *)pk;
+    const int count = (int)(p.vk.size() / 64);
+    if (!out || max < count) return -1;
+    memcpy(out, p.vk.data(), p.vk.size());
+    return count;
+}
+
+int zkp_proof_openings(void *job, void *evals, int max, void *challenges3) {
+    if (!job) return -1;
+    const Job &j = *(const Job *)job;
+    const int count = (int)j.last_evals.size();
+    if (evals) {
+        if (max < count) return -1;
+        memcpy(evals, j.last_evals.data(), (size_t)count * FR);
+    }
+    if (challenges3) memcpy(challenges3, j.last_open_ch, 3 * FR);
+    return count;
 }
 
 hipError_t zkp_pk_destroy(void *pk) {

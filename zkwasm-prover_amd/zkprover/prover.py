@@ -60,6 +60,11 @@ def _decl():
     L.zkp_job_destroy.argtypes = [ctypes.c_void_p]
     L.zkp_prove.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                             ctypes.c_void_p, ctypes.c_void_p]
+    L.zkp_pk_srs_secret.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.zkp_pk_fixed_commitments.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    L.zkp_pk_fixed_commitments.restype = ctypes.c_int
+    L.zkp_proof_openings.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    L.zkp_proof_openings.restype = ctypes.c_int
     return L
 
 
@@ -76,6 +81,20 @@ class ProvingKey:
         self.handle = ctypes.c_void_p()
         check(self.L.zkp_pk_create(ctypes.byref(circuit), seed, stream, ctypes.byref(self.handle)), "zkp_pk_create")
         self.ncommit = self.L.zkp_commitment_count(ctypes.byref(circuit))
+
+    def srs_secret(self):
+        """the synthetic setup's tau (Montgomery Fr, (1, 4) u64): test use only"""
+        out = np.zeros((1, 4), dtype=np.uint64)
+        check(self.L.zkp_pk_srs_secret(self.handle, out.ctypes.data), "zkp_pk_srs_secret")
+        return out
+
+    def fixed_commitments(self):
+        """verifying-key commitments of q, t and the sigma columns, (2 + P, 8) affine"""
+        out = np.zeros((2 + self.circuit.perm_cols, 8), dtype=np.uint64)
+        cnt = self.L.zkp_pk_fixed_commitments(self.handle, out.ctypes.data, len(out))
+        if cnt < 0:
+            raise HipError("zkp_pk_fixed_commitments")
+        return out[:cnt]
 
     def close(self):
         if self.handle:
@@ -100,6 +119,16 @@ class Prover:
                                pieces.ctypes.data if want_pieces else None, ctypes.byref(info), stream),
               "zkp_prove")
         return com, pieces, info
+
+    def openings(self):
+        """the last proof's evaluations ((count, 4) Montgomery, query order)
+        and its opening challenges [v, y', u] ((3, 4))"""
+        cnt = self.L.zkp_proof_openings(self.handle, None, 0, None)
+        ev = np.zeros((max(cnt, 1), 4), dtype=np.uint64)
+        ch = np.zeros((3, 4), dtype=np.uint64)
+        if self.L.zkp_proof_openings(self.handle, ev.ctypes.data, len(ev), ch.ctypes.data) != cnt:
+            raise HipError("zkp_proof_openings")
+        return ev[:cnt], ch
 
     def close(self):
         if self.handle:
