@@ -15,7 +15,9 @@ here with the oracle's curve arithmetic:
     product of (u - p) over the points outside the set,
     (tau - u) [L] = (sum_s v^(k-1-s) z_s (V_s - r_s(u) G) - Z_T(u) [h]) / z_0.
 A wrong evaluation, quotient or challenge breaks the equation for all but a
-negligible set of tau.
+negligible set of tau.  The vanishing argument is checked the verifier's way
+too: h(x) (x^n - 1) equals the composed constraint rebuilt from the opened
+evaluations and the challenges.
 """
 import numpy as np
 import pytest
@@ -233,3 +235,84 @@ def test_tampered_evaluation_fails_verification(setup, zo):
         lhs = g1.mul_aff(com[wbase + q], tau - pts[q])
         ok.append(bool((g1.aff(lhs) == g1.aff(rhs)).all()))
     assert not all(ok)
+
+
+# delta = 7^(2^28) mod r, the generator of the permutation column cosets
+DELTA = 0x09226b6e22c6f0ca64ec26aad4c86e715b5f898e5e963f25870e56bbe533e9a2
+
+
+def _constraints_at_x(circ, evs, x, beta, gamma, y, zo):
+    """the composed constraint C(x) as the prover folds it on the coset (gate;
+    permutation p1, p2, one term per set; logup per lookup; shuffle per
+    pair; each new term after multiplying the sum so far by y), restated
+    with the opened evaluations and the verifier's own l0, l_last, l_active
+    at x"""
+    A, L, P, S = circ.advice, circ.lookups, circ.perm_cols, circ.shuffles
+    Sp = (P + 1) // 2
+    n = 1 << circ.k
+    u = n - (circ.blinding + 1)
+    w = _ints(zo, zo.root_of_unity(circ.k))[0]
+    zh = (pow(x, n, R) - 1) % R
+    ninv = pow(n, -1, R)
+
+    def lag(i):
+        wi = pow(w, i, R)
+        return wi * zh % R * ninv % R * pow(x - wi, -1, R) % R
+
+    l0, l_last = lag(0), lag(u)
+    l_active = (1 - sum(lag(i) for i in range(u, n))) % R
+    s0 = evs[0]
+    adv = [s0[i][0] for i in range(A)]
+    q, t = s0[A][0], s0[A + 1][0]
+    sig = [s0[A + 2 + c][0] for c in range(P)]
+    m = [s0[A + 2 + P + l][0] for l in range(L)]
+    zl = [evs[1][l] for l in range(L)]
+    zs = [evs[1][L + i] for i in range(S)]
+    zp = [evs[2][j] for j in range(Sp - 1)] + [evs[1][L + S]]
+    a, b, c, d = adv[0:4]
+    h = (q * a * b + q * c - q * d) % R
+    if Sp:
+        h = (h * y + l0 * (1 - zp[0][0])) % R
+        h = (h * y + l_last * (zp[-1][0] * zp[-1][0] - zp[-1][0])) % R
+        for j in range(1, Sp):
+            h = (h * y + l0 * (zp[j][0] - zp[j - 1][2])) % R
+        for st in range(Sp):
+            lhs, rhs = zp[st][1], zp[st][0]
+            for col in range(2 * st, min(2 * st + 2, P)):
+                v = adv[4 + L + col]
+                lhs = lhs * (v + gamma + beta * sig[col]) % R
+                rhs = rhs * (v + gamma + beta * pow(DELTA, col, R) * x) % R
+            h = (h * y + l_active * (lhs - rhs)) % R
+    for l in range(L):
+        f, tb = (adv[4 + l] + beta) % R, (t + beta) % R
+        z0, z1 = zl[l][0], zl[l][1]
+        h = (h * y + l0 * z0) % R
+        h = (h * y + l_last * z0) % R
+        h = (h * y + l_active * (((z1 - z0) * tb + m[l]) * f - tb)) % R
+    for i in range(S):
+        inp, sh = (adv[4 + L + P + 2 * i] + beta) % R, (adv[4 + L + P + 2 * i + 1] + beta) % R
+        z0, z1 = zs[i][0], zs[i][1]
+        h = (h * y + l0 * (1 - z0)) % R
+        h = (h * y + l_last * (z0 * z0 - z0)) % R
+        h = (h * y + l_active * (sh * z1 - inp * z0)) % R
+    return h, zh
+
+
+@pytest.mark.parametrize("tamper", [False, True])
+def test_quotient_identity_at_x(setup, zo, tamper):
+    """the vanishing argument: h(x) (x^n - 1) = C(x) from the opened values
+    (h(x) is the folded quotient's evaluation); a witness with one broken
+    gate row makes h no polynomial, and the identity fails"""
+    from zkprover.prover import BATCHED, TAMPER_GATE
+    pk, pr = setup
+    com, _, info = pr.prove(45, flags=BATCHED | (TAMPER_GATE if tamper else 0))
+    ev_m, _ = pr.openings()
+    ev = _ints(zo, ev_m)
+    ch = [_ints(zo, np.array(info.challenges[c], dtype=np.uint64))[0] for c in range(4)]
+    beta, gamma, y, x = ch
+    g1 = G1(zo)
+    sets, _ = _layout(pk.circuit, com, pk.fixed_commitments(), x, g1)
+    evs = _evals_by_set(sets, ev)
+    cx, zh = _constraints_at_x(pk.circuit, evs, x, beta, gamma, y, zo)
+    hx = evs[0][len(sets[0]) - 1][0]
+    assert (hx * zh % R == cx) == (not tamper)
