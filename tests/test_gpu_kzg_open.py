@@ -15,7 +15,11 @@ here with the oracle's curve arithmetic:
     product of (u - p) over the points outside the set,
     (tau - u) [L] = (sum_s v^(k-1-s) z_s (V_s - r_s(u) G) - Z_T(u) [h]) / z_0.
 A wrong evaluation, quotient or challenge breaks the equation for all but a
-negligible set of tau.  The vanishing argument is checked the verifier's way
+negligible set of tau.  The same openings also pass the verifier's own
+pairing checks, which hold only [tau]_2 = tau G2 of the setup (the pairing of
+oracle/bn254_pairing.py): GWC as one product check over all points with a
+random u, Shplonk as e([L], [tau]_2) = e(u [L] + F / z_0, G2); the tampered
+proof fails it.  The vanishing argument is checked the verifier's way
 too: h(x) (x^n - 1) equals the composed constraint rebuilt from the opened
 evaluations and the challenges.
 """
@@ -128,21 +132,15 @@ def test_fixed_commitments_agree_across_bases(setup, zo):
     assert (g1.aff(g1.mul_aff(g1.g, q_tau)) == vk[0:1]).all()
 
 
-@pytest.mark.parametrize("seed", [41, 42])
-def test_gwc_openings_verify(setup, zo, seed):
-    from zkprover.prover import BATCHED
-    pk, pr = setup
-    com, _, info = pr.prove(seed, flags=BATCHED)
-    assert info.open_ok == 1
-    ev_m, ch = pr.openings()
-    ev, v = _ints(zo, ev_m), _ints(zo, ch)[0]
-    tau = _ints(zo, pk.srs_secret())[0]
+def _gwc_groups(pk, com, info, ev, v, zo, g1):
+    """per opening point q: (W_q affine row, z_q, C_q - E_q G Jacobian) with
+    the v-batched commitment C_q and evaluation E_q of the queries at z_q"""
     x = _ints(zo, np.array(info.challenges[3], dtype=np.uint64))[0]
-    g1 = G1(zo)
     sets, wbase = _layout(pk.circuit, com, pk.fixed_commitments(), x, g1)
     pts = _points(pk.circuit, x, zo)
     evs = _evals_by_set(sets, ev)
     ngroups = 3 if sets[2] else (2 if sets[1] else 1)
+    out = []
     for q in range(ngroups):
         pairs, evals = [], []
         for s in range(q, 3):
@@ -152,9 +150,40 @@ def test_gwc_openings_verify(setup, zo, seed):
         k = len(pairs)
         batch = g1.lin([(pairs[i], pow(v, k - 1 - i, R)) for i in range(k)])
         e = sum(evals[i] * pow(v, k - 1 - i, R) for i in range(k)) % R
-        rhs = g1.add(batch, g1.mul_aff(g1.g, -e))
-        lhs = g1.mul_aff(com[wbase + q], tau - pts[q])
+        out.append((com[wbase + q], pts[q], g1.add(batch, g1.mul_aff(g1.g, -e))))
+    return out
+
+
+def _gwc_pairing_accepts(groups, tau, g1, u=0x5eed):
+    """the verifier's one pairing check over all points, holding only
+    [tau]_2 of the setup: with powers of a random u,
+    e(sum u^q W_q, [tau]_2) = e(sum u^q (z_q W_q + C_q - E_q G), G2)"""
+    import bn254_pairing as bp
+    left = g1.lin([(w, pow(u, q, R)) for q, (w, _, _) in enumerate(groups)])
+    right = None
+    for q, (w, z, rhs) in enumerate(groups):
+        t = g1.mul_aff(g1.aff(g1.add(g1.mul_aff(w, z), rhs))[0], pow(u, q, R))
+        right = t if right is None else g1.add(right, t)
+    pt = lambda jac: None if not g1.aff(jac).any() else g1.zo.g1a_to_ints(g1.aff(jac))
+    return bp.pairing_check([(bp.g2_mul(bp.G2, tau), pt(left)), (bp.g2_neg(bp.G2), pt(right))])
+
+
+@pytest.mark.parametrize("seed", [41, 42])
+def test_gwc_openings_verify(setup, zo, seed):
+    from zkprover.prover import BATCHED
+    pk, pr = setup
+    com, _, info = pr.prove(seed, flags=BATCHED)
+    assert info.open_ok == 1
+    ev_m, ch = pr.openings()
+    ev, v = _ints(zo, ev_m), _ints(zo, ch)[0]
+    tau = _ints(zo, pk.srs_secret())[0]
+    g1 = G1(zo)
+    groups = _gwc_groups(pk, com, info, ev, v, zo, g1)
+    for q, (w, z, rhs) in enumerate(groups):
+        lhs = g1.mul_aff(w, tau - z)
         assert (g1.aff(lhs) == g1.aff(rhs)).all(), "GWC opening at point %d" % q
+    if seed == 41:
+        assert _gwc_pairing_accepts(groups, tau, g1)
 
 
 def _lagrange_at(xs, ys, t):
@@ -206,6 +235,11 @@ def test_shplonk_opening_verifies(setup, zo):
     rhs = g1.mul_aff(g1.aff(F)[0], pow(z0, -1, R))
     lhs = g1.mul_aff(lx, tau - u)
     assert (g1.aff(lhs) == g1.aff(rhs)).all()
+    # the verifier's pairing form: e([L], [tau]_2) = e(u [L] + F / z_0, G2)
+    import bn254_pairing as bp
+    right = g1.aff(g1.add(g1.mul_aff(lx, u), rhs))
+    assert bp.pairing_check([(bp.g2_mul(bp.G2, tau), zo.g1a_to_ints(lx)),
+                             (bp.g2_neg(bp.G2), zo.g1a_to_ints(right))])
 
 
 def test_tampered_evaluation_fails_verification(setup, zo):
@@ -216,25 +250,11 @@ def test_tampered_evaluation_fails_verification(setup, zo):
     ev_m, ch = pr.openings()
     ev, v = _ints(zo, ev_m), _ints(zo, ch)[0]
     tau = _ints(zo, pk.srs_secret())[0]
-    x = _ints(zo, np.array(info.challenges[3], dtype=np.uint64))[0]
     g1 = G1(zo)
-    sets, wbase = _layout(pk.circuit, com, pk.fixed_commitments(), x, g1)
-    pts = _points(pk.circuit, x, zo)
-    evs = _evals_by_set(sets, ev)
-    ok = []
-    for q in range(3):
-        pairs, evals = [], []
-        for s in range(q, 3):
-            for i, c in enumerate(sets[s]):
-                pairs.append(c)
-                evals.append(evs[s][i][q])
-        k = len(pairs)
-        batch = g1.lin([(pairs[i], pow(v, k - 1 - i, R)) for i in range(k)])
-        e = sum(evals[i] * pow(v, k - 1 - i, R) for i in range(k)) % R
-        rhs = g1.add(batch, g1.mul_aff(g1.g, -e))
-        lhs = g1.mul_aff(com[wbase + q], tau - pts[q])
-        ok.append(bool((g1.aff(lhs) == g1.aff(rhs)).all()))
+    groups = _gwc_groups(pk, com, info, ev, v, zo, g1)
+    ok = [bool((g1.aff(g1.mul_aff(w, tau - z)) == g1.aff(rhs)).all()) for w, z, rhs in groups]
     assert not all(ok)
+    assert not _gwc_pairing_accepts(groups, tau, g1)
 
 
 # delta = 7^(2^28) mod r, the generator of the permutation column cosets
