@@ -272,9 +272,55 @@ def main():
         out["proof_core"] = bench_proof_core([int(k) for k in args.proof_ks.split(",") if k])
     if args.serve:
         out["serving"] = bench_serving(args.serve_count)
+    out["host_copy"] = bench_host_copy(prim_ms["msm_2^22"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)
+
+
+def bench_host_copy(msm_ms, reps=5):
+    """Host <-> device rates of one k=22 column (2^27 bytes): pinned pool
+    memory against pageable numpy memory, hipMemcpyAsync timed with HIP
+    events on the copying stream; and the 2^22 MSM with its scalars uploaded
+    from pinned host memory first (the rate a caller handing host scalars
+    sees; the bases stay resident as the SRS does)."""
+    import numpy as np
+    import torch
+    from zkprover import device as dv
+    from zkprover import hostmem
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                   ctypes.c_void_p]
+    nbytes = hostmem.CHUNK_BYTES
+    pin = hostmem.PinnedBuffer(nbytes)
+    page = np.ones(nbytes, dtype=np.uint8)
+    pin.array(np.uint8)[:] = 1
+    dev = torch.empty(nbytes, dtype=torch.uint8, device=dv.device())
+    st = dv.stream_handle()
+    res = {"bytes": nbytes}
+    try:
+        for name, dst, src, kind in (("h2d_pinned", dev.data_ptr(), pin.ptr, 1),
+                                     ("d2h_pinned", pin.ptr, dev.data_ptr(), 2),
+                                     ("h2d_pageable", dev.data_ptr(), page.ctypes.data, 1),
+                                     ("d2h_pageable", page.ctypes.data, dev.data_ptr(), 2)):
+            if hip.hipMemcpyAsync(dst, src, nbytes, kind, st) != 0:
+                raise RuntimeError("hipMemcpyAsync " + name)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record()
+            for _ in range(reps):
+                if hip.hipMemcpyAsync(dst, src, nbytes, kind, st) != 0:
+                    raise RuntimeError("hipMemcpyAsync " + name)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = max(e0.elapsed_time(e1), (time.perf_counter() - t0) * 1e3) / reps
+            res[name + "_GBps"] = round(nbytes / (ms * 1e-3) / 1e9, 2)
+    finally:
+        pin.close()
+    upload_ms = nbytes / (res["h2d_pinned_GBps"] * 1e9) * 1e3
+    res["msm_2^22_with_pinned_scalar_upload_ms"] = round(msm_ms + upload_ms, 3)
+    return res
 
 
 class _MsmItem(ctypes.Structure):

@@ -500,6 +500,40 @@ hipError_t zks_wait(void *srv, int id, void *commitments, zks_result *res);
 hipError_t zks_stats(void *srv, int *peak_running, size_t *peak_bytes, size_t *budget);
 hipError_t zks_destroy(void *srv);
 
+/* ---- host residency: pinned pool and device-to-host copy queue --------- */
+/* Process-wide pinned host pool (replaces src/hugetlb.rs:17-229, the pinned
+ * buffer cache behind HugePageAllocator): slabs of 2^28 bytes from
+ * hipHostMalloc, each split into two 2^27-byte chunks (one Fr column of 2^22
+ * rows).  A request of at most a chunk takes a chunk (partly used slabs
+ * first), at most a slab a whole slab; larger requests, and frees of
+ * pointers the pool did not hand out, return hipErrorInvalidValue.  Slabs
+ * stay with the pool until the process ends; reserve pre-grows it
+ * (reserve_pinned_buffer, hugetlb.rs:149). */
+hipError_t zk_pinned_reserve(size_t slabs);
+hipError_t zk_pinned_alloc(size_t bytes, void **out);
+hipError_t zk_pinned_free(void *p);
+hipError_t zk_pinned_stats(size_t *slabs, size_t *chunks_in_use, size_t *slab_bytes, size_t *chunk_bytes);
+
+/* Asynchronous device-to-host copy queue (replaces src/async_copy_queue.rs:8-90,
+ * AsyncCopyQueue::new(28) at src/lib.rs:586).  A push is ordered after the
+ * work enqueued so far on `producer` (an event, no host wait) and runs on the
+ * queue's own stream; with max_inflight copies outstanding a push first waits
+ * for the oldest.  flags: 0 = the caller leaves dev_src untouched until
+ * sync_all; ZK_COPYQ_SNAPSHOT = copied first into the queue's staging buffer,
+ * and `producer` waits for that device copy only (push_with_copy_buffer,
+ * :42-55); ZK_COPYQ_RELEASE = the queue owns dev_src (hipMalloc'd) and frees
+ * it after the copy (push / push_with_new_buffer, :57-82).  The two flags are
+ * exclusive.  host_dst from zk_pinned_alloc makes the copy asynchronous. */
+#define ZK_COPYQ_SNAPSHOT 1
+#define ZK_COPYQ_RELEASE 2
+hipError_t zk_copyq_create(int max_inflight, void **q);
+hipError_t zk_copyq_push(void *q, void *host_dst, const void *dev_src, size_t bytes, hipStream_t producer,
+                         int flags);
+hipError_t zk_copyq_sync_all(void *q);
+hipError_t zk_copyq_stats(void *q, unsigned long long *pushed, unsigned long long *full_waits,
+                          double *max_wait_ms);
+hipError_t zk_copyq_destroy(void *q);
+
 #ifdef __cplusplus
 }
 #endif

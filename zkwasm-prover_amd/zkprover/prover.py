@@ -109,11 +109,17 @@ class Prover:
         self.handle = ctypes.c_void_p()
         check(self.L.zkp_job_create(pk.handle, ctypes.byref(self.handle)), "zkp_job_create")
 
-    def prove(self, seed, flags=BATCHED, stream=0, want_pieces=True):
-        """returns (commitments (ncommit, 8) u64 affine, h_pieces (n, 4) or None, ProofInfo)"""
+    def prove(self, seed, flags=BATCHED, stream=0, want_pieces=True, pieces_out=None):
+        """returns (commitments (ncommit, 8) u64 affine, h_pieces (n, 4) or None, ProofInfo);
+        pieces_out: a caller's (n, 4) u64 array to receive h_pieces (e.g. pinned memory)"""
         n = 1 << self.pk.circuit.k
         com = np.zeros((self.pk.ncommit, 8), dtype=np.uint64)
-        pieces = np.zeros((n, 4), dtype=np.uint64) if want_pieces else None
+        if pieces_out is not None:
+            if pieces_out.shape != (n, 4) or pieces_out.dtype != np.uint64 or not pieces_out.flags.c_contiguous:
+                raise ValueError("pieces_out must be a contiguous (%d, 4) uint64 array" % n)
+            pieces, want_pieces = pieces_out, True
+        else:
+            pieces = np.zeros((n, 4), dtype=np.uint64) if want_pieces else None
         info = ProofInfo()
         check(self.L.zkp_prove(self.handle, seed, flags, com.ctypes.data,
                                pieces.ctypes.data if want_pieces else None, ctypes.byref(info), stream),
