@@ -133,10 +133,12 @@ def test_intt_roundtrip_2_24(gpu, zo):
     assert (back == x).all()
 
 
-@pytest.mark.parametrize("log_n,ext", [(10, 2), (16, 2), (20, 2)])
+@pytest.mark.parametrize("log_n,ext", [(10, 2), (16, 2), (20, 2), (22, 2)])
 def test_coset_extend_matches_oracle(gpu, zo, log_n, ext):
     """extended_prepare (zero-pad + zeta^(i mod 3) scaling) then NTT at 2^(k+ext)
-    (src/eval_h.rs:250-337, cuda/bn254.cu:641-662, :1232-1253)"""
+    (src/eval_h.rs:250-337, cuda/bn254.cu:641-662, :1232-1253).  Since
+    zeta^3 = 1, zeta^(i mod 3) = zeta^i and output j is p(zeta w_N^j): a few
+    outputs are also checked as evaluations of the coefficients there."""
     import ctypes
     from zkprover import device as dv
     from zkprover._lib import call
@@ -160,6 +162,10 @@ def test_coset_extend_matches_oracle(gpu, zo, log_n, ext):
          ctypes.addressof(swap), dv.stream_handle())
     got = dv.from_dev(tmp if swap.value else s)
     assert (got == zo.ntt(exp_prep, log_n + ext, w)).all()
+    wi = zo.from_mont(w)[0]
+    for j in (0, 1, N // 3, N - 1):
+        x = zo.to_mont([pow(7, (R - 1) // 3, R) * pow(wi, j, R) % R])
+        assert (got[j] == zo.poly_eval(coeffs, zo.x_table(x, log_n))[0]).all(), j
     # to_coset = 1 scales the whole extended buffer in place
     call("extended_prepare", dv.ptr(tmp if swap.value else s), dv.ptr(dcp), 3, n, N, 1,
          dv.stream_handle())

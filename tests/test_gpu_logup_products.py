@@ -27,7 +27,7 @@ def _arena():
     return Arena()
 
 
-@pytest.mark.parametrize("n", [1, 7, 512, 513, 4097, 1 << 16, (1 << 20) + 3])
+@pytest.mark.parametrize("n", [1, 7, 512, 513, 4097, 1 << 16, (1 << 20) + 3, 1 << 22])
 def test_batch_invert(gpu, zo, n):
     dv, call = _dv()
     A = _arena()
@@ -38,7 +38,15 @@ def test_batch_invert(gpu, zo, n):
     d = dv.to_dev(x)
     tmp = dv.empty(max(n, 1))
     call("zk_batch_invert", dv.ptr(d), dv.ptr(tmp), n, dv.stream_handle())
-    assert (dv.from_dev(d) == zo.batch_invert(x)).all()
+    got = dv.from_dev(d)
+    assert (got == zo.batch_invert(x)).all()
+    if n >= 1 << 16:
+        # x * x^-1 = 1 on every nonzero entry, zeros stay zero (closed form)
+        nz = np.ones(n, dtype=bool)
+        nz[[3, n - 1]] = False
+        one = zo.to_mont([1])[0]
+        prod = zo.field_op(x, 0, None, got, 0, None, n, 1)
+        assert (prod[nz] == one).all() and (got[~nz] == 0).all()
 
 
 @pytest.mark.parametrize("log_n", [10, 16, 20])
