@@ -136,6 +136,25 @@ hipError_t field_op_batch_mul_sum(void *res, const void *v, const void *rot, int
 /* bn254_c.rs:198 `field_mul_zip`: buf[i] *= coeff[i % coeff_n]. */
 hipError_t field_mul_zip(void *buf, const void *coeff, int coeff_n, int n);
 
+/* bn254_c.rs:71 `field_sum` (declared, never called by the reference):
+ * res[i] = sum_j v[j][i] * omegas[(rot_j * i) mod n] (* *v_c[j] when v_c[j]
+ * is non-null), i.e. coefficient vectors rotated by w^rot_j and summed.  v,
+ * v_c: device arrays of v_n device pointers; v_rot: device array of v_n i32;
+ * omegas: w^i for i < n; v_n <= 256.  Null stream. */
+hipError_t field_sum(void *res, const void *v, const void *v_c, const void *v_rot, const void *omegas, int v_n,
+                     int n);
+
+/* bn254_c.rs:90 `lookup_eval_h` (declared, never called by the reference):
+ * the permuted-column lookup's folds res = ((((res*y + l0 (1 - z))*y
+ * + l_last (z^2 - z))*y + l_active (z(wX)(a' + beta)(s' + gamma)
+ * - z input table))*y + l0 (a' - s'))*y + l_active (a' - s')(a' - a'(w^-1 X)),
+ * with input / table the compressed expressions including beta / gamma and
+ * w X at index i + rot. */
+hipError_t lookup_eval_h(void *res, const void *input, const void *table, const void *permuted_input,
+                         const void *permuted_table, const void *z, const void *l0, const void *l_last,
+                         const void *l_active_row, const void *y, const void *beta, const void *gamma, int rot,
+                         int n, hipStream_t stream);
+
 /* bn254_c.rs:41 `permutation_eval_h_p1`: res = (res*y + l0*(1 - first))*y
  * + l_last*(last^2 - last). */
 hipError_t permutation_eval_h_p1(void *res, const void *first_set, const void *last_set, const void *l0,

@@ -768,6 +768,57 @@ void zo_logup_eval_h_extra_inputs(zo_fr *res, const zo_fr *input_product,
         res[i] = t;
     }
 }
+/* cuda/bn254.cu:496-540: res[i] = sum_j v_j[i] w^(rot_j i mod n) (* c_j) */
+void zo_field_sum(zo_fr *res, const zo_fr *const *v, const zo_fr *const *v_c, const int *v_rot,
+                  const zo_fr *omegas, int v_n, size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        zo_fr acc = {{0}};
+        for (int j = 0; j < v_n; j++) {
+            long long r = ((long long)v_rot[j] % (long long)n + (long long)n) % (long long)n;
+            size_t e = (size_t)(((unsigned long long)r * i) % n);
+            zo_fr t;
+            fr_mul(&t, &v[j][i], &omegas[e]);
+            if (v_c[j]) fr_mul(&t, &t, v_c[j]);
+            if (j == 0) acc = t; else fr_add(&acc, &acc, &t);
+        }
+        res[i] = acc;
+    }
+}
+/* cuda/bn254.cu:798-866 */
+void zo_lookup_eval_h(zo_fr *res, const zo_fr *input, const zo_fr *table, const zo_fr *permuted_input,
+                      const zo_fr *permuted_table, const zo_fr *z, const zo_fr *l0, const zo_fr *l_last,
+                      const zo_fr *l_active_row, const zo_fr *y, const zo_fr *beta, const zo_fr *gamma,
+                      int rot, size_t n) {
+    zo_fr one; fr_one(&one);
+    for (size_t i = 0; i < n; i++) {
+        size_t rn = (i + (size_t)(long)rot) & (n - 1);
+        size_t rp = (i + n - (size_t)(long)rot) & (n - 1);
+        zo_fr t = res[i], u, p, x;
+        fr_mul(&t, &t, y); fr_sub(&u, &one, &z[i]); fr_mul(&u, &l0[i], &u); fr_add(&t, &t, &u);
+        fr_mul(&t, &t, y); fr_mul(&u, &z[i], &z[i]); fr_sub(&u, &u, &z[i]); fr_mul(&u, &l_last[i], &u);
+        fr_add(&t, &t, &u);
+        fr_mul(&t, &t, y);
+        fr_add(&u, &permuted_input[i], beta);
+        fr_add(&p, &permuted_table[i], gamma);
+        fr_mul(&u, &u, &p);
+        fr_mul(&u, &u, &z[rn]);
+        fr_mul(&x, &input[i], &table[i]);
+        fr_mul(&x, &z[i], &x);
+        fr_sub(&u, &u, &x);
+        fr_mul(&u, &u, &l_active_row[i]);
+        fr_add(&t, &t, &u);
+        fr_mul(&t, &t, y);
+        fr_sub(&p, &permuted_input[i], &permuted_table[i]);
+        fr_mul(&u, &l0[i], &p);
+        fr_add(&t, &t, &u);
+        fr_mul(&t, &t, y);
+        fr_sub(&u, &permuted_input[i], &permuted_input[rp]);
+        fr_mul(&u, &u, &p);
+        fr_mul(&u, &u, &l_active_row[i]);
+        fr_add(&t, &t, &u);
+        res[i] = t;
+    }
+}
 /* cuda/bn254.cu:1172-1213 */
 void zo_logup_eval_h_inputs_product_sum(zo_fr *product, zo_fr *product_sum,
                                         const zo_fr *const *set, int n_set, size_t n) {

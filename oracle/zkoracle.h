@@ -149,6 +149,12 @@ void zo_logup_eval_h(zo_fr *res, const zo_fr *input_product, const zo_fr *input_
 void zo_logup_eval_h_extra_inputs(zo_fr *res, const zo_fr *input_product,
                                   const zo_fr *input_product_sum, const zo_fr *z,
                                   const zo_fr *l_active_row, const zo_fr *y, int rot, size_t n);
+void zo_field_sum(zo_fr *res, const zo_fr *const *v, const zo_fr *const *v_c, const int *v_rot,
+                  const zo_fr *omegas, int v_n, size_t n);
+void zo_lookup_eval_h(zo_fr *res, const zo_fr *input, const zo_fr *table, const zo_fr *permuted_input,
+                      const zo_fr *permuted_table, const zo_fr *z, const zo_fr *l0, const zo_fr *l_last,
+                      const zo_fr *l_active_row, const zo_fr *y, const zo_fr *beta, const zo_fr *gamma,
+                      int rot, size_t n);
 void zo_logup_eval_h_inputs_product_sum(zo_fr *product, zo_fr *product_sum,
                                         const zo_fr *const *set, int n_set, size_t n);
 void zo_shuffle_eval_h(zo_fr *res, const zo_fr *input, const zo_fr *table, const zo_fr *z,

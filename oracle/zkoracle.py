@@ -480,6 +480,26 @@ def logup_eval_h_extra_inputs(res, input_product, input_product_sum, z, l_active
     return res
 
 
+def field_sum(vs, cs, rots, omegas):
+    """cuda/bn254.cu:496-540; cs entries (1, 4) or None"""
+    n = len(vs[0])
+    res = fr_zeros(n)
+    keep = [None if c is None else np.ascontiguousarray(c, dtype=np.uint64) for c in cs]
+    cptr = (ctypes.c_void_p * len(cs))(*[None if c is None else c.ctypes.data for c in keep])
+    rot = (ctypes.c_int * len(rots))(*rots)
+    lib().zo_field_sum(_p(res), _ptrs(vs), cptr, rot, _p(np.ascontiguousarray(omegas)), ctypes.c_int(len(vs)),
+                       ctypes.c_size_t(n))
+    return res
+
+
+def lookup_eval_h(res, inp, table, pin, ptab, z, l0, l_last, l_active_row, y, beta, gamma, rot):
+    res = _copy(res)
+    lib().zo_lookup_eval_h(_p(res), _p(inp), _p(table), _p(pin), _p(ptab), _p(z), _p(l0), _p(l_last),
+                           _p(l_active_row), _p(y), _p(beta), _p(gamma), ctypes.c_int(rot),
+                           ctypes.c_size_t(len(res)))
+    return res
+
+
 def logup_eval_h_inputs_product_sum(sets):
     n = len(sets[0])
     prod, psum = fr_zeros(n), fr_zeros(n)

@@ -183,3 +183,42 @@ def test_permutation_eval_h_set(gpu, zo, nc, rot):
     call("zk_permutation_eval_h_set", A.hold(th), A.put(z), A.put_all(vs), A.put_all(ss), nc, A.put(beta),
          A.put(gamma), A.put(bd), A.put(X), A.put(la), A.put(y), rot, n, dv.stream_handle())
     assert (dv.from_dev(th) == exp).all()
+
+
+@pytest.mark.parametrize("log_n", [12, 16])
+def test_field_sum(gpu, zo, log_n):
+    """bn254_c.rs:71 field_sum: rotated coefficient sums, a null coefficient
+    pointer, negative and wrapping rotations, and no terms at all"""
+    import torch
+    dv, call, A = _env()
+    n = 1 << log_n
+    omegas = zo.expand_omega(n, zo.root_of_unity(log_n))
+    vs = [_r(zo, n, 200 + j) for j in range(5)]
+    cs = [None, _r(zo, 1, 210), None, _r(zo, 1, 211), _r(zo, 1, 212)]
+    rots = [0, 4, -4, n + 3, -(n - 1)]
+    vptr = A.put_all(vs)
+    cptr_host = np.array([0 if c is None else A.put(c) for c in cs], dtype=np.uint64)
+    cptr = A.hold(torch.from_numpy(cptr_host.view(np.int64)).to(dv.device()))
+    rptr = A.hold(torch.tensor(rots, dtype=torch.int32, device=dv.device()))
+    res = dv.empty(n)
+    call("field_sum", dv.ptr(res), vptr, cptr, rptr, A.put(omegas), len(vs), n)
+    torch.cuda.synchronize()
+    assert (dv.from_dev(res) == zo.field_sum(vs, cs, rots, omegas)).all()
+    call("field_sum", dv.ptr(res), vptr, cptr, rptr, A.put(omegas), 0, n)
+    torch.cuda.synchronize()
+    assert (dv.from_dev(res) == 0).all()
+
+
+@pytest.mark.parametrize("log_n,rot", [(12, 4), (16, 4), (12, 1)])
+def test_lookup_eval_h(gpu, zo, log_n, rot):
+    """bn254_c.rs:90 lookup_eval_h against the oracle's five folds"""
+    dv, call, A = _env()
+    n = 1 << log_n
+    inp, tab, pin, ptab, z, l0, ll, la = [_r(zo, n, 220 + j) for j in range(8)]
+    y, beta, gamma = _r(zo, 1, 230), _r(zo, 1, 231), _r(zo, 1, 232)
+    res0 = _r(zo, n, 233)
+    d = dv.to_dev(res0)
+    call("lookup_eval_h", dv.ptr(d), *[A.put(x) for x in (inp, tab, pin, ptab, z, l0, ll, la, y, beta, gamma)],
+         rot, n, dv.stream_handle())
+    exp = zo.lookup_eval_h(res0, inp, tab, pin, ptab, z, l0, ll, la, y, beta, gamma, rot)
+    assert (dv.from_dev(d) == exp).all()

@@ -11,7 +11,11 @@ vanishes on every row, so the kernel must return exactly its accumulator
 times y^k (k = the number of folds); with one cell of the witness changed
 some row must differ.  Also: the product-sum helper equals
 prod * sum_j 1 / phi_j, and permutation_eval_h_l adds beta p(X) + gamma as
-polynomials (checked by evaluation at a random point).
+polynomials (checked by evaluation at a random point).  field_sum is the
+rotation theorem: scaling coefficient i by w^(r i) shifts the evaluations by
+r.  lookup_eval_h (the permuted-column lookup) vanishes on a permuted pair
+built as halo2 builds it (sorted inputs, each value's table row at its first
+occurrence) and fails when two table rows are exchanged.
 """
 import numpy as np
 import pytest
@@ -243,3 +247,72 @@ def test_shuffle_v2_vanishes_on_honest_witness(env, g):
     tabs[g - 1][11] = (tabs[g - 1][11] + 1) % R
     out = call(tabs)
     assert [i for i in range(N) if out[i] != res0[i] * pow(y, 3, R) % R] == [11]
+
+
+def test_field_sum_is_rotation_of_coefficients(env):
+    """multiplying coefficient i by w^(r i) evaluates p at w^r X: the NTT of
+    field_sum's output is sum_j c_j NTT(v_j) shifted by r_j"""
+    zo, rand, m, ints, l0, ll, la = env
+    log_n = 6
+    w = zo.root_of_unity(log_n)
+    wi = zo.from_mont(w)[0]
+    omegas = m([pow(wi, i, R) for i in range(N)])
+    vs = [m(rand(N)) for _ in range(4)]
+    rots = [0, 1, -1, 5]
+    cs = [None, m(rand(1)), m(rand(1)), None]
+    out = zo.field_sum(vs, cs, rots, omegas)
+    got = ints(zo.ntt(out, log_n, w))
+    evs = [ints(zo.ntt(v, log_n, w)) for v in vs]
+    cints = [1 if c is None else ints(c)[0] for c in cs]
+    for k in range(N):
+        assert got[k] == sum(cints[j] * evs[j][(k + rots[j]) % N] for j in range(4)) % R, k
+    assert (zo.field_sum(vs[:0] or [m([0] * N)], [None], [0], omegas) == 0).all()
+
+
+def _plookup_witness(rand):
+    """halo2's permuted pair: a' the sorted inputs, s' the table with each
+    input value placed at the value's first row in a', the rest anywhere;
+    z the product of (a + beta)(t + gamma) / ((a' + beta)(s' + gamma))"""
+    tvals = rand(U)
+    rng = np.random.default_rng(7)
+    a = [tvals[int(rng.integers(0, 12))] for _ in range(U)]
+    ap = sorted(a)
+    sp = [None] * U
+    rest = list(tvals)
+    for i in range(U):
+        if i == 0 or ap[i] != ap[i - 1]:
+            sp[i] = ap[i]
+            rest.remove(ap[i])
+    for i in range(U):
+        if sp[i] is None:
+            sp[i] = rest.pop()
+    beta, gamma = rand(2)
+    z = [1] + [0] * (N - 1)
+    for i in range(U):
+        num = (a[i] + beta) * (tvals[i] + gamma) % R
+        den = (ap[i] + beta) * (sp[i] + gamma) % R
+        z[i + 1] = z[i] * num % R * inv(den) % R
+    pad = rand(N - U)
+    return a + pad, tvals + pad, ap + pad, sp + pad, z, beta, gamma
+
+
+def test_lookup_eval_h_vanishes_on_permuted_pair(env):
+    zo, rand, m, ints, l0, ll, la = env
+    a, t, ap, sp, z, beta, gamma = _plookup_witness(rand)
+    assert z[U] == 1, "a valid permuted pair closes the product"
+    for i in range(U + 1, N):
+        z[i] = rand(1)[0]
+    inp = [(x + beta) % R for x in a]
+    tab = [(x + gamma) % R for x in t]
+    y = rand(1)[0]
+    res0 = rand(N)
+    call = lambda spv: ints(zo.lookup_eval_h(m(res0), m(inp), m(tab), m(ap), m(spv), m(z), m(l0), m(ll), m(la),
+                                             m([y]), m([beta]), m([gamma]), 1))
+    assert _expect_fold(call(sp), res0, y, 5)
+    # s' rows 0 and 1 exchanged: a'(0) = s'(0) fails, and so does the
+    # adjacency rule wherever the moved table value lands
+    bad = list(sp)
+    j = next(i for i in range(1, U) if ap[i] != ap[0])
+    bad[0], bad[j] = bad[j], bad[0]
+    out = call(bad)
+    assert out[0] != res0[0] * pow(y, 5, R) % R

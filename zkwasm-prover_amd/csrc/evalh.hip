@@ -262,6 +262,69 @@ __global__ void k_shuffle_v2(Fr *res, const Fr *const *__restrict__ inputs, cons
 
 bool bad_n(int n) { return n <= 0 || (n & (n - 1)); }
 
+// cuda/bn254.cu:496-540: res[i] = sum_j v_j[i] * w^(rot_j * i) (* c_j when
+// v_c[j] is set), the sum of coefficient vectors each rotated by w^rot_j.
+// The term pointers are staged in LDS once per workgroup.  The exponent is
+// taken mod n with 32-bit wrap-around, as the reference's int arithmetic
+// gives for every rotation (n divides 2^32).
+constexpr int kMaxSumTerms = 256;
+__global__ __launch_bounds__(kThreads) void k_field_sum(Fr *res, const Fr *const *__restrict__ v,
+                                                        const Fr *const *__restrict__ v_c,
+                                                        const int *__restrict__ v_rot,
+                                                        const Fr *__restrict__ omegas, int v_n, uint32_t n) {
+    __shared__ const Fr *sv[kMaxSumTerms];
+    __shared__ const Fr *sc[kMaxSumTerms];
+    __shared__ uint32_t sr[kMaxSumTerms];
+    for (int k = threadIdx.x; k < v_n; k += blockDim.x) {
+        sv[k] = v[k];
+        sc[k] = v_c[k];
+        sr[k] = (uint32_t)v_rot[k];
+    }
+    __syncthreads();
+    GRID_LOOP(i, n) {
+        Fr acc = fe_zero<FrParams>();
+        for (int j = 0; j < v_n; j++) {
+            Fr t = ld(&sv[j][i]) * ld(&omegas[(sr[j] * i) & (n - 1)]);
+            if (sc[j]) t = t * ld(sc[j]);
+            acc = j ? acc + t : t;
+        }
+        st(&res[i], acc);
+    }
+}
+
+// cuda/bn254.cu:798-866: the permuted-column lookup's five folds
+//   l0 (1 - z); l_last (z^2 - z);
+//   l_active (z(wX) (a' + beta)(s' + gamma) - z(X) input * table);
+//   l0 (a' - s'); l_active (a' - s')(a' - a'(w^-1 X))
+// where input and table are the compressed expressions with beta and gamma
+// already added.
+__global__ void k_lookup_eval_h(Fr *res, const Fr *__restrict__ input, const Fr *__restrict__ table,
+                                const Fr *__restrict__ pin, const Fr *__restrict__ ptab, const Fr *__restrict__ z,
+                                const Fr *__restrict__ l0, const Fr *__restrict__ l_last,
+                                const Fr *__restrict__ l_active, const Fr *__restrict__ yp,
+                                const Fr *__restrict__ betap, const Fr *__restrict__ gammap, int rot, uint32_t n) {
+    const Fr y = ld(yp), beta = ld(betap), gamma = ld(gammap), one = fe_one<FrParams>();
+    GRID_LOOP(i, n) {
+        const uint32_t rn = (i + (uint32_t)rot) & (n - 1);
+        const uint32_t rp = (i + n - (uint32_t)rot) & (n - 1);
+        const Fr zi = ld(&z[i]), a = ld(&pin[i]), sp = ld(&ptab[i]);
+        Fr t = ld(&res[i]) * y;
+        t = t + ld(&l0[i]) * (one - zi);
+        t = t * y;
+        t = t + ld(&l_last[i]) * (fe_sqr<FrParams>(zi) - zi);
+        t = t * y;
+        const Fr la = ld(&l_active[i]);
+        Fr u = (a + beta) * (sp + gamma) * ld(&z[rn]) - zi * (ld(&input[i]) * ld(&table[i]));
+        t = t + u * la;
+        t = t * y;
+        const Fr d = a - sp;
+        t = t + ld(&l0[i]) * d;
+        t = t * y;
+        t = t + (a - ld(&pin[rp])) * d * la;
+        st(&res[i], t);
+    }
+}
+
 // One permutation set's product term on the extended coset, fused:
 // h = h*y + l_active * ( z(wX) * prod_c (v_c + beta*sigma_c + gamma)
 //                      - z(X)  * prod_c (v_c + gamma + bd_c * X) )
@@ -451,6 +514,35 @@ hipError_t zk_permutation_eval_h_set(void *h, const void *z, const void *v, cons
     k_perm_set_eval<<<grid_for((uint32_t)n), kThreads, 0, stream>>>(
         (Fr *)h, (const Fr *)z, (const Fr *const *)v, (const Fr *const *)sigma, nc, (const Fr *)beta,
         (const Fr *)gamma, (const Fr *)bd, (const Fr *)X, (const Fr *)l_active, (const Fr *)y, rot, (uint32_t)n);
+    return hipGetLastError();
+}
+
+// cuda/bn254.cu:1217-1230 (declared at src/cuda/bn254_c.rs:71; the reference
+// never calls it).  v, v_c: device arrays of v_n device pointers (v_c entries
+// may be null), v_rot: device array of v_n rotations, omegas: w^i for i < n.
+// Runs on the null stream, as the reference's does.
+hipError_t field_sum(void *res, const void *v, const void *v_c, const void *v_rot, const void *omegas, int v_n,
+                     int n) {
+    if (bad_n(n) || v_n < 0 || v_n > kMaxSumTerms || !res || (v_n && (!v || !v_c || !v_rot || !omegas)))
+        return hipErrorInvalidValue;
+    k_field_sum<<<grid_for((uint32_t)n), kThreads, 0, 0>>>((Fr *)res, (const Fr *const *)v, (const Fr *const *)v_c,
+                                                          (const int *)v_rot, (const Fr *)omegas, v_n, (uint32_t)n);
+    return hipGetLastError();
+}
+
+// cuda/bn254.cu:1430-1456 (declared at src/cuda/bn254_c.rs:90; the reference
+// never calls it)
+hipError_t lookup_eval_h(void *res, const void *input, const void *table, const void *permuted_input,
+                         const void *permuted_table, const void *z, const void *l0, const void *l_last,
+                         const void *l_active_row, const void *y, const void *beta, const void *gamma, int rot, int n,
+                         hipStream_t stream) {
+    if (bad_n(n) || !res || !input || !table || !permuted_input || !permuted_table || !z || !l0 || !l_last ||
+        !l_active_row || !y || !beta || !gamma)
+        return hipErrorInvalidValue;
+    k_lookup_eval_h<<<grid_for((uint32_t)n), kThreads, 0, stream>>>(
+        (Fr *)res, (const Fr *)input, (const Fr *)table, (const Fr *)permuted_input, (const Fr *)permuted_table,
+        (const Fr *)z, (const Fr *)l0, (const Fr *)l_last, (const Fr *)l_active_row, (const Fr *)y, (const Fr *)beta,
+        (const Fr *)gamma, rot, (uint32_t)n);
     return hipGetLastError();
 }
 

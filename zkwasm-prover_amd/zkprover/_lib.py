@@ -25,6 +25,8 @@ SIGNATURES = {
     "field_op": [P, P, I32, P, P, I32, P, I32, I32, P],
     "field_op_batch_mul_sum": [P, P, P, I32, I32, P],
     "field_mul_zip": [P, P, I32, I32],
+    "field_sum": [P, P, P, P, P, I32, I32],
+    "lookup_eval_h": [P, P, P, P, P, P, P, P, P, P, P, P, I32, I32, P],
     "permutation_eval_h_p1": [P, P, P, P, P, P, I32],
     "permutation_eval_h_p2": [P, P, P, P, P, I32, I32, I32],
     "permutation_eval_h_l": [P, P, P, P, I32, P],
