@@ -169,6 +169,21 @@ def main():
         call("ntt", buf_a.data_ptr(), buf_b.data_ptr(), None, om_N.data_ptr(), None, LOG_N + EXT, 8,
              ctypes.addressof(swap), st)
 
+    class _CosetItem(ctypes.Structure):
+        _fields_ = [("src", ctypes.c_void_p), ("buf", ctypes.c_void_p), ("tmp", ctypes.c_void_p),
+                    ("omegas", ctypes.c_void_p), ("log_n", ctypes.c_int), ("log_ext", ctypes.c_int)]
+    L.zk_coset_extend_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+    coset_item = _CosetItem(coset_in.data_ptr(), buf_a.data_ptr(), buf_b.data_ptr(), om_N.data_ptr(), LOG_N, EXT)
+    coset_sw = ctypes.c_ubyte(0)
+
+    def run_coset_fused():
+        # the same extension in one pipeline: the first pass reads the 2^22
+        # coefficients and scales them on load
+        if L.zk_coset_extend_batch(ctypes.addressof(coset_item), 1, coset_powers.data_ptr(), 3,
+                                   ctypes.addressof(coset_sw), st) != 0:
+            raise RuntimeError("zk_coset_extend_batch")
+
     def restore(dst, src):
         return lambda: dst[: src.numel()].copy_(src)
 
@@ -177,6 +192,7 @@ def main():
              ("ntt_2^22", restore(buf_a, ntt_in), lambda: run_ntt(LOG_N, om_n, None)),
              ("intt_2^22", restore(buf_a, ntt_in), lambda: run_ntt(LOG_N, om_n_inv, n_inv)),
              ("coset_ntt_2^22_to_2^24", restore(buf_a, coset_in), run_coset),
+             ("coset_extend_fused_2^22_to_2^24", None, run_coset_fused),
              ("batch_inversion_2^22", restore(inv_buf, inv_in),
               lambda: call("zk_batch_invert", inv_buf.data_ptr(), inv_tmp.data_ptr(), n, st)),
              ("prepare_table_lookup_2^22", None, run_prepare_table),
@@ -246,6 +262,8 @@ def main():
         "intt_2^22": fracs(prim_ms["intt_2^22"], 2 * n * 32, n // 2 * LOG_N + n),
         "coset_ntt_2^22_to_2^24": fracs(prim_ms["coset_ntt_2^22_to_2^24"], n * 32 + N * 32,
                                         N // 2 * (LOG_N + EXT) + n),
+        "coset_extend_fused_2^22_to_2^24": fracs(prim_ms["coset_extend_fused_2^22_to_2^24"], n * 32 + N * 32,
+                                                 N // 2 * (LOG_N + EXT) + n),
         "batch_inversion_2^22": fracs(prim_ms["batch_inversion_2^22"], 2 * n * 32, 3 * n),
         "logup_eval_h_2^24": fracs(prim_ms["logup_eval_h_2^24"], 11 * N * 32, 9 * N),
     }

@@ -303,6 +303,25 @@ typedef struct {
 } zk_ntt_item;
 hipError_t zk_ntt_batch(const zk_ntt_item *items, int count, unsigned char *swapped, hipStream_t stream);
 
+/* Coset extension in one pipeline (the reference's extended_prepare followed
+ * by ntt, bn254_c.rs:31 and :6, as called from src/eval_h.rs:250-337): for
+ * each item, src's 2^log_n coefficients zero-padded to 2^(log_n + log_ext),
+ * scaled by coset_powers[(i % coset_powers_n) - 1] where i % coset_powers_n
+ * != 0, and transformed with omegas (the 2^(log_n + log_ext) table).  The
+ * first pass reads only the coefficients and scales them on load; no zero
+ * padding is written.  Results equal the three-call sequence; each lands in
+ * buf, or in tmp when swapped[m] is set.  src must not alias buf or tmp. */
+typedef struct {
+    const void *src;
+    void *buf;
+    void *tmp;
+    const void *omegas;
+    int log_n;
+    int log_ext;
+} zk_coset_item;
+hipError_t zk_coset_extend_batch(const zk_coset_item *items, int count, const void *coset_powers,
+                                 int coset_powers_n, unsigned char *swapped, hipStream_t stream);
+
 /* Radix choice per size (autotuner hook): largest pass degree for 2^log_n
  * transforms, 1..8 (0 = default 8). */
 hipError_t zk_ntt_set_max_deg(int log_n, int max_deg);

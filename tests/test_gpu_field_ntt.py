@@ -5,6 +5,8 @@ inputs.  NTT cases follow the reference's own test (src/profile.rs:174-269):
 s_0 random, s_i = s_{i-1} + i, forward NTT equal to best_fft (the DFT) and the
 iNTT with divisor 2^-k round-tripping exactly.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -272,3 +274,54 @@ def test_fp29_ops_match_oracle(gpu, zo, op):
     exp = {0: [x * y % P for x, y in zip(xs, ys)], 1: [(x + y) % P for x, y in zip(xs, ys)],
            2: [(x - y) % P for x, y in zip(xs, ys)], 3: [x * x % P for x in xs], 4: xs}[op]
     assert [zo.fp_from_mont_int(got[i]) for i in range(n)] == exp
+
+
+class _CosetItem(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("buf", ctypes.c_void_p), ("tmp", ctypes.c_void_p),
+                ("omegas", ctypes.c_void_p), ("log_n", ctypes.c_int), ("log_ext", ctypes.c_int)]
+
+
+def _coset_ref(zo, dv, call, coeffs, cp, log_n, log_ext, om):
+    """the three-call sequence: copy into an extended buffer, extended_prepare, ntt"""
+    import ctypes
+    n, N = 1 << log_n, 1 << (log_n + log_ext)
+    host = np.zeros((N, 4), dtype=np.uint64)
+    host[:n] = coeffs
+    s, tmp = dv.to_dev(host), dv.empty(N)
+    dcp = dv.to_dev(cp)
+    call("extended_prepare", dv.ptr(s), dv.ptr(dcp), len(cp) + 1, n, N, 0, dv.stream_handle())
+    swap = ctypes.c_bool(False)
+    call("ntt", dv.ptr(s), dv.ptr(tmp), None, dv.ptr(om), None, log_n + log_ext, 8, ctypes.addressof(swap),
+         dv.stream_handle())
+    return dv.from_dev(tmp if swap.value else s)
+
+
+@pytest.mark.parametrize("shapes", [[(10, 2)], [(22, 2)], [(12, 1), (9, 3), (16, 2), (4, 2)]])
+def test_coset_extend_batch_equals_three_calls(gpu, zo, shapes):
+    import ctypes
+    from zkprover import device as dv
+    from zkprover._lib import call, lib
+    f = lib().zk_coset_extend_batch
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    zeta = zo.to_mont([pow(7, (R - 1) // 3, R)])
+    cp = np.concatenate([zeta, zo.fr_mul(zeta, zeta)])
+    keep, items, exps = [], (_CosetItem * len(shapes))(), []
+    for m, (log_n, log_ext) in enumerate(shapes):
+        n, N = 1 << log_n, 1 << (log_n + log_ext)
+        coeffs = zo.rand_fr(n, 900 + 7 * log_n + log_ext)
+        om = _omegas_dev(zo, log_n + log_ext, zo.root_of_unity(log_n + log_ext))
+        exps.append(_coset_ref(zo, dv, call, coeffs, cp, log_n, log_ext, om))
+        src, buf, tmp = dv.to_dev(coeffs), dv.empty(N), dv.empty(N)
+        keep += [src, buf, tmp, om]
+        items[m] = _CosetItem(src.data_ptr(), buf.data_ptr(), tmp.data_ptr(), om.data_ptr(), log_n, log_ext)
+    dcp = dv.to_dev(cp)
+    sw = (ctypes.c_ubyte * len(shapes))()
+    assert f(ctypes.addressof(items), len(shapes), dcp.data_ptr(), 3, sw, dv.stream_handle()) == 0
+    for m in range(len(shapes)):
+        got = dv.from_dev(keep[4 * m + 2] if sw[m] else keep[4 * m + 1])
+        assert (got == exps[m]).all(), shapes[m]
+    # a source that aliases the output is refused
+    bad = (_CosetItem * 1)(_CosetItem(keep[1].data_ptr(), keep[1].data_ptr(), keep[2].data_ptr(), keep[3].data_ptr(),
+                                      shapes[0][0], shapes[0][1]))
+    assert f(ctypes.addressof(bad), 1, dcp.data_ptr(), 3, sw, dv.stream_handle()) != 0

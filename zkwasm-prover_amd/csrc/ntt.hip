@@ -37,6 +37,11 @@ struct NttJob {
     Fr *y;
     const Fr *omegas;
     const Fr *n_inv;  // null except on an iNTT's last pass
+    // first pass of a coset extension: x holds nz coefficients (the rest of
+    // the 2^log_n input is zero and never read), element i scaled by
+    // coset[(i % cn) - 1] when i % cn != 0; nz = 0: x is the whole input
+    const Fr *coset;
+    uint32_t nz, cn;
     uint32_t block0;
     uint8_t log_n, log_p, deg, log_g;
 };
@@ -59,6 +64,8 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
     const Fr *__restrict__ omegas = a.job[lo].omegas;
     const Fr *__restrict__ n_inv = a.job[lo].n_inv;
     const uint32_t log_n = a.job[lo].log_n, log_p = a.job[lo].log_p, deg = a.job[lo].deg, log_g = a.job[lo].log_g;
+    const uint32_t nz = a.job[lo].nz, cn = a.job[lo].cn;
+    const Fr *__restrict__ coset = a.job[lo].coset;
     const uint32_t blk = blockIdx.x - a.job[lo].block0;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t D = 1u << deg;
@@ -76,7 +83,17 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
     for (uint32_t e = tid; e < tile; e += bs) {
         uint32_t g = e & (G - 1), i = e >> log_g;
         uint32_t index = index0 + g;
-        Fr v = fe_load<FrParams>(&x[(size_t)index + ((size_t)i << log_t)]);
+        const size_t pos = (size_t)index + ((size_t)i << log_t);
+        Fr v;
+        if (nz && pos >= nz) {
+            v = fe_zero<FrParams>();
+        } else {
+            v = fe_load<FrParams>(&x[pos]);
+            if (cn > 1) {
+                const uint32_t r = (uint32_t)(pos % cn);
+                if (r) v = v * fe_load<FrParams>(&coset[r - 1]);
+            }
+        }
         if (log_p) {
             uint32_t k = index & (p - 1);
             uint32_t ex = (k * i) << base_shift;
@@ -177,11 +194,15 @@ int ntt_plan(uint32_t log_n, uint32_t *degs) {
     return (int)passes;
 }
 
-// One transform of a batch: input in a, the other buffer b.
+// One transform of a batch: input in a, the other buffer b.  With src set,
+// the first pass reads its input from src instead (nz coefficients, the rest
+// zero, scaled by coset as in NttJob) and a only receives later passes.
 struct NttSpec {
     Fr *a, *b;
     const Fr *omegas, *n_inv;
     uint32_t log_n;
+    const Fr *src = nullptr, *coset = nullptr;
+    uint32_t nz = 0, cn = 0;
 };
 
 // Runs a batch of NTTs: pass r of every transform that has one goes into the
@@ -223,6 +244,12 @@ hipError_t ntt_run_batch(const NttSpec *specs, uint32_t count, hipStream_t strea
             NttJob &j = args.job[args.count++];
             j.x = (r & 1) ? sp.b : sp.a;
             j.y = (r & 1) ? sp.a : sp.b;
+            if (r == 0 && sp.src) {
+                j.x = sp.src;
+                j.nz = sp.nz;
+                j.coset = sp.coset;
+                j.cn = sp.cn;
+            }
             j.omegas = sp.omegas;
             j.n_inv = r == np[m] - 1 ? sp.n_inv : nullptr;
             j.block0 = blocks;
@@ -339,6 +366,38 @@ hipError_t zk_ntt_batch(const zk_ntt_item *items, int count, unsigned char *swap
             return hipErrorInvalidValue;
         specs[m] = NttSpec{(Fr *)it.buf, (Fr *)it.tmp, (const Fr *)it.omegas, (const Fr *)it.n_inv,
                            (uint32_t)it.log_n};
+    }
+    std::vector<int> passes(count);
+    ZK_CHECK(ntt_run_batch(specs.data(), (uint32_t)count, stream, passes.data()));
+    for (int m = 0; m < count; m++) swapped[m] = (passes[m] & 1) != 0;
+    return hipSuccess;
+}
+
+// The coset extension in one pipeline: for each item, the 2^log_n
+// coefficients at src, zero-padded to 2^(log_n + log_ext), scaled by
+// coset_powers[(i % coset_powers_n) - 1] where i % coset_powers_n != 0 and
+// transformed with omegas (the 2^(log_n + log_ext) w^i table) -- the same
+// values as copying src into buf, extended_prepare(buf, coset_powers,
+// coset_powers_n, 2^log_n, 2^(log_n + log_ext), 0) and ntt(buf, tmp).  The
+// first pass reads only the coefficients and scales them as it loads, so the
+// zero padding is never written or read.  The result lands in buf, or in tmp
+// when swapped[m] is set; src may not alias buf or tmp.
+hipError_t zk_coset_extend_batch(const zk_coset_item *items, int count, const void *coset_powers,
+                                 int coset_powers_n, unsigned char *swapped, hipStream_t stream) {
+    if (count <= 0 || !items || !swapped || coset_powers_n < 0 || (coset_powers_n > 1 && !coset_powers))
+        return hipErrorInvalidValue;
+    std::vector<NttSpec> specs(count);
+    for (int m = 0; m < count; m++) {
+        const zk_coset_item &it = items[m];
+        if (it.log_n < 0 || it.log_ext < 0 || it.log_n + it.log_ext < 1 || it.log_n + it.log_ext > 30 || !it.src ||
+            !it.buf || !it.tmp || !it.omegas || it.src == it.buf || it.src == it.tmp)
+            return hipErrorInvalidValue;
+        NttSpec &sp = specs[m];
+        sp = NttSpec{(Fr *)it.buf, (Fr *)it.tmp, (const Fr *)it.omegas, nullptr, (uint32_t)(it.log_n + it.log_ext)};
+        sp.src = (const Fr *)it.src;
+        sp.nz = 1u << it.log_n;
+        sp.coset = (const Fr *)coset_powers;
+        sp.cn = coset_powers_n > 1 ? (uint32_t)coset_powers_n : 0u;
     }
     std::vector<int> passes(count);
     ZK_CHECK(ntt_run_batch(specs.data(), (uint32_t)count, stream, passes.data()));

@@ -219,14 +219,13 @@ hipError_t omega_table(void *tab, const void *w, int log, const void *one, hipSt
     if (log > 0) PCHECK(hipMemcpyAsync((uint8_t *)tab + FR, w, FR, hipMemcpyDeviceToDevice, st));
     return zk_expand_omega_buffer_on(tab, log, st);
 }
-// coefficients (n, in place in buf with tmp) -> extended coset values (N)
-// ext: copy, zero-pad, distribute [zeta, zeta^2], NTT of 2^ek
+// coefficients (n) -> extended coset values (N): zero-pad, distribute
+// [zeta, zeta^2], NTT of 2^ek, in one pipeline (the padding is never written)
 hipError_t to_coset(const Pk &pk, void *ext, const void *coef, void *tmp_e, void **result, hipStream_t st) {
     const Shape &s = pk.sh;
-    PCHECK(hipMemcpyAsync(ext, coef, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
-    PCHECK(extended_prepare(ext, pk.cf(C_FWD0), 3, (int)s.n, (int)s.N, 0, st));
-    bool sw = false;
-    PCHECK(ntt(ext, tmp_e, nullptr, pk.w_e, nullptr, s.ek, 8, &sw, st));
+    zk_coset_item it{coef, ext, tmp_e, pk.w_e, s.k, kExtBits};
+    unsigned char sw = 0;
+    PCHECK(zk_coset_extend_batch(&it, 1, pk.cf(C_FWD0), 3, &sw, st));
     *result = sw ? tmp_e : ext;
     return hipSuccess;
 }
@@ -548,15 +547,11 @@ hipError_t extend_all(Job &j, const std::vector<const void *> &coefs, int first,
     const Pk &pk = *j.pk;
     for (size_t g = 0; g < coefs.size(); g += kExtGroup) {
         int cnt = (int)(coefs.size() - g < (size_t)kExtGroup ? coefs.size() - g : kExtGroup);
-        std::vector<zk_ntt_item> it(cnt);
-        for (int m = 0; m < cnt; m++) {
-            void *e = j.ext[first + g + m];
-            PCHECK(hipMemcpyAsync(e, coefs[g + m], (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
-            PCHECK(extended_prepare(e, pk.cf(C_FWD0), 3, (int)s.n, (int)s.N, 0, st));
-            it[m] = zk_ntt_item{e, j.ext_tmp[m], pk.w_e, nullptr, s.ek, 0};
-        }
+        std::vector<zk_coset_item> it(cnt);
+        for (int m = 0; m < cnt; m++)
+            it[m] = zk_coset_item{coefs[g + m], j.ext[first + g + m], j.ext_tmp[m], pk.w_e, s.k, kExtBits};
         std::vector<unsigned char> sw(cnt);
-        PCHECK(zk_ntt_batch(it.data(), cnt, sw.data(), st));
+        PCHECK(zk_coset_extend_batch(it.data(), cnt, pk.cf(C_FWD0), 3, sw.data(), st));
         for (int m = 0; m < cnt; m++)
             if (sw[m]) std::swap(j.ext[first + g + m], j.ext_tmp[m]);
     }
