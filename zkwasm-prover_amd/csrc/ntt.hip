@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "zkcore.h"
+#include "zkinternal.h"
 
 using namespace zk;
 
@@ -276,6 +277,22 @@ hipError_t ntt_run(Fr *a, Fr *b, const Fr *omegas, const Fr *n_inv, uint32_t log
     int passes = 0;
     ZK_CHECK(ntt_run_batch(&sp, 1, stream, &passes));
     if (passes_out) *passes_out = passes;
+    return hipSuccess;
+}
+
+hipError_t ntt_batch_from(const void *const *src, void *const *buf, void *const *tmp, const void *omegas,
+                          const void *n_inv, int log_n, int count, unsigned char *swapped, hipStream_t stream) {
+    if (count <= 0 || log_n < 1 || log_n > 30 || !omegas) return hipErrorInvalidValue;
+    std::vector<NttSpec> specs(count);
+    for (int m = 0; m < count; m++) {
+        if (!src[m] || !buf[m] || !tmp[m] || src[m] == buf[m] || src[m] == tmp[m]) return hipErrorInvalidValue;
+        specs[m] = NttSpec{(Fr *)buf[m], (Fr *)tmp[m], (const Fr *)omegas, (const Fr *)n_inv, (uint32_t)log_n};
+        specs[m].src = (const Fr *)src[m];
+        specs[m].nz = 1u << log_n;
+    }
+    std::vector<int> passes(count);
+    ZK_CHECK(ntt_run_batch(specs.data(), (uint32_t)count, stream, passes.data()));
+    for (int m = 0; m < count; m++) swapped[m] = (passes[m] & 1) != 0;
     return hipSuccess;
 }
 

@@ -528,13 +528,10 @@ hipError_t intt_all(Job &j, std::vector<void *> &lag, std::vector<void *> &coef,
     const Pk &pk = *j.pk;
     size_t c = lag.size();
     if (!c) return hipSuccess;
-    std::vector<zk_ntt_item> it(c);
-    for (size_t i = 0; i < c; i++) {
-        PCHECK(hipMemcpyAsync(coef[i], lag[i], (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
-        it[i] = zk_ntt_item{coef[i], tmp[i], pk.w_n_inv, pk.cf(C_NINV), s.k, 0};
-    }
+    // the first pass reads the Lagrange values in place: no copy into coef
     std::vector<unsigned char> sw(c);
-    PCHECK(zk_ntt_batch(it.data(), (int)c, sw.data(), st));
+    PCHECK(zk::ntt_batch_from(lag.data(), coef.data(), tmp.data(), pk.w_n_inv, pk.cf(C_NINV), s.k, (int)c, sw.data(),
+                              st));
     for (size_t i = 0; i < c; i++)
         if (sw[i]) std::swap(coef[i], tmp[i]);
     return hipSuccess;

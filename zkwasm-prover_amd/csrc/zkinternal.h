@@ -20,4 +20,10 @@ hipError_t wit_sigma_rows(void *const *sig_dev, const uint32_t *target, const vo
 hipError_t wit_fill_value(void *col, const void *v, uint32_t n, hipStream_t s);
 hipError_t wit_any_nonzero(const void *col, uint64_t lo, uint64_t hi, uint32_t *flag, hipStream_t s);
 
+// NTTs whose first pass reads src[m] (2^log_n elements, left unchanged)
+// instead of buf[m]: the values of copying src into buf and running ntt with
+// tmp.  swapped[m] set: the result is in tmp[m], else in buf[m]. (ntt.hip)
+hipError_t ntt_batch_from(const void *const *src, void *const *buf, void *const *tmp, const void *omegas,
+                          const void *n_inv, int log_n, int count, unsigned char *swapped, hipStream_t s);
+
 }  // namespace zk
