@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "inv_bingcd.h"
+
 namespace zk {
 
 struct FrParams {
@@ -28,6 +30,9 @@ struct FrParams {
     static constexpr uint32_t R3[8] = {0xb4bf0040u, 0x5e94d8e1u, 0x1cfbb6b8u, 0x2a489cbeu,
                                        0xa19fcfedu, 0x893cc664u, 0x7fcc657cu, 0x0cf8594bu};
     static constexpr uint32_t INV = 0xefffffffu;  // -r^-1 mod 2^32
+    // 2^36 R^3 mod r: turns the binary GCD's cofactor into the Montgomery inverse
+    static constexpr uint32_t INVC[8] = {0x962db482u, 0x4cfbeddbu, 0xd900a306u, 0xb4125eb3u,
+                                         0x544b9a4du, 0x9d64b74cu, 0x0a6a2c2du, 0x2731b13cu};
 };
 
 struct FpParams {
@@ -40,6 +45,9 @@ struct FpParams {
     static constexpr uint32_t R3[8] = {0xda1530dfu, 0xb1cd6dafu, 0xa7283db6u, 0x62f210e6u,
                                        0x0ada0afbu, 0xef7f0b0cu, 0x2d592544u, 0x20fd6e90u};
     static constexpr uint32_t INV = 0xe4866389u;  // -p^-1 mod 2^32
+    // 2^36 R^3 mod p (as INVC of FrParams)
+    static constexpr uint32_t INVC[8] = {0xc57e1fafu, 0x12f8df1fu, 0xa75c06e1u, 0x516b8b3cu,
+                                         0xe6486ed4u, 0x448dbc80u, 0x2a2a6c68u, 0x00d762b6u};
 };
 
 template <class P>
@@ -328,9 +336,25 @@ __device__ Fe<P> fe_inv_raw(const Fe<P> &a) {
     }
     return raw_is_one<P>(u) ? fe_from_raw<P>(x1) : fe_from_raw<P>(x2);
 }
-// Montgomery-domain inverse: (aR)^-1 * R^3 * R^-1 = a^-1 R.
+// The optimized binary GCD (inv_bingcd.h) as one out-of-line device
+// function per translation unit: it is large, and inversions are rare.
+__device__ __noinline__ inline bool bingcd_dev(uint32_t v[8], const uint32_t y[8], const uint32_t M[8],
+                                               uint32_t minv) {
+    return zkinv::bingcd(v, y, M, minv);
+}
+
+// Montgomery-domain inverse, a^-1 R for a = aR: the binary GCD's cofactor v
+// (y^-1 = v 2^36 for y = aR) times INVC = 2^36 R^3 under one Montgomery
+// product.  The result is checked (a * a^-1 = R); should the check fail, the
+// one-bit-at-a-time extended Euclid gives (aR)^-1 and R^3 makes it a^-1 R.
+// 0 -> 0.
 template <class P>
 __device__ Fe<P> fe_inv(const Fe<P> &a) {
+    if (fe_is_zero<P>(a)) return fe_zero<P>();
+    uint32_t v[8];
+    const bool ok = bingcd_dev(v, a.v, P::M, P::INV);
+    Fe<P> r = fe_mul<P>(fe_from_raw<P>(v), fe_from_raw<P>(P::INVC));
+    if (ok && fe_eq<P>(fe_mul<P>(a, r), fe_one<P>())) return r;
     Fe<P> x = fe_inv_raw<P>(a);
     return fe_mul<P>(x, fe_from_raw<P>(P::R3));
 }
