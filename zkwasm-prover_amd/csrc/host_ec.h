@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include "inv_bingcd.h"
+
 namespace zkh {
 
 typedef unsigned __int128 u128;
@@ -100,7 +102,7 @@ inline Fp fp_mul(const Fp &a, const Fp &b) {
 inline Fp fp_sqr(const Fp &a) { return fp_mul(a, a); }
 inline Fp fp_dbl(const Fp &a) { return fp_add(a, a); }
 // a^(p-2) (Fermat); 0 -> 0
-inline Fp fp_inv(const Fp &a) {
+inline Fp fp_inv_fermat(const Fp &a) {
     uint64_t e[4] = {kP[0] - 2, kP[1], kP[2], kP[3]};
     Fp acc = fp_one(), base = a;
     for (int i = 0; i < 4; i++)
@@ -109,6 +111,27 @@ inline Fp fp_inv(const Fp &a) {
             base = fp_sqr(base);
         }
     return acc;
+}
+// Montgomery inverse by the optimized binary GCD (inv_bingcd.h): the cofactor
+// v of y = aR times 2^36 R^3 under one Montgomery product is a^-1 R; checked
+// by one product, Fermat otherwise.  0 -> 0.
+inline Fp fp_inv(const Fp &a) {
+    if (fp_is_zero(a)) return fp_zero();
+    static const uint32_t kP32[8] = {0xd87cfd47u, 0x3c208c16u, 0x6871ca8du, 0x97816a91u,
+                                     0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+    static const Fp kInvC = {{0x12f8df1fc57e1fafULL, 0x516b8b3ca75c06e1ULL, 0x448dbc80e6486ed4ULL,
+                              0x00d762b62a2a6c68ULL}};  // 2^36 R^3 mod p
+    uint32_t y[8], v[8];
+    for (int i = 0; i < 4; i++) {
+        y[2 * i] = (uint32_t)a.l[i];
+        y[2 * i + 1] = (uint32_t)(a.l[i] >> 32);
+    }
+    const bool ok = zkinv::bingcd(v, y, kP32, 0xe4866389u);
+    Fp r;
+    for (int i = 0; i < 4; i++) r.l[i] = (uint64_t)v[2 * i] | ((uint64_t)v[2 * i + 1] << 32);
+    r = fp_mul(r, kInvC);
+    if (ok && fp_eq(fp_mul(a, r), fp_one())) return r;
+    return fp_inv_fermat(a);
 }
 
 inline Jac jac_id() { return Jac{fp_zero(), fp_one(), fp_zero()}; }
