@@ -1181,11 +1181,17 @@ hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stre
         for (uint32_t m = 0; m < batch; m++) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
     } else {
         std::vector<std::thread> th;
-        for (unsigned t = 0; t < nt; t++)
-            th.emplace_back([&, t]() {
-                for (uint32_t m = t; m < batch; m += nt) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
-            });
+        unsigned started = 0;
+        try {
+            for (unsigned t = 0; t < nt; t++, started++)
+                th.emplace_back([&, t]() {
+                    for (uint32_t m = t; m < batch; m += nt) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
+                });
+        } catch (...) {
+        }
         for (auto &x : th) x.join();
+        for (unsigned t = started; t < nt; t++)  // shares whose thread could not be started
+            for (uint32_t m = t; m < batch; m += nt) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
     }
     for (uint32_t m = 0; m < batch; m++)
         ZK_CHECK(hipMemcpyAsync(slices[m], &out[(size_t)m * 96], 96, hipMemcpyHostToDevice, stream));
