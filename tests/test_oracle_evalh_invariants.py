@@ -15,7 +15,10 @@ polynomials (checked by evaluation at a random point).  field_sum is the
 rotation theorem: scaling coefficient i by w^(r i) shifts the evaluations by
 r.  lookup_eval_h (the permuted-column lookup) vanishes on a permuted pair
 built as halo2 builds it (sorted inputs, each value's table row at its first
-occurrence) and fails when two table rows are exchanged.
+occurrence) and fails when two table rows are exchanged.  On arbitrary values
+shuffle_eval_h_v2 over g groups equals shuffle_eval_h on the group products,
+which ties v2's fold order (the one the prover core's proofs check) to
+shuffle_eval_h and to several groups.
 """
 import numpy as np
 import pytest
@@ -316,3 +319,33 @@ def test_lookup_eval_h_vanishes_on_permuted_pair(env):
     bad[0], bad[j] = bad[j], bad[0]
     out = call(bad)
     assert out[0] != res0[0] * pow(y, 5, R) % R
+
+
+@pytest.mark.parametrize("g,rot", [(1, 1), (2, 4), (3, 1), (16, 4)])
+def test_shuffle_v2_is_shuffle_on_group_products(env, g, rot):
+    """on arbitrary (not honest) values, shuffle_eval_h_v2 over g groups equals
+    shuffle_eval_h with input prod_j (a_j + beta_j) and table prod_j (s_j +
+    beta_j), row by row: v2 folds the same three terms in the same order.  The
+    prover core calls v2 with one group, so its fold order is the one the
+    proof's quotient identity checks; this carries that to shuffle_eval_h and
+    to several groups"""
+    zo, rand, m, ints, l0, ll, la = env
+    betas = rand(g)
+    ins = [rand(N) for _ in range(g)]
+    tabs = [rand(N) for _ in range(g)]
+    z, res0, y = rand(N), rand(N), rand(1)[0]
+    num, den = [1] * N, [1] * N
+    for i in range(N):
+        for j in range(g):
+            num[i] = num[i] * (ins[j][i] + betas[j]) % R
+            den[i] = den[i] * (tabs[j][i] + betas[j]) % R
+    v2 = zo.shuffle_eval_h_v2(m(res0), [m(x) for x in ins], [m(x) for x in tabs], m(betas), m(z), m(l0), m(ll),
+                              m(la), m([y]), rot)
+    one = zo.shuffle_eval_h(m(res0), m(num), m(den), m(z), m(l0), m(ll), m(la), m([y]), rot)
+    assert ints(v2) == ints(one)
+    # a swapped pair of groups' betas changes the products, so some row differs
+    if g > 1:
+        sw = betas[1:2] + betas[:1] + betas[2:]
+        v2b = zo.shuffle_eval_h_v2(m(res0), [m(x) for x in ins], [m(x) for x in tabs], m(sw), m(z), m(l0), m(ll),
+                                   m(la), m([y]), rot)
+        assert ints(v2b) != ints(one)

@@ -128,7 +128,7 @@ bool make_shape(const zkp_circuit *c, Shape &s) {
     s.F = c->fixed < 2 ? 2 : c->fixed;
     s.I = c->instance < 0 ? 0 : c->instance;
     if (s.A < s.col_shuffle + 2 * s.S) return false;
-    if (4 * s.P + 2 * s.Sp + 24 > 512) return false;  // device pointer area of a job
+    if (4 * s.P + 2 * s.Sp + 24 + 2 * s.S > 512) return false;  // device pointer area of a job
     return true;
 }
 
@@ -1069,10 +1069,12 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
                                     (int)s.n, st));
     }
     // device pointer arrays: [values | sigmas | zs], then the gate term list,
-    // its rotations and the permutation sets; inputs / shuffles reuse the start
+    // its rotations, the permutation sets, the permutation columns' cosets and
+    // the shuffle inputs' / tables' cosets; shuffle z inputs reuse the start
     void **pa = j.ptrs;
     void **pa_gate = pa + 2 * s.P + s.Sp, **pa_sets = pa_gate + 16 + 8;
-    void **pa_pv = pa_sets + s.Sp, **pa_ps = pa_pv + s.P;  // permutation columns' cosets
+    void **pa_pv = pa_sets + s.Sp, **pa_ps = pa_pv + s.P;
+    void **pa_sh = pa_ps + s.P;
     int *pa_rots = (int *)(pa_gate + 16);
     if (s.P > 0) {
         std::vector<void *> host(2 * s.P + s.Sp);
@@ -1201,12 +1203,20 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         PCHECK(logup_eval_h(j.h, j.aux0, pk.ones_ext, j.aux1, j.ext[E_M + l], j.ext[E_ZL + l], j.ext[E_ZL + l],
                             pk.l0_ext, pk.llast_ext, pk.lact_ext, y, rot1, N, st));
     }
-    // shuffle: (input + beta), (shuffle + beta)
-    for (int t = 0; t < s.S; t++) {
-        PCHECK(field_op(j.aux0, j.ext[E_IN + 2 * t], 0, nullptr, nullptr, 0, beta, N, 0, st));
-        PCHECK(field_op(j.aux1, j.ext[E_IN + 2 * t + 1], 0, nullptr, nullptr, 0, beta, N, 0, st));
-        PCHECK(zk_shuffle_eval_h_on(j.h, j.aux0, j.aux1, j.ext[E_ZS + t], pk.l0_ext, pk.llast_ext, pk.lact_ext, y,
-                                    rot1, N, st));
+    // shuffle: each group one plain input column and one plain table column,
+    // the case the reference hands to shuffle_eval_h_v2 with betas [beta, ...]
+    // (src/eval_h.rs:1095-1152); the kernel adds beta itself
+    if (s.S > 0) {
+        std::vector<void *> sh(2 * s.S);
+        for (int t = 0; t < s.S; t++) {
+            sh[t] = j.ext[E_IN + 2 * t];
+            sh[s.S + t] = j.ext[E_IN + 2 * t + 1];
+        }
+        PCHECK(hipMemcpyAsync(pa_sh, sh.data(), sh.size() * 8, hipMemcpyHostToDevice, st));
+        for (int t = 0; t < s.S; t++)
+            PCHECK(zk_shuffle_eval_h_v2_on(j.h, pa_sh + t, pa_sh + s.S + t, beta, 1, j.ext[E_ZS + t], pk.l0_ext,
+                                           pk.llast_ext, pk.lact_ext, y, rot1, N, st));
+        PCHECK(hipStreamSynchronize(st));  // the host pointer array
     }
     PCHECK(mark(4));
 
