@@ -390,7 +390,13 @@ hipError_t zk_msm_autotune(const void *points, const void *scalars, int n, int r
                            float *ms);
 
 /* Element-wise field / curve operations and calibration kernels (tests and
- * bench only). */
+ * bench only).  zk_fr_op / zk_fp_op: op 0 a+b, 1 a*b, 2 a-b, 3 a^-1 (0 -> 0),
+ * 4 a^2, 5 -a, 6 to Montgomery, 7 from Montgomery; b may be null (read as 0).
+ * zk_g1_op (a, b affine, out Jacobian): op 0 mixed XYZZ add, 1 Jacobian add,
+ * 2 XYZZ double of a, 3 Jacobian double of a, 4 XYZZ add, normalised; b is
+ * read by every op.  Out-of-range ops, null buffers and empty or malformed
+ * calibration launches (block outside 1..1024, threads not a multiple of it,
+ * iters < 0) return hipErrorInvalidValue. */
 hipError_t zk_fr_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
 hipError_t zk_fp_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);
 hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream);

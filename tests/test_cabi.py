@@ -133,6 +133,34 @@ def test_invalid_arguments_rejected_without_gpu(lib):
     assert lib.eval_logup_z_pure(None, None, None, None, 5, 4, None) != 0
 
 
+def test_helper_entry_points_reject_bad_arguments_without_gpu(lib):
+    """the element-wise helpers and calibration launches check buffers, op
+    codes and launch shapes on the host (a zero block would otherwise divide
+    by zero, a negative iteration count would run for 2^32 steps); the
+    non-null dummy address is never dereferenced on these paths"""
+    v, i = ctypes.c_void_p, ctypes.c_int
+    dummy = ctypes.c_void_p(ctypes.addressof(ctypes.create_string_buffer(64)))
+    for f, args in (("zk_fr_op", [None, dummy, None, 1, 4, None]), ("zk_fr_op", [dummy, dummy, None, 8, 4, None]),
+                    ("zk_fp_op", [dummy, None, None, 0, 4, None]), ("zk_fp_op", [dummy, dummy, None, -1, 4, None]),
+                    ("zk_fp29_op", [dummy, dummy, None, 5, 4, None]), ("zk_g1_op", [dummy, dummy, None, 0, 4, None]),
+                    ("zk_g1_op", [dummy, dummy, dummy, 5, 4, None]),
+                    ("zk_bench_mul", [dummy, 256, 0, 4, None]), ("zk_bench_mul", [dummy, 256, 96, 4, None]),
+                    ("zk_bench_mul", [dummy, 256, 2048, 4, None]), ("zk_bench_mul", [dummy, 256, 64, -1, None]),
+                    ("zk_bench_mul", [None, 256, 64, 4, None]), ("zk_bench_mul_cios", [dummy, 0, 64, 4, None]),
+                    ("zk_bench_dbl", [dummy, -64, 64, 4, None]), ("zk_bench_mul29", [dummy, 256, 0, 4, 0, None]),
+                    ("zk_bench_madd", [dummy, None, 4, 256, 64, 4, None]),
+                    ("zk_bench_madd", [dummy, dummy, 4, 256, 0, 4, None])):
+        fn = getattr(lib, f)
+        fn.restype = i
+        fn.argtypes = [i if isinstance(a, int) else v for a in args]
+        assert fn(*args) != 0, (f, args)
+    lib.zk_copy.restype = i
+    lib.zk_copy.argtypes = [v, v, ctypes.c_size_t, v]
+    assert lib.zk_copy(None, dummy, 64, None) != 0
+    assert lib.zk_copy(dummy, dummy, 60, None) != 0
+    assert lib.zk_copy(None, None, 0, None) == 0  # nothing to copy, nothing launched
+
+
 def test_tuning_tables_host_only(lib, tmp_path):
     """the per-size window / radix tables are host state: set, read back,
     reset, reject out-of-range values; autotune.save/load round-trips them"""

@@ -224,32 +224,40 @@ hipError_t zk_g1_mul_scalars(void *out, const void *scalars, int n, hipStream_t 
 
 hipError_t zk_fr_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
     if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    if (!out || !a || op < 0 || op > 7) return hipErrorInvalidValue;
     k_fe_binop<FrParams><<<(n + 255) / 256, 256, 0, stream>>>((Fr *)out, (const Fr *)a, (const Fr *)b, op, n);
     return hipGetLastError();
 }
 hipError_t zk_fp_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
     if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    if (!out || !a || op < 0 || op > 7) return hipErrorInvalidValue;
     k_fe_binop<FpParams><<<(n + 255) / 256, 256, 0, stream>>>((Fp *)out, (const Fp *)a, (const Fp *)b, op, n);
     return hipGetLastError();
 }
 hipError_t zk_fp29_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
     if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
-    if (!out || !a) return hipErrorInvalidValue;
+    if (!out || !a || op < 0 || op > 4) return hipErrorInvalidValue;
     k_fp29_op<<<(n + 255) / 256, 256, 0, stream>>>((Fp *)out, (const Fp *)a, (const Fp *)b, op, (uint32_t)n);
     return hipGetLastError();
 }
 hipError_t zk_g1_op(void *out, const void *a, const void *b, int op, int n, hipStream_t stream) {
     if (n <= 0) return n == 0 ? hipSuccess : hipErrorInvalidValue;
+    if (!out || !a || !b || op < 0 || op > 4) return hipErrorInvalidValue;  // every op reads b
     k_g1_op<<<(n + 63) / 64, 64, 0, stream>>>((G1Jac *)out, (const G1Aff *)a, (const G1Aff *)b, op, n);
     return hipGetLastError();
 }
+// a calibration launch: threads / block workgroups of block threads, each
+// thread running iters dependent steps on its own element of buf
+static bool bench_shape_ok(const void *buf, int threads, int block, int iters) {
+    return buf && threads > 0 && block > 0 && block <= 1024 && threads % block == 0 && iters >= 0;
+}
 hipError_t zk_bench_mul(void *buf, int threads, int block, int iters, hipStream_t stream) {
-    if (threads % block) return hipErrorInvalidValue;
+    if (!bench_shape_ok(buf, threads, block, iters)) return hipErrorInvalidValue;
     k_bench_mul<<<threads / block, block, 0, stream>>>((Fr *)buf, iters);
     return hipGetLastError();
 }
 hipError_t zk_bench_mul29(void *buf, int threads, int block, int iters, int variant, hipStream_t stream) {
-    if (threads % block) return hipErrorInvalidValue;
+    if (!bench_shape_ok(buf, threads, block, iters)) return hipErrorInvalidValue;
     if (variant)
         k_bench_mul29<1><<<threads / block, block, 0, stream>>>((Fp *)buf, iters);
     else
@@ -257,23 +265,24 @@ hipError_t zk_bench_mul29(void *buf, int threads, int block, int iters, int vari
     return hipGetLastError();
 }
 hipError_t zk_bench_mul_cios(void *buf, int threads, int block, int iters, hipStream_t stream) {
-    if (threads % block) return hipErrorInvalidValue;
+    if (!bench_shape_ok(buf, threads, block, iters)) return hipErrorInvalidValue;
     k_bench_mul_cios<<<threads / block, block, 0, stream>>>((Fr *)buf, iters);
     return hipGetLastError();
 }
 hipError_t zk_bench_dbl(void *buf, int threads, int block, int iters, hipStream_t stream) {
-    if (threads % block) return hipErrorInvalidValue;
+    if (!bench_shape_ok(buf, threads, block, iters)) return hipErrorInvalidValue;
     k_bench_dbl<<<threads / block, block, 0, stream>>>((G1Jac *)buf, iters);
     return hipGetLastError();
 }
 hipError_t zk_bench_madd(void *acc, const void *pts, int npts, int threads, int block, int iters,
                          hipStream_t stream) {
-    if (threads % block || npts <= 0) return hipErrorInvalidValue;
+    if (!bench_shape_ok(acc, threads, block, iters) || !pts || npts <= 0) return hipErrorInvalidValue;
     k_bench_madd<<<threads / block, block, 0, stream>>>((G1Xyzz *)acc, (const G1Aff *)pts, npts, iters);
     return hipGetLastError();
 }
 hipError_t zk_copy(void *dst, const void *src, size_t bytes, hipStream_t stream) {
-    if (bytes % 16) return hipErrorInvalidValue;
+    if (bytes % 16 || (bytes && (!dst || !src))) return hipErrorInvalidValue;
+    if (!bytes) return hipSuccess;
     size_t n16 = bytes / 16;
     k_copy<<<4096, 256, 0, stream>>>((uint4 *)dst, (const uint4 *)src, n16);
     return hipGetLastError();
