@@ -328,7 +328,9 @@ hipError_t zk_ntt_set_max_deg(int log_n, int max_deg);
 int zk_ntt_get_max_deg(int log_n);
 /* Autotuner: times each distinct radix plan (largest pass degree 4..8) for
  * 2^log_n on the given buffers (overwritten) and keeps the fastest; ms
- * (optional, 9 floats) gets the time per max degree. */
+ * (optional, 9 floats) gets the time per max degree.  The table is safe to
+ * read and set from any thread; an autotune is a set-up call: NTTs of its
+ * size run meanwhile on other threads stay correct but use its trial plans. */
 hipError_t zk_ntt_autotune(void *buf, void *tmp, const void *omegas, int log_n, int reps, hipStream_t stream,
                            int *best, float *ms);
 
@@ -372,14 +374,16 @@ hipError_t zk_graph_destroy(void *exec);
 
 /* MSM phase timing for the bench: HIP events on the launch stream around each
  * phase (count, scan, scatter, accumulate, fixup, reduce, finalize); reading
- * returns the number of profiled calls.  Profiling waits for each MSM. */
+ * returns the number of profiled calls.  Profiling waits for each MSM and
+ * shares one set of events: enable it only while one thread runs MSMs. */
 void zk_msm_profile_enable(int on);
 int zk_msm_profile_read(double *times, int ntimes, int reset);
 int zk_msm_default_window(int n);
 /* Window table per size class ceil(log2 n) (0 = model default), and the
  * autotuner that fills it: times zk_msm for windows around the model's choice
  * on the given device points / scalars and keeps the fastest for every later
- * MSM of that class (msm, zk_msm, zk_msm_batch).  ms: optional 18 floats. */
+ * MSM of that class (msm, zk_msm, zk_msm_batch).  ms: optional 18 floats.
+ * Thread-safety as for the NTT table above. */
 /* The window msm() picks with the reference driver's buffers (sizes of
  * src/cuda/msm.rs:284-311) for array_len points, 0 if they cannot hold its
  * scratch.  Host-only. */

@@ -33,6 +33,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <chrono>
 #include <thread>
 #include <vector>
@@ -198,15 +199,18 @@ bool place(MsmPlan &p, uint8_t *const *base, const size_t *bytes, int nreg) {
 }
 
 // Tuned windows per size class ceil(log2 n); 0 = not tuned.  Written by the
-// autotuner (zk_msm_autotune / zk_msm_set_window) before use.
-uint8_t g_window[32] = {0};
+// autotuner (zk_msm_autotune / zk_msm_set_window), read by every MSM: atomic,
+// since those calls may come from other threads than the MSMs (the window
+// changes only the speed of an MSM, never its result).
+std::atomic<uint8_t> g_window[32];
 
 // Window for a single MSM of n points: the tuned entry for its size class,
 // else the model log2(n) - 5 clamped to [6, 17] (bucket reduction ~ 2^c per
 // window against n additions per window).
 uint32_t default_window(uint32_t n) {
     uint32_t lg = log2_exact(n ? n : 1);
-    if (lg < 32 && g_window[lg]) return g_window[lg];
+    const uint8_t tuned = lg < 32 ? g_window[lg].load(std::memory_order_relaxed) : 0;
+    if (tuned) return tuned;
     int c = (int)lg - 5;
     if (c < 6) c = 6;
     if (c > 17) c = 17;
@@ -1382,7 +1386,7 @@ int zk_msm_plan_window(int array_len, int windows, int window_bits, int workers)
 // Window table access (a tuned table saved by the caller can be restored).
 hipError_t zk_msm_set_window(int log_n, int c) {
     if (log_n < 0 || log_n > 31 || c < 0 || c == 1 || c > 17) return hipErrorInvalidValue;
-    g_window[log_n] = (uint8_t)c;
+    g_window[log_n].store((uint8_t)c, std::memory_order_relaxed);
     return hipSuccess;
 }
 int zk_msm_get_window(int log_n) {
@@ -1399,8 +1403,8 @@ hipError_t zk_msm_autotune(const void *points, const void *scalars, int n, int r
     if (n <= 0 || !points || !scalars) return hipErrorInvalidValue;
     if (reps < 1) reps = 1;
     const uint32_t lg = log2_exact((uint32_t)n);
-    const uint8_t saved = lg < 32 ? g_window[lg] : 0;
-    if (lg < 32) g_window[lg] = 0;
+    const uint8_t saved = lg < 32 ? g_window[lg].load(std::memory_order_relaxed) : 0;
+    if (lg < 32) g_window[lg].store(0, std::memory_order_relaxed);
     const int c0 = (int)default_window((uint32_t)n);
     int lo = c0 - 2 < 4 ? 4 : c0 - 2, hi = c0 + 2 > 17 ? 17 : c0 + 2;
     size_t need = 0;
@@ -1434,10 +1438,10 @@ hipError_t zk_msm_autotune(const void *points, const void *scalars, int n, int r
     if (scratch) (void)hipFree(scratch);
     if (out) (void)hipFree(out);
     if (err != hipSuccess) {
-        if (lg < 32) g_window[lg] = saved;
+        if (lg < 32) g_window[lg].store(saved, std::memory_order_relaxed);
         return err;
     }
-    if (lg < 32) g_window[lg] = (uint8_t)best_c;
+    if (lg < 32) g_window[lg].store((uint8_t)best_c, std::memory_order_relaxed);
     if (best) *best = best_c;
     return hipSuccess;
 }

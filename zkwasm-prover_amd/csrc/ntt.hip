@@ -17,6 +17,7 @@
 // n^-1 scaling of the iNTT is fused into the last pass.
 #include <string.h>
 
+#include <atomic>
 #include <vector>
 
 #include "zkcore.h"
@@ -178,13 +179,15 @@ __global__ void k_coset_scale(Fr *s, const Fr *__restrict__ cp, uint32_t cn, uin
 namespace zk {
 
 // Largest pass degree per transform size; 0 = kMaxDeg.  Set by the
-// autotuner (zk_ntt_set_max_deg), read by every plan.
-uint8_t g_max_deg[31] = {0};
+// autotuner (zk_ntt_set_max_deg), read by every plan, possibly from other
+// threads: atomic (the plan changes only the speed, never the result).
+std::atomic<uint8_t> g_max_deg[31];
 
 // Radix plan: ceil(log_n / max_deg) passes, bits spread evenly (larger first).
 int ntt_plan(uint32_t log_n, uint32_t *degs) {
     if (log_n == 0) return 0;
-    uint32_t md = log_n < 31 && g_max_deg[log_n] ? g_max_deg[log_n] : kMaxDeg;
+    const uint8_t tuned = log_n < 31 ? g_max_deg[log_n].load(std::memory_order_relaxed) : 0;
+    const uint32_t md = tuned ? tuned : kMaxDeg;
     uint32_t passes = (log_n + md - 1) / md;
     uint32_t rem = log_n;
     for (uint32_t i = 0; i < passes; i++) {
@@ -426,12 +429,13 @@ hipError_t zk_coset_extend_batch(const zk_coset_item *items, int count, const vo
 // of 2^log_n elements; 0 restores the default (8).
 hipError_t zk_ntt_set_max_deg(int log_n, int max_deg) {
     if (log_n < 1 || log_n > 30 || max_deg < 0 || max_deg > (int)kMaxDeg) return hipErrorInvalidValue;
-    g_max_deg[log_n] = (uint8_t)max_deg;
+    g_max_deg[log_n].store((uint8_t)max_deg, std::memory_order_relaxed);
     return hipSuccess;
 }
 int zk_ntt_get_max_deg(int log_n) {
     if (log_n < 1 || log_n > 30) return 0;
-    return g_max_deg[log_n] ? g_max_deg[log_n] : (int)kMaxDeg;
+    const uint8_t tuned = g_max_deg[log_n].load(std::memory_order_relaxed);
+    return tuned ? tuned : (int)kMaxDeg;
 }
 
 // Autotuner: times the NTT of 2^log_n elements (buf / tmp device buffers,
@@ -442,7 +446,7 @@ hipError_t zk_ntt_autotune(void *buf, void *tmp, const void *omegas, int log_n, 
                            int *best, float *ms) {
     if (log_n < 1 || log_n > 30 || !buf || !tmp || !omegas) return hipErrorInvalidValue;
     if (reps < 1) reps = 1;
-    const uint8_t saved = g_max_deg[log_n];
+    const uint8_t saved = g_max_deg[log_n].load(std::memory_order_relaxed);
     hipEvent_t e0, e1;
     ZK_CHECK(hipEventCreate(&e0));
     hipError_t err = hipEventCreate(&e1);
@@ -456,7 +460,7 @@ hipError_t zk_ntt_autotune(void *buf, void *tmp, const void *omegas, int log_n, 
     int best_d = kMaxDeg;
     uint32_t prev_passes = 0;
     for (int d = (int)kMaxDeg; d >= 4 && err == hipSuccess; d--) {
-        g_max_deg[log_n] = (uint8_t)d;
+        g_max_deg[log_n].store((uint8_t)d, std::memory_order_relaxed);
         uint32_t degs[32];
         uint32_t passes = (uint32_t)ntt_plan((uint32_t)log_n, degs);
         if (passes == prev_passes) continue;  // same plan as a larger max degree
@@ -480,7 +484,7 @@ hipError_t zk_ntt_autotune(void *buf, void *tmp, const void *omegas, int log_n, 
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    g_max_deg[log_n] = err == hipSuccess ? (uint8_t)best_d : saved;
+    g_max_deg[log_n].store(err == hipSuccess ? (uint8_t)best_d : saved, std::memory_order_relaxed);
     if (err == hipSuccess && best) *best = best_d;
     return err;
 }
