@@ -27,6 +27,7 @@
 #include <mutex>
 #include <vector>
 
+#include "guard.h"
 #include "host_pool.h"
 #include "zkwasm_prover_kernel.h"
 
@@ -125,13 +126,17 @@ int take_stage(CopyQueue &cq, size_t bytes, hipError_t *err) {
 extern "C" {
 
 hipError_t zk_pinned_reserve(size_t slabs) {
-    return pool().reserve(slabs) ? hipSuccess : hipErrorOutOfMemory;
+    return zk::guard([&]() -> hipError_t {
+        return pool().reserve(slabs) ? hipSuccess : hipErrorOutOfMemory;
+    });
 }
 
 hipError_t zk_pinned_alloc(size_t bytes, void **out) {
-    if (!out || bytes == 0 || bytes > zkh::kSlabBytes) return hipErrorInvalidValue;
-    *out = pool().alloc(bytes);
-    return *out ? hipSuccess : hipErrorOutOfMemory;
+    return zk::guard([&]() -> hipError_t {
+        if (!out || bytes == 0 || bytes > zkh::kSlabBytes) return hipErrorInvalidValue;
+        *out = pool().alloc(bytes);
+        return *out ? hipSuccess : hipErrorOutOfMemory;
+    });
 }
 
 hipError_t zk_pinned_free(void *p) { return pool().release(p) ? hipSuccess : hipErrorInvalidValue; }
@@ -144,69 +149,73 @@ hipError_t zk_pinned_stats(size_t *slabs, size_t *chunks_in_use, size_t *slab_by
 }
 
 hipError_t zk_copyq_create(int max_inflight, void **out) {
-    if (!out || max_inflight < 1) return hipErrorInvalidValue;
-    auto *cq = new CopyQueue();
-    cq->max = max_inflight;
-    hipError_t err = hipStreamCreateWithFlags(&cq->cs, hipStreamNonBlocking);
-    if (err != hipSuccess) {
-        delete cq;
-        return err;
-    }
-    *out = cq;
-    return hipSuccess;
+    return zk::guard([&]() -> hipError_t {
+        if (!out || max_inflight < 1) return hipErrorInvalidValue;
+        auto *cq = new CopyQueue();
+        cq->max = max_inflight;
+        hipError_t err = hipStreamCreateWithFlags(&cq->cs, hipStreamNonBlocking);
+        if (err != hipSuccess) {
+            delete cq;
+            return err;
+        }
+        *out = cq;
+        return hipSuccess;
+    });
 }
 
 hipError_t zk_copyq_push(void *q, void *host_dst, const void *dev_src, size_t bytes, hipStream_t producer,
                          int flags) {
-    if (!q || !host_dst || !dev_src || bytes == 0) return hipErrorInvalidValue;
-    if ((flags & ~(ZK_COPYQ_SNAPSHOT | ZK_COPYQ_RELEASE)) ||
-        (flags & ZK_COPYQ_SNAPSHOT && flags & ZK_COPYQ_RELEASE))
-        return hipErrorInvalidValue;
-    CopyQueue &cq = *(CopyQueue *)q;
-    std::lock_guard<std::mutex> g(cq.mu);
-    hipError_t err = hipSuccess;
-    if ((int)cq.q.size() >= cq.max) {
-        auto t0 = std::chrono::steady_clock::now();
-        Entry e = cq.q.front();
-        cq.q.pop_front();
-        err = finish(cq, e);
-        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        cq.full_waits++;
-        if (ms > cq.max_wait_ms) cq.max_wait_ms = ms;
-        if (err != hipSuccess) return err;
-    }
-    Entry e;
-#define QCHECK(x)                                                                    \
-    do {                                                                             \
-        err = (x);                                                                   \
-        if (err != hipSuccess) {                                                     \
-            if (e.stage >= 0) cq.stages[e.stage].busy = false;                       \
-            for (hipEvent_t ev : {e.ready, e.snap, e.done})                          \
-                if (ev) (void)hipEventDestroy(ev);                                   \
-            return err;                                                              \
-        }                                                                            \
-    } while (0)
-    QCHECK(hipEventCreateWithFlags(&e.ready, hipEventDisableTiming));
-    QCHECK(hipEventCreateWithFlags(&e.done, hipEventDisableTiming));
-    QCHECK(hipEventRecord(e.ready, producer));
-    QCHECK(hipStreamWaitEvent(cq.cs, e.ready, 0));
-    const void *src = dev_src;
-    if (flags & ZK_COPYQ_SNAPSHOT) {
-        e.stage = take_stage(cq, bytes, &err);
-        if (e.stage < 0) QCHECK(err);
-        QCHECK(hipEventCreateWithFlags(&e.snap, hipEventDisableTiming));
-        QCHECK(hipMemcpyAsync(cq.stages[e.stage].p, dev_src, bytes, hipMemcpyDeviceToDevice, cq.cs));
-        QCHECK(hipEventRecord(e.snap, cq.cs));
-        QCHECK(hipStreamWaitEvent(producer, e.snap, 0));
-        src = cq.stages[e.stage].p;
-    }
-    QCHECK(hipMemcpyAsync(host_dst, src, bytes, hipMemcpyDeviceToHost, cq.cs));
-    QCHECK(hipEventRecord(e.done, cq.cs));
-#undef QCHECK
-    if (flags & ZK_COPYQ_RELEASE) e.release = (void *)dev_src;
-    cq.q.push_back(e);
-    cq.pushed++;
-    return hipSuccess;
+    return zk::guard([&]() -> hipError_t {
+        if (!q || !host_dst || !dev_src || bytes == 0) return hipErrorInvalidValue;
+        if ((flags & ~(ZK_COPYQ_SNAPSHOT | ZK_COPYQ_RELEASE)) ||
+            (flags & ZK_COPYQ_SNAPSHOT && flags & ZK_COPYQ_RELEASE))
+            return hipErrorInvalidValue;
+        CopyQueue &cq = *(CopyQueue *)q;
+        std::lock_guard<std::mutex> g(cq.mu);
+        hipError_t err = hipSuccess;
+        if ((int)cq.q.size() >= cq.max) {
+            auto t0 = std::chrono::steady_clock::now();
+            Entry e = cq.q.front();
+            cq.q.pop_front();
+            err = finish(cq, e);
+            double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            cq.full_waits++;
+            if (ms > cq.max_wait_ms) cq.max_wait_ms = ms;
+            if (err != hipSuccess) return err;
+        }
+        Entry e;
+    #define QCHECK(x)                                                                    \
+        do {                                                                             \
+            err = (x);                                                                   \
+            if (err != hipSuccess) {                                                     \
+                if (e.stage >= 0) cq.stages[e.stage].busy = false;                       \
+                for (hipEvent_t ev : {e.ready, e.snap, e.done})                          \
+                    if (ev) (void)hipEventDestroy(ev);                                   \
+                return err;                                                              \
+            }                                                                            \
+        } while (0)
+        QCHECK(hipEventCreateWithFlags(&e.ready, hipEventDisableTiming));
+        QCHECK(hipEventCreateWithFlags(&e.done, hipEventDisableTiming));
+        QCHECK(hipEventRecord(e.ready, producer));
+        QCHECK(hipStreamWaitEvent(cq.cs, e.ready, 0));
+        const void *src = dev_src;
+        if (flags & ZK_COPYQ_SNAPSHOT) {
+            e.stage = take_stage(cq, bytes, &err);
+            if (e.stage < 0) QCHECK(err);
+            QCHECK(hipEventCreateWithFlags(&e.snap, hipEventDisableTiming));
+            QCHECK(hipMemcpyAsync(cq.stages[e.stage].p, dev_src, bytes, hipMemcpyDeviceToDevice, cq.cs));
+            QCHECK(hipEventRecord(e.snap, cq.cs));
+            QCHECK(hipStreamWaitEvent(producer, e.snap, 0));
+            src = cq.stages[e.stage].p;
+        }
+        QCHECK(hipMemcpyAsync(host_dst, src, bytes, hipMemcpyDeviceToHost, cq.cs));
+        QCHECK(hipEventRecord(e.done, cq.cs));
+    #undef QCHECK
+        if (flags & ZK_COPYQ_RELEASE) e.release = (void *)dev_src;
+        cq.q.push_back(e);
+        cq.pushed++;
+        return hipSuccess;
+    });
 }
 
 hipError_t zk_copyq_sync_all(void *q) {

@@ -38,6 +38,7 @@
 #include <thread>
 #include <vector>
 
+#include "guard.h"
 #include "ff29.h"
 #include "host_ec.h"
 #include "zkcore.h"
@@ -1241,24 +1242,26 @@ hipError_t msm(void *res, const void *points, const void *scalars, void *indices
                void *indices_buf2, void *indices_buf3, void *sort_buf, void *acc_indices_buf, int array_len,
                int windows, int window_bits, int threads, int workers,
                int prepared_sort_indices_temp_storage_bytes, int skip_zero, hipStream_t stream) {
-    (void)acc_indices_buf;
-    (void)threads;
-    (void)prepared_sort_indices_temp_storage_bytes;
-    (void)skip_zero;  // zero digits never reach a bucket here; the result is the same
-    if (array_len <= 0 || windows <= 0 || window_bits <= 0 || window_bits > 30 || workers <= 0)
-        return hipErrorInvalidValue;
-    size_t idx_bytes = (size_t)array_len * (size_t)windows * 4;
-    size_t slice_bytes = (((size_t)windows << window_bits) + 2 * (size_t)workers) * sizeof(G1Jac);
-    // the curve slice beyond the first kSliceReserve bytes (header, masked and
-    // window sums) is free scratch as well: at least 2^10 * 26 Jacobian points
-    uint8_t *slice_free = slice_bytes > kSliceReserve ? (uint8_t *)res + kSliceReserve : nullptr;
-    uint8_t *regions[6] = {(uint8_t *)sort_buf, (uint8_t *)indices_buf0, (uint8_t *)indices_buf1,
-                           (uint8_t *)indices_buf2, (uint8_t *)indices_buf3, slice_free};
-    size_t bytes[6] = {3 * idx_bytes, idx_bytes, idx_bytes, idx_bytes, idx_bytes,
-                       slice_free ? slice_bytes - kSliceReserve : 0};
-    MsmJob job{(const G1Aff *)points, (const Fr *)scalars, (uint32_t)array_len, (uint8_t *)res};
-    return msm_run(&job, 1, slice_bytes < kSliceReserve ? slice_bytes : kSliceReserve, 0, regions, bytes, 6,
-                   stream, nullptr);
+    return zk::guard([&]() -> hipError_t {
+        (void)acc_indices_buf;
+        (void)threads;
+        (void)prepared_sort_indices_temp_storage_bytes;
+        (void)skip_zero;  // zero digits never reach a bucket here; the result is the same
+        if (array_len <= 0 || windows <= 0 || window_bits <= 0 || window_bits > 30 || workers <= 0)
+            return hipErrorInvalidValue;
+        size_t idx_bytes = (size_t)array_len * (size_t)windows * 4;
+        size_t slice_bytes = (((size_t)windows << window_bits) + 2 * (size_t)workers) * sizeof(G1Jac);
+        // the curve slice beyond the first kSliceReserve bytes (header, masked and
+        // window sums) is free scratch as well: at least 2^10 * 26 Jacobian points
+        uint8_t *slice_free = slice_bytes > kSliceReserve ? (uint8_t *)res + kSliceReserve : nullptr;
+        uint8_t *regions[6] = {(uint8_t *)sort_buf, (uint8_t *)indices_buf0, (uint8_t *)indices_buf1,
+                               (uint8_t *)indices_buf2, (uint8_t *)indices_buf3, slice_free};
+        size_t bytes[6] = {3 * idx_bytes, idx_bytes, idx_bytes, idx_bytes, idx_bytes,
+                           slice_free ? slice_bytes - kSliceReserve : 0};
+        MsmJob job{(const G1Aff *)points, (const Fr *)scalars, (uint32_t)array_len, (uint8_t *)res};
+        return msm_run(&job, 1, slice_bytes < kSliceReserve ? slice_bytes : kSliceReserve, 0, regions, bytes, 6,
+                       stream, nullptr);
+    });
 }
 
 // cuda/msm.cu:434-467 (caller src/cuda/msm.rs:369-455).  buckets is a device
@@ -1268,15 +1271,17 @@ hipError_t msm(void *res, const void *points, const void *scalars, void *indices
 hipError_t batch_msm_collect(void *remain_indices, void *remain_acc, void *next_remain_indices,
                              void *next_remain_acc, void *buckets, unsigned workers, unsigned windows,
                              unsigned window_bits, unsigned batch_size, hipStream_t stream) {
-    (void)remain_indices;
-    (void)remain_acc;
-    (void)next_remain_indices;
-    (void)next_remain_acc;
-    (void)workers;
-    (void)windows;
-    (void)window_bits;
-    if (!buckets && batch_size) return hipErrorInvalidValue;
-    return msm_collect((uint8_t *const *)buckets, batch_size, stream);
+    return zk::guard([&]() -> hipError_t {
+        (void)remain_indices;
+        (void)remain_acc;
+        (void)next_remain_indices;
+        (void)next_remain_acc;
+        (void)workers;
+        (void)windows;
+        (void)window_bits;
+        if (!buckets && batch_size) return hipErrorInvalidValue;
+        return msm_collect((uint8_t *const *)buckets, batch_size, stream);
+    });
 }
 
 // Phase timing (bench instrumentation): enable / read / reset.  times[k]
@@ -1319,14 +1324,16 @@ static hipError_t finish_batch(uint8_t *const *slices, uint8_t **slots_dev, uint
 // the default window for n.  Returns after the result is written.
 hipError_t zk_msm(void *out, const void *points, const void *scalars, int n, int c, void *scratch,
                   size_t scratch_bytes, hipStream_t stream) {
-    if (n <= 0 || !out || !scratch) return hipErrorInvalidValue;
-    uint8_t *regions[1] = {(uint8_t *)scratch};
-    size_t bytes[1] = {scratch_bytes};
-    uint8_t *slice = (uint8_t *)out;
-    MsmJob job{(const G1Aff *)points, (const Fr *)scalars, (uint32_t)n, slice};
-    uint8_t **slots = nullptr;
-    ZK_CHECK(msm_run(&job, 1, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1, stream, &slots));
-    return finish_batch(&slice, slots, 1, stream);
+    return zk::guard([&]() -> hipError_t {
+        if (n <= 0 || !out || !scratch) return hipErrorInvalidValue;
+        uint8_t *regions[1] = {(uint8_t *)scratch};
+        size_t bytes[1] = {scratch_bytes};
+        uint8_t *slice = (uint8_t *)out;
+        MsmJob job{(const G1Aff *)points, (const Fr *)scalars, (uint32_t)n, slice};
+        uint8_t **slots = nullptr;
+        ZK_CHECK(msm_run(&job, 1, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1, stream, &slots));
+        return finish_batch(&slice, slots, 1, stream);
+    });
 }
 
 // Scratch bytes zk_msm_batch needs for these items at window c (0: default).
@@ -1347,21 +1354,23 @@ size_t zk_msm_batch_scratch_bytes(const zk_msm_item *items, int count, int c) {
 // are written.
 hipError_t zk_msm_batch(void *out, const zk_msm_item *items, int count, int c, void *scratch,
                         size_t scratch_bytes, hipStream_t stream) {
-    if (count <= 0 || !items || !out || !scratch) return hipErrorInvalidValue;
-    std::vector<MsmJob> jobs(count);
-    std::vector<uint8_t *> slices(count);
-    for (int m = 0; m < count; m++) {
-        if (items[m].n <= 0) return hipErrorInvalidValue;
-        slices[m] = (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES;
-        jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n,
-                         slices[m]};
-    }
-    uint8_t *regions[1] = {(uint8_t *)scratch};
-    size_t bytes[1] = {scratch_bytes};
-    uint8_t **slots = nullptr;
-    ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1,
-                     stream, &slots));
-    return finish_batch(slices.data(), slots, (uint32_t)count, stream);
+    return zk::guard([&]() -> hipError_t {
+        if (count <= 0 || !items || !out || !scratch) return hipErrorInvalidValue;
+        std::vector<MsmJob> jobs(count);
+        std::vector<uint8_t *> slices(count);
+        for (int m = 0; m < count; m++) {
+            if (items[m].n <= 0) return hipErrorInvalidValue;
+            slices[m] = (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES;
+            jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n,
+                             slices[m]};
+        }
+        uint8_t *regions[1] = {(uint8_t *)scratch};
+        size_t bytes[1] = {scratch_bytes};
+        uint8_t **slots = nullptr;
+        ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1,
+                         stream, &slots));
+        return finish_batch(slices.data(), slots, (uint32_t)count, stream);
+    });
 }
 
 // The window msm() would use with the reference driver's buffers for
@@ -1400,50 +1409,52 @@ int zk_msm_get_window(int log_n) {
 // per window (0 for windows not tried).  Allocates its own scratch.
 hipError_t zk_msm_autotune(const void *points, const void *scalars, int n, int reps, hipStream_t stream, int *best,
                            float *ms) {
-    if (n <= 0 || !points || !scalars) return hipErrorInvalidValue;
-    if (reps < 1) reps = 1;
-    const uint32_t lg = log2_exact((uint32_t)n);
-    const uint8_t saved = lg < 32 ? g_window[lg].load(std::memory_order_relaxed) : 0;
-    if (lg < 32) g_window[lg].store(0, std::memory_order_relaxed);
-    const int c0 = (int)default_window((uint32_t)n);
-    int lo = c0 - 2 < 4 ? 4 : c0 - 2, hi = c0 + 2 > 17 ? 17 : c0 + 2;
-    size_t need = 0;
-    for (int c = lo; c <= hi; c++) {
-        size_t b = zk_msm_scratch_bytes(n, c);
-        if (b > need) need = b;
-    }
-    void *scratch = nullptr, *out = nullptr;
-    hipError_t err = hipMalloc(&scratch, need);
-    if (err == hipSuccess) err = hipMalloc(&out, ZK_MSM_OUT_BYTES);
-    float best_ms = 0;
-    int best_c = c0;
-    if (ms)
-        for (int c = 0; c < 18; c++) ms[c] = 0;
-    for (int c = lo; c <= hi && err == hipSuccess; c++) {
-        err = zk_msm(out, points, scalars, n, c, scratch, need, stream);  // warm-up
-        double t = 1e30;
-        for (int r = 0; r < reps && err == hipSuccess; r++) {
-            auto t0 = std::chrono::steady_clock::now();
-            err = zk_msm(out, points, scalars, n, c, scratch, need, stream);
-            double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            if (dt < t) t = dt;
+    return zk::guard([&]() -> hipError_t {
+        if (n <= 0 || !points || !scalars) return hipErrorInvalidValue;
+        if (reps < 1) reps = 1;
+        const uint32_t lg = log2_exact((uint32_t)n);
+        const uint8_t saved = lg < 32 ? g_window[lg].load(std::memory_order_relaxed) : 0;
+        if (lg < 32) g_window[lg].store(0, std::memory_order_relaxed);
+        const int c0 = (int)default_window((uint32_t)n);
+        int lo = c0 - 2 < 4 ? 4 : c0 - 2, hi = c0 + 2 > 17 ? 17 : c0 + 2;
+        size_t need = 0;
+        for (int c = lo; c <= hi; c++) {
+            size_t b = zk_msm_scratch_bytes(n, c);
+            if (b > need) need = b;
         }
-        if (err != hipSuccess) break;
-        if (ms) ms[c] = (float)t;
-        if (best_ms == 0 || t < best_ms) {
-            best_ms = (float)t;
-            best_c = c;
+        void *scratch = nullptr, *out = nullptr;
+        hipError_t err = hipMalloc(&scratch, need);
+        if (err == hipSuccess) err = hipMalloc(&out, ZK_MSM_OUT_BYTES);
+        float best_ms = 0;
+        int best_c = c0;
+        if (ms)
+            for (int c = 0; c < 18; c++) ms[c] = 0;
+        for (int c = lo; c <= hi && err == hipSuccess; c++) {
+            err = zk_msm(out, points, scalars, n, c, scratch, need, stream);  // warm-up
+            double t = 1e30;
+            for (int r = 0; r < reps && err == hipSuccess; r++) {
+                auto t0 = std::chrono::steady_clock::now();
+                err = zk_msm(out, points, scalars, n, c, scratch, need, stream);
+                double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (dt < t) t = dt;
+            }
+            if (err != hipSuccess) break;
+            if (ms) ms[c] = (float)t;
+            if (best_ms == 0 || t < best_ms) {
+                best_ms = (float)t;
+                best_c = c;
+            }
         }
-    }
-    if (scratch) (void)hipFree(scratch);
-    if (out) (void)hipFree(out);
-    if (err != hipSuccess) {
-        if (lg < 32) g_window[lg].store(saved, std::memory_order_relaxed);
-        return err;
-    }
-    if (lg < 32) g_window[lg].store((uint8_t)best_c, std::memory_order_relaxed);
-    if (best) *best = best_c;
-    return hipSuccess;
+        if (scratch) (void)hipFree(scratch);
+        if (out) (void)hipFree(out);
+        if (err != hipSuccess) {
+            if (lg < 32) g_window[lg].store(saved, std::memory_order_relaxed);
+            return err;
+        }
+        if (lg < 32) g_window[lg].store((uint8_t)best_c, std::memory_order_relaxed);
+        if (best) *best = best_c;
+        return hipSuccess;
+    });
 }
 
 // zk_msm_batch without any host wait: the window combination runs on the
@@ -1452,19 +1463,21 @@ hipError_t zk_msm_autotune(const void *points, const void *scalars, int n, int r
 // the stream reaches them.  items is read during the call only.
 hipError_t zk_msm_batch_async(void *out, const zk_msm_item *items, int count, int c, void *scratch,
                               size_t scratch_bytes, hipStream_t stream) {
-    if (count <= 0 || !items || !out || !scratch) return hipErrorInvalidValue;
-    std::vector<MsmJob> jobs(count);
-    for (int m = 0; m < count; m++) {
-        if (items[m].n <= 0) return hipErrorInvalidValue;
-        jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n,
-                         (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES};
-    }
-    uint8_t *regions[1] = {(uint8_t *)scratch};
-    size_t bytes[1] = {scratch_bytes};
-    uint8_t **slots = nullptr;
-    ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1,
-                     stream, &slots));
-    return device_collect(slots, (uint32_t)count, stream);
+    return zk::guard([&]() -> hipError_t {
+        if (count <= 0 || !items || !out || !scratch) return hipErrorInvalidValue;
+        std::vector<MsmJob> jobs(count);
+        for (int m = 0; m < count; m++) {
+            if (items[m].n <= 0) return hipErrorInvalidValue;
+            jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n,
+                             (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES};
+        }
+        uint8_t *regions[1] = {(uint8_t *)scratch};
+        size_t bytes[1] = {scratch_bytes};
+        uint8_t **slots = nullptr;
+        ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1,
+                         stream, &slots));
+        return device_collect(slots, (uint32_t)count, stream);
+    });
 }
 
 }  // extern "C"

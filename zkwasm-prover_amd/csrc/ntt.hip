@@ -20,6 +20,7 @@
 #include <atomic>
 #include <vector>
 
+#include "guard.h"
 #include "zkcore.h"
 #include "zkinternal.h"
 
@@ -326,15 +327,17 @@ extern "C" {
 // in-pass twiddles from the w^i table that expand_omega_buffer produced.
 hipError_t ntt(void *buf, void *tmp, const void *pq, const void *omegas, const void *n_inv,
                int log_n, int max_deg, bool *swap, hipStream_t stream) {
-    (void)pq;
-    (void)max_deg;
-    if (log_n < 0 || log_n > 30 || !buf || !omegas) return hipErrorInvalidValue;
-    if (log_n > 0 && !tmp) return hipErrorInvalidValue;
-    int passes = 0;
-    ZK_CHECK(ntt_run((Fr *)buf, (Fr *)tmp, (const Fr *)omegas, (const Fr *)n_inv, (uint32_t)log_n,
-                     stream, &passes));
-    if (swap) *swap = (passes & 1) != 0;
-    return hipSuccess;
+    return zk::guard([&]() -> hipError_t {
+        (void)pq;
+        (void)max_deg;
+        if (log_n < 0 || log_n > 30 || !buf || !omegas) return hipErrorInvalidValue;
+        if (log_n > 0 && !tmp) return hipErrorInvalidValue;
+        int passes = 0;
+        ZK_CHECK(ntt_run((Fr *)buf, (Fr *)tmp, (const Fr *)omegas, (const Fr *)n_inv, (uint32_t)log_n,
+                         stream, &passes));
+        if (swap) *swap = (passes & 1) != 0;
+        return hipSuccess;
+    });
 }
 
 // cuda/bn254.cu:1333-1355.  buf[0] = 1 and buf[1] = w are set by the caller
@@ -378,19 +381,21 @@ hipError_t extended_prepare(void *s, const void *coset_powers, unsigned coset_po
 // items is a host array; swapped[m] (host) is set when result m is in tmp.
 // Asynchronous on stream, like `ntt`.
 hipError_t zk_ntt_batch(const zk_ntt_item *items, int count, unsigned char *swapped, hipStream_t stream) {
-    if (count <= 0 || !items || !swapped) return hipErrorInvalidValue;
-    std::vector<NttSpec> specs(count);
-    for (int m = 0; m < count; m++) {
-        const zk_ntt_item &it = items[m];
-        if (it.log_n < 0 || it.log_n > 30 || !it.buf || !it.omegas || (it.log_n > 0 && !it.tmp))
-            return hipErrorInvalidValue;
-        specs[m] = NttSpec{(Fr *)it.buf, (Fr *)it.tmp, (const Fr *)it.omegas, (const Fr *)it.n_inv,
-                           (uint32_t)it.log_n};
-    }
-    std::vector<int> passes(count);
-    ZK_CHECK(ntt_run_batch(specs.data(), (uint32_t)count, stream, passes.data()));
-    for (int m = 0; m < count; m++) swapped[m] = (passes[m] & 1) != 0;
-    return hipSuccess;
+    return zk::guard([&]() -> hipError_t {
+        if (count <= 0 || !items || !swapped) return hipErrorInvalidValue;
+        std::vector<NttSpec> specs(count);
+        for (int m = 0; m < count; m++) {
+            const zk_ntt_item &it = items[m];
+            if (it.log_n < 0 || it.log_n > 30 || !it.buf || !it.omegas || (it.log_n > 0 && !it.tmp))
+                return hipErrorInvalidValue;
+            specs[m] = NttSpec{(Fr *)it.buf, (Fr *)it.tmp, (const Fr *)it.omegas, (const Fr *)it.n_inv,
+                               (uint32_t)it.log_n};
+        }
+        std::vector<int> passes(count);
+        ZK_CHECK(ntt_run_batch(specs.data(), (uint32_t)count, stream, passes.data()));
+        for (int m = 0; m < count; m++) swapped[m] = (passes[m] & 1) != 0;
+        return hipSuccess;
+    });
 }
 
 // The coset extension in one pipeline: for each item, the 2^log_n
@@ -404,25 +409,27 @@ hipError_t zk_ntt_batch(const zk_ntt_item *items, int count, unsigned char *swap
 // when swapped[m] is set; src may not alias buf or tmp.
 hipError_t zk_coset_extend_batch(const zk_coset_item *items, int count, const void *coset_powers,
                                  int coset_powers_n, unsigned char *swapped, hipStream_t stream) {
-    if (count <= 0 || !items || !swapped || coset_powers_n < 0 || (coset_powers_n > 1 && !coset_powers))
-        return hipErrorInvalidValue;
-    std::vector<NttSpec> specs(count);
-    for (int m = 0; m < count; m++) {
-        const zk_coset_item &it = items[m];
-        if (it.log_n < 0 || it.log_ext < 0 || it.log_n + it.log_ext < 1 || it.log_n + it.log_ext > 30 || !it.src ||
-            !it.buf || !it.tmp || !it.omegas || it.src == it.buf || it.src == it.tmp)
+    return zk::guard([&]() -> hipError_t {
+        if (count <= 0 || !items || !swapped || coset_powers_n < 0 || (coset_powers_n > 1 && !coset_powers))
             return hipErrorInvalidValue;
-        NttSpec &sp = specs[m];
-        sp = NttSpec{(Fr *)it.buf, (Fr *)it.tmp, (const Fr *)it.omegas, nullptr, (uint32_t)(it.log_n + it.log_ext)};
-        sp.src = (const Fr *)it.src;
-        sp.nz = 1u << it.log_n;
-        sp.coset = (const Fr *)coset_powers;
-        sp.cn = coset_powers_n > 1 ? (uint32_t)coset_powers_n : 0u;
-    }
-    std::vector<int> passes(count);
-    ZK_CHECK(ntt_run_batch(specs.data(), (uint32_t)count, stream, passes.data()));
-    for (int m = 0; m < count; m++) swapped[m] = (passes[m] & 1) != 0;
-    return hipSuccess;
+        std::vector<NttSpec> specs(count);
+        for (int m = 0; m < count; m++) {
+            const zk_coset_item &it = items[m];
+            if (it.log_n < 0 || it.log_ext < 0 || it.log_n + it.log_ext < 1 || it.log_n + it.log_ext > 30 || !it.src ||
+                !it.buf || !it.tmp || !it.omegas || it.src == it.buf || it.src == it.tmp)
+                return hipErrorInvalidValue;
+            NttSpec &sp = specs[m];
+            sp = NttSpec{(Fr *)it.buf, (Fr *)it.tmp, (const Fr *)it.omegas, nullptr, (uint32_t)(it.log_n + it.log_ext)};
+            sp.src = (const Fr *)it.src;
+            sp.nz = 1u << it.log_n;
+            sp.coset = (const Fr *)coset_powers;
+            sp.cn = coset_powers_n > 1 ? (uint32_t)coset_powers_n : 0u;
+        }
+        std::vector<int> passes(count);
+        ZK_CHECK(ntt_run_batch(specs.data(), (uint32_t)count, stream, passes.data()));
+        for (int m = 0; m < count; m++) swapped[m] = (passes[m] & 1) != 0;
+        return hipSuccess;
+    });
 }
 
 // Autotuner hook: the largest pass degree (radix 2^max_deg) for transforms
@@ -444,49 +451,51 @@ int zk_ntt_get_max_deg(int log_n) {
 // the time per max degree.  The data in buf / tmp is overwritten.
 hipError_t zk_ntt_autotune(void *buf, void *tmp, const void *omegas, int log_n, int reps, hipStream_t stream,
                            int *best, float *ms) {
-    if (log_n < 1 || log_n > 30 || !buf || !tmp || !omegas) return hipErrorInvalidValue;
-    if (reps < 1) reps = 1;
-    const uint8_t saved = g_max_deg[log_n].load(std::memory_order_relaxed);
-    hipEvent_t e0, e1;
-    ZK_CHECK(hipEventCreate(&e0));
-    hipError_t err = hipEventCreate(&e1);
-    if (err != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        return err;
-    }
-    if (ms)
-        for (int d = 0; d <= (int)kMaxDeg; d++) ms[d] = 0;
-    float best_ms = 0;
-    int best_d = kMaxDeg;
-    uint32_t prev_passes = 0;
-    for (int d = (int)kMaxDeg; d >= 4 && err == hipSuccess; d--) {
-        g_max_deg[log_n].store((uint8_t)d, std::memory_order_relaxed);
-        uint32_t degs[32];
-        uint32_t passes = (uint32_t)ntt_plan((uint32_t)log_n, degs);
-        if (passes == prev_passes) continue;  // same plan as a larger max degree
-        prev_passes = passes;
-        int np = 0;
-        err = ntt_run((Fr *)buf, (Fr *)tmp, (const Fr *)omegas, nullptr, (uint32_t)log_n, stream, &np);
-        if (err == hipSuccess) err = hipEventRecord(e0, stream);
-        for (int r = 0; r < reps && err == hipSuccess; r++)
-            err = ntt_run((Fr *)buf, (Fr *)tmp, (const Fr *)omegas, nullptr, (uint32_t)log_n, stream, &np);
-        if (err == hipSuccess) err = hipEventRecord(e1, stream);
-        if (err == hipSuccess) err = hipEventSynchronize(e1);
-        float t = 0;
-        if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
-        if (err != hipSuccess) break;
-        t /= reps;
-        if (ms) ms[d] = t;
-        if (best_ms == 0 || t < best_ms) {
-            best_ms = t;
-            best_d = d;
+    return zk::guard([&]() -> hipError_t {
+        if (log_n < 1 || log_n > 30 || !buf || !tmp || !omegas) return hipErrorInvalidValue;
+        if (reps < 1) reps = 1;
+        const uint8_t saved = g_max_deg[log_n].load(std::memory_order_relaxed);
+        hipEvent_t e0, e1;
+        ZK_CHECK(hipEventCreate(&e0));
+        hipError_t err = hipEventCreate(&e1);
+        if (err != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return err;
         }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    g_max_deg[log_n].store(err == hipSuccess ? (uint8_t)best_d : saved, std::memory_order_relaxed);
-    if (err == hipSuccess && best) *best = best_d;
-    return err;
+        if (ms)
+            for (int d = 0; d <= (int)kMaxDeg; d++) ms[d] = 0;
+        float best_ms = 0;
+        int best_d = kMaxDeg;
+        uint32_t prev_passes = 0;
+        for (int d = (int)kMaxDeg; d >= 4 && err == hipSuccess; d--) {
+            g_max_deg[log_n].store((uint8_t)d, std::memory_order_relaxed);
+            uint32_t degs[32];
+            uint32_t passes = (uint32_t)ntt_plan((uint32_t)log_n, degs);
+            if (passes == prev_passes) continue;  // same plan as a larger max degree
+            prev_passes = passes;
+            int np = 0;
+            err = ntt_run((Fr *)buf, (Fr *)tmp, (const Fr *)omegas, nullptr, (uint32_t)log_n, stream, &np);
+            if (err == hipSuccess) err = hipEventRecord(e0, stream);
+            for (int r = 0; r < reps && err == hipSuccess; r++)
+                err = ntt_run((Fr *)buf, (Fr *)tmp, (const Fr *)omegas, nullptr, (uint32_t)log_n, stream, &np);
+            if (err == hipSuccess) err = hipEventRecord(e1, stream);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float t = 0;
+            if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+            if (err != hipSuccess) break;
+            t /= reps;
+            if (ms) ms[d] = t;
+            if (best_ms == 0 || t < best_ms) {
+                best_ms = t;
+                best_d = d;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        g_max_deg[log_n].store(err == hipSuccess ? (uint8_t)best_d : saved, std::memory_order_relaxed);
+        if (err == hipSuccess && best) *best = best_d;
+        return err;
+    });
 }
 
 }  // extern "C"

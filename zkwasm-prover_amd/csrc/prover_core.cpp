@@ -40,6 +40,7 @@
 #include <chrono>
 #include <vector>
 
+#include "guard.h"
 #include "host_ec.h"
 #include "host_fr.h"
 #include "zkinternal.h"
@@ -884,31 +885,35 @@ size_t zkp_job_bytes(const zkp_circuit *c) {
 
 // Builds the shape-level data once (stays resident for every proof of it).
 hipError_t zkp_pk_create(const zkp_circuit *c, unsigned long long seed, hipStream_t stream, void **pk_out) {
-    if (!pk_out) return hipErrorInvalidValue;
-    Pk *pk = new Pk();
-    if (!make_shape(c, pk->sh)) {
-        delete pk;
-        return hipErrorInvalidValue;
-    }
-    Arena dry;
-    pk_layout(*pk, dry);
-    pk->bytes = dry.off + 256;
-    hipError_t e = hipMalloc((void **)&pk->mem, pk->bytes);
-    if (e != hipSuccess) {
-        delete pk;
-        return e;
-    }
-    Arena a;
-    a.base = pk->mem;
-    pk_layout(*pk, a);
-    e = pk_build(*pk, seed, stream);
-    if (e != hipSuccess) {
-        (void)hipFree(pk->mem);
-        delete pk;
-        return e;
-    }
-    *pk_out = pk;
-    return hipSuccess;
+    return zk::guard([&]() -> hipError_t {
+        if (!pk_out) return hipErrorInvalidValue;
+        Pk *pk = new Pk();
+        if (!make_shape(c, pk->sh)) {
+            delete pk;
+            return hipErrorInvalidValue;
+        }
+        Arena dry;
+        pk_layout(*pk, dry);
+        pk->bytes = dry.off + 256;
+        hipError_t e = hipMalloc((void **)&pk->mem, pk->bytes);
+        if (e != hipSuccess) {
+            delete pk;
+            return e;
+        }
+        Arena a;
+        a.base = pk->mem;
+        e = zk::guard([&] {  // layout and build free below if they throw
+            pk_layout(*pk, a);
+            return pk_build(*pk, seed, stream);
+        });
+        if (e != hipSuccess) {
+            (void)hipFree(pk->mem);
+            delete pk;
+            return e;
+        }
+        *pk_out = pk;
+        return hipSuccess;
+    });
 }
 
 hipError_t zkp_pk_srs_secret(void *pk, void *tau_out) {
@@ -956,22 +961,32 @@ int zkp_commitment_count(const zkp_circuit *c) {
 }
 
 hipError_t zkp_job_create(void *pk, void **job_out) {
-    if (!pk || !job_out) return hipErrorInvalidValue;
-    Job *j = new Job();
-    j->pk = (const Pk *)pk;
-    Arena dry;
-    job_layout(*j, dry);
-    j->bytes = dry.off + 256;
-    hipError_t e = hipMalloc((void **)&j->mem, j->bytes);
-    if (e != hipSuccess) {
-        delete j;
-        return e;
-    }
-    Arena a;
-    a.base = j->mem;
-    job_layout(*j, a);
-    *job_out = j;
-    return hipSuccess;
+    return zk::guard([&]() -> hipError_t {
+        if (!pk || !job_out) return hipErrorInvalidValue;
+        Job *j = new Job();
+        j->pk = (const Pk *)pk;
+        Arena dry;
+        job_layout(*j, dry);
+        j->bytes = dry.off + 256;
+        hipError_t e = hipMalloc((void **)&j->mem, j->bytes);
+        if (e != hipSuccess) {
+            delete j;
+            return e;
+        }
+        Arena a;
+        a.base = j->mem;
+        e = zk::guard([&] {
+            job_layout(*j, a);
+            return hipSuccess;
+        });
+        if (e != hipSuccess) {
+            (void)hipFree(j->mem);
+            delete j;
+            return e;
+        }
+        *job_out = j;
+        return hipSuccess;
+    });
 }
 
 hipError_t zkp_job_destroy(void *job) {
@@ -987,291 +1002,293 @@ hipError_t zkp_job_destroy(void *job) {
 // phase order; h_pieces (host, n Fr, may be null) the folded quotient.
 hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commitments, void *h_pieces,
                      zkp_proof_info *info, hipStream_t stream) {
-    if (!job || !commitments) return hipErrorInvalidValue;
-    const int batched = flags & ZKP_BATCHED;
-    Job &j = *(Job *)job;
-    const Pk &pk = *j.pk;
-    const Shape &s = pk.sh;
-    hipStream_t st = stream;
-    j.seed = seed;
-    zkp_proof_info inf;
-    memset(&inf, 0, sizeof(inf));
-    uint8_t *out = (uint8_t *)commitments;
-    int ncom = 0;
-    double t0 = now_ms(), tp = t0;
-    auto mark = [&](int ph) -> hipError_t {
-        PCHECK(hipStreamSynchronize(st));
-        double t = now_ms();
-        inf.phase_ms[ph] = t - tp;
-        tp = t;
-        return hipSuccess;
-    };
-
-    // Fiat-Shamir: commitments and evaluations are absorbed in the order the
-    // reference writes them; every challenge is squeezed from the transcript
-    Transcript tr;
-
-    // ---- 0. witness (the caller's advice in the reference) -----------------
-    PCHECK(witness(j, st));
-    if (flags & ZKP_TAMPER_GATE) {
-        // self-test of the quotient check: one usable row breaks the gate
-        uint8_t *d = (uint8_t *)j.adv_lag[s.col_gate + 3] + (size_t)(s.u / 2) * FR;
-        PCHECK(zk_fr_op(d, d, pk.cf(C_ONE), OP_ADD, 1, st));
-    }
-    PCHECK(mark(0));
-    const double t_start = tp;
-
-    // ---- 1. advice commitments (Lagrange basis) and iNTT --------------------
-    {
-        std::vector<const void *> polys(j.adv_lag.begin(), j.adv_lag.end());
-        PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
-        tr.points(out + (size_t)ncom * 64, s.A);
-        ncom += s.A;
-        (void)tr.challenge();  // theta (src/lib.rs:395): this circuit's lookups are single columns
-        PCHECK(intt_all(j, j.adv_lag, j.adv_coef, j.adv_tmp, st));
-    }
-    PCHECK(mark(1));
-
-    // ---- 2. lookup multiplicities -------------------------------------------
-    // as src/lib.rs:404-504: index the table, then count each lookup's input
-    // (one table column serves every lookup of this circuit); the grand-product
-    // buffers are free until phase 3 and hold the index and the counters
-    if (s.L)
-        PCHECK(prepare_table_lookup(pk.t_lag, nullptr, nullptr, nullptr, nullptr, j.w_temp, nullptr, 0, s.k, s.u,
-                                    st));
-    for (int l = 0; l < s.L; l++) {
-        PCHECK(zk::wit_zero_rows(j.m_lag[l], 0, s.n, st));
-        PCHECK(calc_m(j.m_lag[l], pk.t_lag, j.adv_lag[s.col_lookup + l], nullptr, j.w_temp, j.w_sum, j.w_tab,
-                      nullptr, 0, s.k, s.u, st));
-    }
-    if (s.L) {
-        std::vector<const void *> polys(j.m_lag.begin(), j.m_lag.end());
-        PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
-        tr.points(out + (size_t)ncom * 64, s.L);
-        ncom += s.L;
-        PCHECK(intt_all(j, j.m_lag, j.m_coef, j.m_tmp, st));
-    }
-    {
-        const zkh::Fr bg[2] = {tr.challenge(), tr.challenge()};  // beta, gamma (src/lib.rs:531-532)
-        PCHECK(put_challenges(j, J_BETA, bg, 2, st));
-        memcpy(inf.challenges[0], &bg[0], FR);
-        memcpy(inf.challenges[1], &bg[1], FR);
-    }
-    PCHECK(mark(2));
-
-    // ---- 3. grand products ----------------------------------------------------
-    void *beta = j.cf(J_BETA), *gamma = j.cf(J_GAMMA);
-    for (int l = 0; l < s.L; l++) {
-        PCHECK(logup_sum_input_inv(j.w_sum, j.adv_lag[s.col_lookup + l], j.w_temp, beta, 0, (int)s.n, st));
-        PCHECK(hipMemcpyAsync(j.w_tab, pk.t_lag, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
-        PCHECK(hipMemsetAsync(j.cf(J_LASTZ), 0, FR, st));
-        PCHECK(eval_logup_z_compose(j.zl_lag[l], j.w_sum, j.w_tab, j.m_lag[l], beta, j.cf(J_LASTZ), (int)s.u,
-                                    (int)s.n, st));
-    }
-    // device pointer arrays: [values | sigmas | zs], then the gate term list,
-    // its rotations, the permutation sets, the permutation columns' cosets and
-    // the shuffle inputs' / tables' cosets; shuffle z inputs reuse the start
-    void **pa = j.ptrs;
-    void **pa_gate = pa + 2 * s.P + s.Sp, **pa_sets = pa_gate + 16 + 8;
-    void **pa_pv = pa_sets + s.Sp, **pa_ps = pa_pv + s.P;
-    void **pa_sh = pa_ps + s.P;
-    int *pa_rots = (int *)(pa_gate + 16);
-    if (s.P > 0) {
-        std::vector<void *> host(2 * s.P + s.Sp);
-        for (int c = 0; c < s.P; c++) {
-            host[c] = j.adv_lag[s.col_perm + c];
-            host[s.P + c] = pk.sig_lag[c];
-        }
-        for (int t = 0; t < s.Sp; t++) host[2 * s.P + t] = j.zp_lag[t];
-        PCHECK(hipMemcpyAsync(pa, host.data(), host.size() * 8, hipMemcpyHostToDevice, st));
-        for (int t = 0; t < s.Sp; t++) {
-            int c0 = t * kChunk, nc = s.P - c0 < kChunk ? s.P - c0 : kChunk;
-            PCHECK(zk_permutation_z_set(j.zp_lag[t], pa + c0, pa + s.P + c0, nc, beta, gamma, pk.cf(C_DELTAS + c0),
-                                        pk.cf(C_DELTA), pk.cf(C_WN), j.w_temp, (int)s.n, st));
-        }
-        PCHECK(zk_permutation_chain(pa + 2 * s.P, s.Sp, (int)s.u, (int)s.n, j.w_sum, st));
-        PCHECK(hipStreamSynchronize(st));  // the host pointer array
-    }
-    for (int t = 0; t < s.S; t++) {
-        void *host[2] = {j.adv_lag[s.col_shuffle + 2 * t], j.adv_lag[s.col_shuffle + 2 * t + 1]};
-        PCHECK(hipMemcpyAsync(pa, host, 16, hipMemcpyHostToDevice, st));
-        PCHECK(zk_shuffle_z(j.zs_lag[t], pa, pa + 1, 1, beta, j.w_temp, (int)s.n, st));
-        PCHECK(hipStreamSynchronize(st));
-    }
-    {
-        // permutation, lookup, shuffle: the reference's transcript order
-        // (src/lib.rs:968-980)
-        std::vector<const void *> polys;
-        for (auto p : j.zp_lag) polys.push_back(p);
-        for (auto p : j.zl_lag) polys.push_back(p);
-        for (auto p : j.zs_lag) polys.push_back(p);
-        if (!polys.empty()) PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
-        tr.points(out + (size_t)ncom * 64, (int)polys.size());
-        ncom += (int)polys.size();
-        const zkh::Fr yc = tr.challenge();  // y (src/lib.rs:988)
-        PCHECK(put_challenges(j, J_Y, &yc, 1, st));
-        memcpy(inf.challenges[2], &yc, FR);
-        // closure of every grand product at row u: logup 0, permutation (last
-        // set) 1, shuffle 1
-        int bad = 0;  // bit 0: a logup sum, bit 1: the permutation, bit 2: a shuffle
-        uint64_t one[4], got[4];
-        PCHECK(hipMemcpyAsync(one, pk.cf(C_ONE), FR, hipMemcpyDeviceToHost, st));
-        PCHECK(hipStreamSynchronize(st));
-        auto at_u = [&](const void *z) -> hipError_t {
-            PCHECK(hipMemcpyAsync(got, (const uint8_t *)z + (size_t)s.u * FR, FR, hipMemcpyDeviceToHost, st));
-            return hipStreamSynchronize(st);
+    return zk::guard([&]() -> hipError_t {
+        if (!job || !commitments) return hipErrorInvalidValue;
+        const int batched = flags & ZKP_BATCHED;
+        Job &j = *(Job *)job;
+        const Pk &pk = *j.pk;
+        const Shape &s = pk.sh;
+        hipStream_t st = stream;
+        j.seed = seed;
+        zkp_proof_info inf;
+        memset(&inf, 0, sizeof(inf));
+        uint8_t *out = (uint8_t *)commitments;
+        int ncom = 0;
+        double t0 = now_ms(), tp = t0;
+        auto mark = [&](int ph) -> hipError_t {
+            PCHECK(hipStreamSynchronize(st));
+            double t = now_ms();
+            inf.phase_ms[ph] = t - tp;
+            tp = t;
+            return hipSuccess;
         };
-        for (auto z : j.zl_lag) {
-            PCHECK(at_u(z));
-            if (got[0] | got[1] | got[2] | got[3]) bad |= 1;
-        }
-        if (s.Sp) {
-            PCHECK(at_u(j.zp_lag[s.Sp - 1]));
-            if (memcmp(got, one, FR) != 0) bad |= 2;
-        }
-        for (auto z : j.zs_lag) {
-            PCHECK(at_u(z));
-            if (memcmp(got, one, FR) != 0) bad |= 4;
-        }
-        inf.z_ok = bad ? 0 : 1;
-        inf.z_fail = bad;
-        PCHECK(intt_all(j, j.zl_lag, j.zl_coef, j.zl_tmp, st));
-        PCHECK(intt_all(j, j.zp_lag, j.zp_coef, j.zp_tmp, st));
-        PCHECK(intt_all(j, j.zs_lag, j.zs_coef, j.zs_tmp, st));
-    }
-    PCHECK(mark(3));
 
-    // ---- 4. h(X) on the extended coset ----------------------------------------
-    // ext slots: gate a,b,c,d | f_l | m_l | zl_l | perm v | zp | shuffle in, sh | zs
-    const int E_GATE = 0, E_F = 4, E_M = E_F + s.L, E_ZL = E_M + s.L, E_V = E_ZL + s.L, E_ZP = E_V + s.P,
-              E_IN = E_ZP + s.Sp, E_ZS = E_IN + 2 * s.S;
-    {
-        std::vector<const void *> c;
-        for (int i = 0; i < 4; i++) c.push_back(j.adv_coef[s.col_gate + i]);
-        for (int l = 0; l < s.L; l++) c.push_back(j.adv_coef[s.col_lookup + l]);
-        for (auto p : j.m_coef) c.push_back(p);
-        for (auto p : j.zl_coef) c.push_back(p);
-        for (int i = 0; i < s.P; i++) c.push_back(j.adv_coef[s.col_perm + i]);
-        for (auto p : j.zp_coef) c.push_back(p);
-        for (int t = 0; t < 2 * s.S; t++) c.push_back(j.adv_coef[s.col_shuffle + t]);
-        for (auto p : j.zs_coef) c.push_back(p);
-        PCHECK(extend_all(j, c, 0, st));
-    }
-    void *y = j.cf(J_Y);
-    const int rot1 = 1 << kExtBits;  // w X on the coset
-    const int N = (int)s.N;
-    PCHECK(hipMemsetAsync(j.h, 0, (size_t)s.N * FR, st));
-    // gate: q a b + q c - q d
-    {
-        void *hv[13] = {pk.cf(C_ONE), pk.q_ext, j.ext[E_GATE], j.ext[E_GATE + 1], nullptr,
-                        pk.cf(C_ONE), pk.q_ext, j.ext[E_GATE + 2], nullptr,
-                        pk.cf(C_MONE), pk.q_ext, j.ext[E_GATE + 3], nullptr};
-        int rots[7] = {0, 0, 0, 0, 0, 0, 0};
-        PCHECK(hipMemcpyAsync(pa_gate, hv, sizeof(hv), hipMemcpyHostToDevice, st));
-        PCHECK(hipMemcpyAsync(pa_rots, rots, sizeof(rots), hipMemcpyHostToDevice, st));
-        PCHECK(field_op_batch_mul_sum(j.h, pa_gate, pa_rots, 13, N, st));
-        PCHECK(hipStreamSynchronize(st));
-    }
-    // permutation
-    if (s.Sp > 0) {
-        PCHECK(zk_permutation_eval_h_p1_on(j.h, j.ext[E_ZP], j.ext[E_ZP + s.Sp - 1], pk.l0_ext, pk.llast_ext, y, N,
-                                           st));
-        std::vector<void *> sets(s.Sp);
-        for (int t = 0; t < s.Sp; t++) sets[t] = j.ext[E_ZP + t];
-        PCHECK(hipMemcpyAsync(pa_sets, sets.data(), (size_t)s.Sp * 8, hipMemcpyHostToDevice, st));
-        PCHECK(zk_permutation_eval_h_p2_on(j.h, pa_sets, pk.l0_ext, pk.llast_ext, y, s.Sp, (int)s.u * rot1, N, st));
-        PCHECK(hipStreamSynchronize(st));
-        // per set: the fused product term (sigma and X resident as cosets)
-        std::vector<void *> cols(2 * s.P);
-        for (int c = 0; c < s.P; c++) {
-            cols[c] = j.ext[E_V + c];
-            cols[s.P + c] = pk.sig_ext[c];
-            PCHECK(zk_fr_op(j.cf(J_BDELTA + c), beta, pk.cf(C_DELTAS + c), OP_MUL, 1, st));
+        // Fiat-Shamir: commitments and evaluations are absorbed in the order the
+        // reference writes them; every challenge is squeezed from the transcript
+        Transcript tr;
+
+        // ---- 0. witness (the caller's advice in the reference) -----------------
+        PCHECK(witness(j, st));
+        if (flags & ZKP_TAMPER_GATE) {
+            // self-test of the quotient check: one usable row breaks the gate
+            uint8_t *d = (uint8_t *)j.adv_lag[s.col_gate + 3] + (size_t)(s.u / 2) * FR;
+            PCHECK(zk_fr_op(d, d, pk.cf(C_ONE), OP_ADD, 1, st));
         }
-        PCHECK(hipMemcpyAsync(pa_pv, cols.data(), cols.size() * 8, hipMemcpyHostToDevice, st));
-        for (int t = 0; t < s.Sp; t++) {
-            int c0 = t * kChunk, nc = s.P - c0 < kChunk ? s.P - c0 : kChunk;
-            PCHECK(zk_permutation_eval_h_set(j.h, j.ext[E_ZP + t], pa_pv + c0, pa_ps + c0, nc, beta, gamma,
-                                             j.cf(J_BDELTA + c0), pk.x_ext, pk.lact_ext, y, rot1, N, st));
+        PCHECK(mark(0));
+        const double t_start = tp;
+
+        // ---- 1. advice commitments (Lagrange basis) and iNTT --------------------
+        {
+            std::vector<const void *> polys(j.adv_lag.begin(), j.adv_lag.end());
+            PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
+            tr.points(out + (size_t)ncom * 64, s.A);
+            ncom += s.A;
+            (void)tr.challenge();  // theta (src/lib.rs:395): this circuit's lookups are single columns
+            PCHECK(intt_all(j, j.adv_lag, j.adv_coef, j.adv_tmp, st));
         }
-        PCHECK(hipStreamSynchronize(st));  // the host pointer array
-    }
-    // logup: (t + beta), (f + beta), product-sum 1
-    for (int l = 0; l < s.L; l++) {
-        PCHECK(field_op(j.aux0, j.ext[E_F + l], 0, nullptr, nullptr, 0, beta, N, 0, st));
-        PCHECK(field_op(j.aux1, pk.t_ext, 0, nullptr, nullptr, 0, beta, N, 0, st));
-        PCHECK(logup_eval_h(j.h, j.aux0, pk.ones_ext, j.aux1, j.ext[E_M + l], j.ext[E_ZL + l], j.ext[E_ZL + l],
-                            pk.l0_ext, pk.llast_ext, pk.lact_ext, y, rot1, N, st));
-    }
-    // shuffle: each group one plain input column and one plain table column,
-    // the case the reference hands to shuffle_eval_h_v2 with betas [beta, ...]
-    // (src/eval_h.rs:1095-1152); the kernel adds beta itself
-    if (s.S > 0) {
-        std::vector<void *> sh(2 * s.S);
+        PCHECK(mark(1));
+
+        // ---- 2. lookup multiplicities -------------------------------------------
+        // as src/lib.rs:404-504: index the table, then count each lookup's input
+        // (one table column serves every lookup of this circuit); the grand-product
+        // buffers are free until phase 3 and hold the index and the counters
+        if (s.L)
+            PCHECK(prepare_table_lookup(pk.t_lag, nullptr, nullptr, nullptr, nullptr, j.w_temp, nullptr, 0, s.k, s.u,
+                                        st));
+        for (int l = 0; l < s.L; l++) {
+            PCHECK(zk::wit_zero_rows(j.m_lag[l], 0, s.n, st));
+            PCHECK(calc_m(j.m_lag[l], pk.t_lag, j.adv_lag[s.col_lookup + l], nullptr, j.w_temp, j.w_sum, j.w_tab,
+                          nullptr, 0, s.k, s.u, st));
+        }
+        if (s.L) {
+            std::vector<const void *> polys(j.m_lag.begin(), j.m_lag.end());
+            PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
+            tr.points(out + (size_t)ncom * 64, s.L);
+            ncom += s.L;
+            PCHECK(intt_all(j, j.m_lag, j.m_coef, j.m_tmp, st));
+        }
+        {
+            const zkh::Fr bg[2] = {tr.challenge(), tr.challenge()};  // beta, gamma (src/lib.rs:531-532)
+            PCHECK(put_challenges(j, J_BETA, bg, 2, st));
+            memcpy(inf.challenges[0], &bg[0], FR);
+            memcpy(inf.challenges[1], &bg[1], FR);
+        }
+        PCHECK(mark(2));
+
+        // ---- 3. grand products ----------------------------------------------------
+        void *beta = j.cf(J_BETA), *gamma = j.cf(J_GAMMA);
+        for (int l = 0; l < s.L; l++) {
+            PCHECK(logup_sum_input_inv(j.w_sum, j.adv_lag[s.col_lookup + l], j.w_temp, beta, 0, (int)s.n, st));
+            PCHECK(hipMemcpyAsync(j.w_tab, pk.t_lag, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
+            PCHECK(hipMemsetAsync(j.cf(J_LASTZ), 0, FR, st));
+            PCHECK(eval_logup_z_compose(j.zl_lag[l], j.w_sum, j.w_tab, j.m_lag[l], beta, j.cf(J_LASTZ), (int)s.u,
+                                        (int)s.n, st));
+        }
+        // device pointer arrays: [values | sigmas | zs], then the gate term list,
+        // its rotations, the permutation sets, the permutation columns' cosets and
+        // the shuffle inputs' / tables' cosets; shuffle z inputs reuse the start
+        void **pa = j.ptrs;
+        void **pa_gate = pa + 2 * s.P + s.Sp, **pa_sets = pa_gate + 16 + 8;
+        void **pa_pv = pa_sets + s.Sp, **pa_ps = pa_pv + s.P;
+        void **pa_sh = pa_ps + s.P;
+        int *pa_rots = (int *)(pa_gate + 16);
+        if (s.P > 0) {
+            std::vector<void *> host(2 * s.P + s.Sp);
+            for (int c = 0; c < s.P; c++) {
+                host[c] = j.adv_lag[s.col_perm + c];
+                host[s.P + c] = pk.sig_lag[c];
+            }
+            for (int t = 0; t < s.Sp; t++) host[2 * s.P + t] = j.zp_lag[t];
+            PCHECK(hipMemcpyAsync(pa, host.data(), host.size() * 8, hipMemcpyHostToDevice, st));
+            for (int t = 0; t < s.Sp; t++) {
+                int c0 = t * kChunk, nc = s.P - c0 < kChunk ? s.P - c0 : kChunk;
+                PCHECK(zk_permutation_z_set(j.zp_lag[t], pa + c0, pa + s.P + c0, nc, beta, gamma, pk.cf(C_DELTAS + c0),
+                                            pk.cf(C_DELTA), pk.cf(C_WN), j.w_temp, (int)s.n, st));
+            }
+            PCHECK(zk_permutation_chain(pa + 2 * s.P, s.Sp, (int)s.u, (int)s.n, j.w_sum, st));
+            PCHECK(hipStreamSynchronize(st));  // the host pointer array
+        }
         for (int t = 0; t < s.S; t++) {
-            sh[t] = j.ext[E_IN + 2 * t];
-            sh[s.S + t] = j.ext[E_IN + 2 * t + 1];
+            void *host[2] = {j.adv_lag[s.col_shuffle + 2 * t], j.adv_lag[s.col_shuffle + 2 * t + 1]};
+            PCHECK(hipMemcpyAsync(pa, host, 16, hipMemcpyHostToDevice, st));
+            PCHECK(zk_shuffle_z(j.zs_lag[t], pa, pa + 1, 1, beta, j.w_temp, (int)s.n, st));
+            PCHECK(hipStreamSynchronize(st));
         }
-        PCHECK(hipMemcpyAsync(pa_sh, sh.data(), sh.size() * 8, hipMemcpyHostToDevice, st));
-        for (int t = 0; t < s.S; t++)
-            PCHECK(zk_shuffle_eval_h_v2_on(j.h, pa_sh + t, pa_sh + s.S + t, beta, 1, j.ext[E_ZS + t], pk.l0_ext,
-                                           pk.llast_ext, pk.lact_ext, y, rot1, N, st));
-        PCHECK(hipStreamSynchronize(st));  // the host pointer array
-    }
-    PCHECK(mark(4));
+        {
+            // permutation, lookup, shuffle: the reference's transcript order
+            // (src/lib.rs:968-980)
+            std::vector<const void *> polys;
+            for (auto p : j.zp_lag) polys.push_back(p);
+            for (auto p : j.zl_lag) polys.push_back(p);
+            for (auto p : j.zs_lag) polys.push_back(p);
+            if (!polys.empty()) PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
+            tr.points(out + (size_t)ncom * 64, (int)polys.size());
+            ncom += (int)polys.size();
+            const zkh::Fr yc = tr.challenge();  // y (src/lib.rs:988)
+            PCHECK(put_challenges(j, J_Y, &yc, 1, st));
+            memcpy(inf.challenges[2], &yc, FR);
+            // closure of every grand product at row u: logup 0, permutation (last
+            // set) 1, shuffle 1
+            int bad = 0;  // bit 0: a logup sum, bit 1: the permutation, bit 2: a shuffle
+            uint64_t one[4], got[4];
+            PCHECK(hipMemcpyAsync(one, pk.cf(C_ONE), FR, hipMemcpyDeviceToHost, st));
+            PCHECK(hipStreamSynchronize(st));
+            auto at_u = [&](const void *z) -> hipError_t {
+                PCHECK(hipMemcpyAsync(got, (const uint8_t *)z + (size_t)s.u * FR, FR, hipMemcpyDeviceToHost, st));
+                return hipStreamSynchronize(st);
+            };
+            for (auto z : j.zl_lag) {
+                PCHECK(at_u(z));
+                if (got[0] | got[1] | got[2] | got[3]) bad |= 1;
+            }
+            if (s.Sp) {
+                PCHECK(at_u(j.zp_lag[s.Sp - 1]));
+                if (memcmp(got, one, FR) != 0) bad |= 2;
+            }
+            for (auto z : j.zs_lag) {
+                PCHECK(at_u(z));
+                if (memcmp(got, one, FR) != 0) bad |= 4;
+            }
+            inf.z_ok = bad ? 0 : 1;
+            inf.z_fail = bad;
+            PCHECK(intt_all(j, j.zl_lag, j.zl_coef, j.zl_tmp, st));
+            PCHECK(intt_all(j, j.zp_lag, j.zp_coef, j.zp_tmp, st));
+            PCHECK(intt_all(j, j.zs_lag, j.zs_coef, j.zs_tmp, st));
+        }
+        PCHECK(mark(3));
 
-    // ---- 5. vanishing -----------------------------------------------------------
-    const void *hpoly = nullptr;  // h(X) folded in x^n: opened at x in phase 6
-    {
-        PCHECK(zk_field_mul_zip_on(j.h, pk.cf(C_TEV), 4, N, st));
-        bool sw = false;
-        PCHECK(ntt(j.h, j.h_tmp, nullptr, pk.w_e_inv, pk.cf(C_NNINV), s.ek, 8, &sw, st));
-        void *c = sw ? j.h_tmp : j.h;
-        PCHECK(extended_prepare(c, pk.cf(C_INV0), 3, (int)s.n, N, 1, st));
-        // all constraints hold iff h is a polynomial of degree < 3n
-        PCHECK(hipMemsetAsync(j.flag, 0, 4, st));
-        PCHECK(zk::wit_any_nonzero(c, (uint64_t)kQuot * s.n, s.N, j.flag, st));
-        uint32_t flag = 1;
-        PCHECK(hipMemcpyAsync(&flag, j.flag, 4, hipMemcpyDeviceToHost, st));
-        PCHECK(hipStreamSynchronize(st));
-        inf.quotient_ok = flag == 0 ? 1 : 0;
-        std::vector<const void *> pieces;
-        for (int i = 0; i < kQuot; i++) pieces.push_back((uint8_t *)c + (size_t)i * s.n * FR);
-        PCHECK(commit(j, pk.g, pieces, out + (size_t)ncom * 64, batched != 0, st));
-        tr.points(out + (size_t)ncom * 64, kQuot);
-        ncom += kQuot;
-        const zkh::Fr xc = tr.challenge();  // x
-        PCHECK(put_challenges(j, J_X, &xc, 1, st));
-        memcpy(inf.challenges[3], &xc, FR);
-        // x^n, then h_pieces = sum_i piece_i x^(n i) by Horner into the last piece
-        PCHECK(square_times(j.cf(J_XN), j.cf(J_X), s.k, st));
-        uint8_t *last = (uint8_t *)c + (size_t)(kQuot - 1) * s.n * FR;
-        for (int i = kQuot - 2; i >= 0; i--)
-            PCHECK(field_op(last, last, 0, j.cf(J_XN), (uint8_t *)c + (size_t)i * s.n * FR, 0, nullptr, (int)s.n, 0,
-                            st));
-        if (h_pieces) PCHECK(hipMemcpyAsync(h_pieces, last, (size_t)s.n * FR, hipMemcpyDeviceToHost, st));
-        hpoly = last;
-    }
-    PCHECK(mark(5));
+        // ---- 4. h(X) on the extended coset ----------------------------------------
+        // ext slots: gate a,b,c,d | f_l | m_l | zl_l | perm v | zp | shuffle in, sh | zs
+        const int E_GATE = 0, E_F = 4, E_M = E_F + s.L, E_ZL = E_M + s.L, E_V = E_ZL + s.L, E_ZP = E_V + s.P,
+                  E_IN = E_ZP + s.Sp, E_ZS = E_IN + 2 * s.S;
+        {
+            std::vector<const void *> c;
+            for (int i = 0; i < 4; i++) c.push_back(j.adv_coef[s.col_gate + i]);
+            for (int l = 0; l < s.L; l++) c.push_back(j.adv_coef[s.col_lookup + l]);
+            for (auto p : j.m_coef) c.push_back(p);
+            for (auto p : j.zl_coef) c.push_back(p);
+            for (int i = 0; i < s.P; i++) c.push_back(j.adv_coef[s.col_perm + i]);
+            for (auto p : j.zp_coef) c.push_back(p);
+            for (int t = 0; t < 2 * s.S; t++) c.push_back(j.adv_coef[s.col_shuffle + t]);
+            for (auto p : j.zs_coef) c.push_back(p);
+            PCHECK(extend_all(j, c, 0, st));
+        }
+        void *y = j.cf(J_Y);
+        const int rot1 = 1 << kExtBits;  // w X on the coset
+        const int N = (int)s.N;
+        PCHECK(hipMemsetAsync(j.h, 0, (size_t)s.N * FR, st));
+        // gate: q a b + q c - q d
+        {
+            void *hv[13] = {pk.cf(C_ONE), pk.q_ext, j.ext[E_GATE], j.ext[E_GATE + 1], nullptr,
+                            pk.cf(C_ONE), pk.q_ext, j.ext[E_GATE + 2], nullptr,
+                            pk.cf(C_MONE), pk.q_ext, j.ext[E_GATE + 3], nullptr};
+            int rots[7] = {0, 0, 0, 0, 0, 0, 0};
+            PCHECK(hipMemcpyAsync(pa_gate, hv, sizeof(hv), hipMemcpyHostToDevice, st));
+            PCHECK(hipMemcpyAsync(pa_rots, rots, sizeof(rots), hipMemcpyHostToDevice, st));
+            PCHECK(field_op_batch_mul_sum(j.h, pa_gate, pa_rots, 13, N, st));
+            PCHECK(hipStreamSynchronize(st));
+        }
+        // permutation
+        if (s.Sp > 0) {
+            PCHECK(zk_permutation_eval_h_p1_on(j.h, j.ext[E_ZP], j.ext[E_ZP + s.Sp - 1], pk.l0_ext, pk.llast_ext, y, N,
+                                               st));
+            std::vector<void *> sets(s.Sp);
+            for (int t = 0; t < s.Sp; t++) sets[t] = j.ext[E_ZP + t];
+            PCHECK(hipMemcpyAsync(pa_sets, sets.data(), (size_t)s.Sp * 8, hipMemcpyHostToDevice, st));
+            PCHECK(zk_permutation_eval_h_p2_on(j.h, pa_sets, pk.l0_ext, pk.llast_ext, y, s.Sp, (int)s.u * rot1, N, st));
+            PCHECK(hipStreamSynchronize(st));
+            // per set: the fused product term (sigma and X resident as cosets)
+            std::vector<void *> cols(2 * s.P);
+            for (int c = 0; c < s.P; c++) {
+                cols[c] = j.ext[E_V + c];
+                cols[s.P + c] = pk.sig_ext[c];
+                PCHECK(zk_fr_op(j.cf(J_BDELTA + c), beta, pk.cf(C_DELTAS + c), OP_MUL, 1, st));
+            }
+            PCHECK(hipMemcpyAsync(pa_pv, cols.data(), cols.size() * 8, hipMemcpyHostToDevice, st));
+            for (int t = 0; t < s.Sp; t++) {
+                int c0 = t * kChunk, nc = s.P - c0 < kChunk ? s.P - c0 : kChunk;
+                PCHECK(zk_permutation_eval_h_set(j.h, j.ext[E_ZP + t], pa_pv + c0, pa_ps + c0, nc, beta, gamma,
+                                                 j.cf(J_BDELTA + c0), pk.x_ext, pk.lact_ext, y, rot1, N, st));
+            }
+            PCHECK(hipStreamSynchronize(st));  // the host pointer array
+        }
+        // logup: (t + beta), (f + beta), product-sum 1
+        for (int l = 0; l < s.L; l++) {
+            PCHECK(field_op(j.aux0, j.ext[E_F + l], 0, nullptr, nullptr, 0, beta, N, 0, st));
+            PCHECK(field_op(j.aux1, pk.t_ext, 0, nullptr, nullptr, 0, beta, N, 0, st));
+            PCHECK(logup_eval_h(j.h, j.aux0, pk.ones_ext, j.aux1, j.ext[E_M + l], j.ext[E_ZL + l], j.ext[E_ZL + l],
+                                pk.l0_ext, pk.llast_ext, pk.lact_ext, y, rot1, N, st));
+        }
+        // shuffle: each group one plain input column and one plain table column,
+        // the case the reference hands to shuffle_eval_h_v2 with betas [beta, ...]
+        // (src/eval_h.rs:1095-1152); the kernel adds beta itself
+        if (s.S > 0) {
+            std::vector<void *> sh(2 * s.S);
+            for (int t = 0; t < s.S; t++) {
+                sh[t] = j.ext[E_IN + 2 * t];
+                sh[s.S + t] = j.ext[E_IN + 2 * t + 1];
+            }
+            PCHECK(hipMemcpyAsync(pa_sh, sh.data(), sh.size() * 8, hipMemcpyHostToDevice, st));
+            for (int t = 0; t < s.S; t++)
+                PCHECK(zk_shuffle_eval_h_v2_on(j.h, pa_sh + t, pa_sh + s.S + t, beta, 1, j.ext[E_ZS + t], pk.l0_ext,
+                                               pk.llast_ext, pk.lact_ext, y, rot1, N, st));
+            PCHECK(hipStreamSynchronize(st));  // the host pointer array
+        }
+        PCHECK(mark(4));
 
-    // ---- 6. multi-open ------------------------------------------------------------
-    if (!(flags & ZKP_NO_OPEN)) {
-        int ok = 0, ne = 0;
-        PCHECK(open_phase(j, tr, hpoly, (flags & ZKP_SHPLONK) != 0, batched != 0, (flags & ZKP_TAMPER_EVAL) != 0,
-                          out, ncom, ok, ne, st));
-        inf.open_ok = ok;
-        inf.nevals = ne;
-    }
-    PCHECK(mark(6));
-    {
-        uint8_t d[64];
-        zk_transcript_squeeze(tr.t, nullptr, d);
-        memcpy(inf.transcript, d, sizeof(inf.transcript));
-    }
-    inf.total_ms = tp - t_start;
-    inf.ncommit = ncom;
-    if (info) *info = inf;
-    return hipSuccess;
+        // ---- 5. vanishing -----------------------------------------------------------
+        const void *hpoly = nullptr;  // h(X) folded in x^n: opened at x in phase 6
+        {
+            PCHECK(zk_field_mul_zip_on(j.h, pk.cf(C_TEV), 4, N, st));
+            bool sw = false;
+            PCHECK(ntt(j.h, j.h_tmp, nullptr, pk.w_e_inv, pk.cf(C_NNINV), s.ek, 8, &sw, st));
+            void *c = sw ? j.h_tmp : j.h;
+            PCHECK(extended_prepare(c, pk.cf(C_INV0), 3, (int)s.n, N, 1, st));
+            // all constraints hold iff h is a polynomial of degree < 3n
+            PCHECK(hipMemsetAsync(j.flag, 0, 4, st));
+            PCHECK(zk::wit_any_nonzero(c, (uint64_t)kQuot * s.n, s.N, j.flag, st));
+            uint32_t flag = 1;
+            PCHECK(hipMemcpyAsync(&flag, j.flag, 4, hipMemcpyDeviceToHost, st));
+            PCHECK(hipStreamSynchronize(st));
+            inf.quotient_ok = flag == 0 ? 1 : 0;
+            std::vector<const void *> pieces;
+            for (int i = 0; i < kQuot; i++) pieces.push_back((uint8_t *)c + (size_t)i * s.n * FR);
+            PCHECK(commit(j, pk.g, pieces, out + (size_t)ncom * 64, batched != 0, st));
+            tr.points(out + (size_t)ncom * 64, kQuot);
+            ncom += kQuot;
+            const zkh::Fr xc = tr.challenge();  // x
+            PCHECK(put_challenges(j, J_X, &xc, 1, st));
+            memcpy(inf.challenges[3], &xc, FR);
+            // x^n, then h_pieces = sum_i piece_i x^(n i) by Horner into the last piece
+            PCHECK(square_times(j.cf(J_XN), j.cf(J_X), s.k, st));
+            uint8_t *last = (uint8_t *)c + (size_t)(kQuot - 1) * s.n * FR;
+            for (int i = kQuot - 2; i >= 0; i--)
+                PCHECK(field_op(last, last, 0, j.cf(J_XN), (uint8_t *)c + (size_t)i * s.n * FR, 0, nullptr, (int)s.n, 0,
+                                st));
+            if (h_pieces) PCHECK(hipMemcpyAsync(h_pieces, last, (size_t)s.n * FR, hipMemcpyDeviceToHost, st));
+            hpoly = last;
+        }
+        PCHECK(mark(5));
+
+        // ---- 6. multi-open ------------------------------------------------------------
+        if (!(flags & ZKP_NO_OPEN)) {
+            int ok = 0, ne = 0;
+            PCHECK(open_phase(j, tr, hpoly, (flags & ZKP_SHPLONK) != 0, batched != 0, (flags & ZKP_TAMPER_EVAL) != 0,
+                              out, ncom, ok, ne, st));
+            inf.open_ok = ok;
+            inf.nevals = ne;
+        }
+        PCHECK(mark(6));
+        {
+            uint8_t d[64];
+            zk_transcript_squeeze(tr.t, nullptr, d);
+            memcpy(inf.transcript, d, sizeof(inf.transcript));
+        }
+        inf.total_ms = tp - t_start;
+        inf.ncommit = ncom;
+        if (info) *info = inf;
+        return hipSuccess;
+    });
 }
 
 }  // extern "C"

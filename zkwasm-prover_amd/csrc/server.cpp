@@ -23,6 +23,7 @@
 #include <thread>
 #include <vector>
 
+#include "guard.h"
 #include "zkwasm_prover_kernel.h"
 
 namespace {
@@ -154,83 +155,99 @@ extern "C" {
 // device reports free after the keys, less 4 GiB).
 hipError_t zks_create(const zkp_circuit *shapes, int nshapes, int slots, size_t hbm_budget, int flags,
                       unsigned long long key_seed, void **srv_out) {
-    if (!shapes || nshapes <= 0 || slots <= 0 || slots > 32 || !srv_out) return hipErrorInvalidValue;
-    Server *s = new Server();
-    s->flags = flags & (ZKP_BATCHED | ZKP_SHPLONK | ZKP_NO_OPEN);
-    hipError_t e = hipGetDevice(&s->device);
-    s->shapes.resize(nshapes);
-    for (int i = 0; i < nshapes && e == hipSuccess; i++) {
-        ShapeState &sh = s->shapes[i];
-        sh.circuit = shapes[i];
-        sh.job_bytes = zkp_job_bytes(&shapes[i]);
-        sh.ncommit = zkp_commitment_count(&shapes[i]);
-        if (!sh.job_bytes) e = hipErrorInvalidValue;
-        if (e == hipSuccess) e = zkp_pk_create(&shapes[i], key_seed + (unsigned long long)i, nullptr, &sh.pk);
-    }
-    if (e == hipSuccess) {
-        size_t free_b = 0, total_b = 0;
-        e = hipMemGetInfo(&free_b, &total_b);
-        size_t avail = free_b > ((size_t)4 << 30) ? free_b - ((size_t)4 << 30) : 0;
-        s->budget = hbm_budget && hbm_budget < avail ? hbm_budget : avail;
-        for (auto &sh : s->shapes)
-            if (sh.job_bytes > s->budget) e = hipErrorOutOfMemory;  // a shape that could never run
-    }
-    if (e != hipSuccess) {
-        for (auto &sh : s->shapes)
-            if (sh.pk) (void)zkp_pk_destroy(sh.pk);
-        delete s;
-        return e;
-    }
-    s->streams.resize(slots);
-    for (int w = 0; w < slots; w++) (void)hipStreamCreate(&s->streams[w]);
-    for (int w = 0; w < slots; w++) s->workers.emplace_back([s, w] { s->worker(w); });
-    *srv_out = s;
-    return hipSuccess;
+    return zk::guard([&]() -> hipError_t {
+        if (!shapes || nshapes <= 0 || slots <= 0 || slots > 32 || !srv_out) return hipErrorInvalidValue;
+        Server *s = new Server();
+        s->flags = flags & (ZKP_BATCHED | ZKP_SHPLONK | ZKP_NO_OPEN);
+        hipError_t e = hipGetDevice(&s->device);
+        s->shapes.resize(nshapes);
+        for (int i = 0; i < nshapes && e == hipSuccess; i++) {
+            ShapeState &sh = s->shapes[i];
+            sh.circuit = shapes[i];
+            sh.job_bytes = zkp_job_bytes(&shapes[i]);
+            sh.ncommit = zkp_commitment_count(&shapes[i]);
+            if (!sh.job_bytes) e = hipErrorInvalidValue;
+            if (e == hipSuccess) e = zkp_pk_create(&shapes[i], key_seed + (unsigned long long)i, nullptr, &sh.pk);
+        }
+        if (e == hipSuccess) {
+            size_t free_b = 0, total_b = 0;
+            e = hipMemGetInfo(&free_b, &total_b);
+            size_t avail = free_b > ((size_t)4 << 30) ? free_b - ((size_t)4 << 30) : 0;
+            s->budget = hbm_budget && hbm_budget < avail ? hbm_budget : avail;
+            for (auto &sh : s->shapes)
+                if (sh.job_bytes > s->budget) e = hipErrorOutOfMemory;  // a shape that could never run
+        }
+        if (e != hipSuccess) {
+            for (auto &sh : s->shapes)
+                if (sh.pk) (void)zkp_pk_destroy(sh.pk);
+            delete s;
+            return e;
+        }
+        s->streams.assign(slots, nullptr);
+        for (int w = 0; w < slots && e == hipSuccess; w++) e = hipStreamCreate(&s->streams[w]);
+        if (e == hipSuccess) {
+            try {
+                for (int w = 0; w < slots; w++) s->workers.emplace_back([s, w] { s->worker(w); });
+            } catch (...) {
+                e = hipErrorUnknown;  // a worker thread could not be started
+            }
+        }
+        if (e != hipSuccess) {
+            (void)zks_destroy(s);  // stops and joins the workers started, frees streams and keys
+            return e;
+        }
+        *srv_out = s;
+        return hipSuccess;
+    });
 }
 
 // Queues one proof; returns its request id (>= 0) or -1.
 int zks_submit(void *srv, int shape, unsigned long long seed) {
-    Server *s = (Server *)srv;
-    if (!s || shape < 0 || shape >= (int)s->shapes.size()) return -1;
-    int id;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        id = (int)s->reqs.size();
-        s->reqs.emplace_back();
-        Request &r = s->reqs.back();
-        r.shape = shape;
-        r.seed = seed;
-        r.t_submit = now_ms();
-        r.commitments.assign((size_t)s->shapes[shape].ncommit * 64, 0);
-        s->queue.push_back(id);
-    }
-    s->cv_work.notify_all();
-    return id;
+    return zk::guard_or<int>(-1, [&]() -> int {
+        Server *s = (Server *)srv;
+        if (!s || shape < 0 || shape >= (int)s->shapes.size()) return -1;
+        int id;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            id = (int)s->reqs.size();
+            s->reqs.emplace_back();
+            Request &r = s->reqs.back();
+            r.shape = shape;
+            r.seed = seed;
+            r.t_submit = now_ms();
+            r.commitments.assign((size_t)s->shapes[shape].ncommit * 64, 0);
+            s->queue.push_back(id);
+        }
+        s->cv_work.notify_all();
+        return id;
+    });
 }
 
 // Waits for request id; copies its commitments (ncommit * 64 B, may be null)
 // and its result record.
 hipError_t zks_wait(void *srv, int id, void *commitments, zks_result *res) {
-    Server *s = (Server *)srv;
-    if (!s) return hipErrorInvalidValue;
-    std::unique_lock<std::mutex> lk(s->mu);
-    if (id < 0 || id >= (int)s->reqs.size()) return hipErrorInvalidValue;
-    s->cv_done.wait(lk, [&] { return s->reqs[id].done; });
-    Request &r = s->reqs[id];
-    if (commitments) memcpy(commitments, r.commitments.data(), r.commitments.size());
-    if (res) {
-        memset(res, 0, sizeof(*res));
-        res->shape = r.shape;
-        res->err = (int)r.err;
-        res->z_ok = r.info.z_ok;
-        res->quotient_ok = r.info.quotient_ok;
-        res->ncommit = r.info.ncommit;
-        res->queue_ms = r.t_start - r.t_submit;
-        res->run_ms = r.t_done - r.t_start;
-        res->latency_ms = r.t_done - r.t_submit;
-        res->prove_ms = r.info.total_ms;
-    }
-    return r.err;
+    return zk::guard([&]() -> hipError_t {
+        Server *s = (Server *)srv;
+        if (!s) return hipErrorInvalidValue;
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (id < 0 || id >= (int)s->reqs.size()) return hipErrorInvalidValue;
+        s->cv_done.wait(lk, [&] { return s->reqs[id].done; });
+        Request &r = s->reqs[id];
+        if (commitments) memcpy(commitments, r.commitments.data(), r.commitments.size());
+        if (res) {
+            memset(res, 0, sizeof(*res));
+            res->shape = r.shape;
+            res->err = (int)r.err;
+            res->z_ok = r.info.z_ok;
+            res->quotient_ok = r.info.quotient_ok;
+            res->ncommit = r.info.ncommit;
+            res->queue_ms = r.t_start - r.t_submit;
+            res->run_ms = r.t_done - r.t_start;
+            res->latency_ms = r.t_done - r.t_submit;
+            res->prove_ms = r.info.total_ms;
+        }
+        return r.err;
+    });
 }
 
 // Server-wide counters: peak proofs in flight, peak and budget bytes.
@@ -246,26 +263,29 @@ hipError_t zks_stats(void *srv, int *peak_running, size_t *peak_bytes, size_t *b
 
 // Finishes every queued request, stops the workers, frees all device state.
 hipError_t zks_destroy(void *srv) {
-    Server *s = (Server *)srv;
-    if (!s) return hipSuccess;
-    {
-        std::unique_lock<std::mutex> lk(s->mu);
-        s->cv_done.wait(lk, [&] {
-            for (auto &r : s->reqs)
-                if (!r.done) return false;
-            return true;
-        });
-        s->stop = true;
-    }
-    s->cv_work.notify_all();
-    for (auto &t : s->workers) t.join();
-    for (auto st : s->streams) (void)hipStreamDestroy(st);
-    for (auto &sh : s->shapes) {
-        for (void *j : sh.idle) (void)zkp_job_destroy(j);
-        if (sh.pk) (void)zkp_pk_destroy(sh.pk);
-    }
-    delete s;
-    return hipSuccess;
+    return zk::guard([&]() -> hipError_t {
+        Server *s = (Server *)srv;
+        if (!s) return hipSuccess;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_done.wait(lk, [&] {
+                for (auto &r : s->reqs)
+                    if (!r.done) return false;
+                return true;
+            });
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        for (auto &t : s->workers) t.join();
+        for (auto st : s->streams)
+            if (st) (void)hipStreamDestroy(st);
+        for (auto &sh : s->shapes) {
+            for (void *j : sh.idle) (void)zkp_job_destroy(j);
+            if (sh.pk) (void)zkp_pk_destroy(sh.pk);
+        }
+        delete s;
+        return hipSuccess;
+    });
 }
 
 }  // extern "C"

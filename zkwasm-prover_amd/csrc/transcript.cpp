@@ -12,6 +12,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include "guard.h"
 #include "host_fr.h"
 #include "zkwasm_prover_kernel.h"
 
@@ -135,9 +136,11 @@ int zk_blake2b(void *out, int outlen, const void *in, size_t inlen, const void *
 }
 
 void *zk_transcript_new(void) {
-    Blake2b *b = new Blake2b();
-    b->init(64, kPersonal);
-    return b;
+    return zk::guard_or<void *>(nullptr, [&]() -> void * {
+        Blake2b *b = new Blake2b();
+        b->init(64, kPersonal);
+        return b;
+    });
 }
 void zk_transcript_free(void *t) { delete (Blake2b *)t; }
 
