@@ -1212,10 +1212,11 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
             }
             PCHECK(hipStreamSynchronize(st));  // the host pointer array
         }
-        // logup: (t + beta), (f + beta), product-sum 1
+        // logup: (t + beta) once (every lookup reads the one table), (f + beta),
+        // product-sum 1
+        if (s.L > 0) PCHECK(field_op(j.aux1, pk.t_ext, 0, nullptr, nullptr, 0, beta, N, 0, st));
         for (int l = 0; l < s.L; l++) {
             PCHECK(field_op(j.aux0, j.ext[E_F + l], 0, nullptr, nullptr, 0, beta, N, 0, st));
-            PCHECK(field_op(j.aux1, pk.t_ext, 0, nullptr, nullptr, 0, beta, N, 0, st));
             PCHECK(logup_eval_h(j.h, j.aux0, pk.ones_ext, j.aux1, j.ext[E_M + l], j.ext[E_ZL + l], j.ext[E_ZL + l],
                                 pk.l0_ext, pk.llast_ext, pk.lact_ext, y, rot1, N, st));
         }
