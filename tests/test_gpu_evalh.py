@@ -158,6 +158,24 @@ def test_shuffle_kernels(gpu, zo, log_n):
     assert (dv.from_dev(d) == exp).all()
 
 
+@pytest.mark.parametrize("g,rot", [(1, 1), (1, 4), (16, 4)])
+def test_shuffle_v2_group_counts(gpu, zo, g, rot):
+    """one group (the prover core's call) and 16 groups (the size of the
+    reference's betas vector, src/eval_h.rs:1080-1083), bit-exact"""
+    dv, call, A = _env()
+    n = 1 << 12
+    z, l0, ll, la = [_r(zo, n, 130 + j) for j in range(4)]
+    y, res0 = _r(zo, 1, 135), _r(zo, n, 136)
+    ins = [_r(zo, n, 140 + j) for j in range(g)]
+    tbs = [_r(zo, n, 160 + j) for j in range(g)]
+    betas = _r(zo, g, 180)
+    d = dv.to_dev(res0)
+    call("shuffle_eval_h_v2", dv.ptr(d), A.put_all(ins), A.put_all(tbs), A.put(betas), g, A.put(z), A.put(l0),
+         A.put(ll), A.put(la), A.put(y), rot, n)
+    exp = zo.shuffle_eval_h_v2(res0, ins, tbs, betas, z, l0, ll, la, y, rot)
+    assert (dv.from_dev(d) == exp).all()
+
+
 @pytest.mark.parametrize("nc,rot", [(1, 4), (2, 4), (3, -4)])
 def test_permutation_eval_h_set(gpu, zo, nc, rot):
     """the fused permutation-set term equals the same expression built from the
