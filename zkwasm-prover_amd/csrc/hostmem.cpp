@@ -105,7 +105,12 @@ int take_stage(CopyQueue &cq, size_t bytes, hipError_t *err) {
         if (pick < 0) pick = (int)i;  // a free one to grow
     }
     if (pick < 0) {
-        cq.stages.push_back(Stage{});
+        try {
+            cq.stages.push_back(Stage{});
+        } catch (...) {
+            *err = hipErrorOutOfMemory;
+            return -1;
+        }
         pick = (int)cq.stages.size() - 1;
     }
     Stage &s = cq.stages[pick];
@@ -183,7 +188,10 @@ hipError_t zk_copyq_push(void *q, void *host_dst, const void *dev_src, size_t by
             if (ms > cq.max_wait_ms) cq.max_wait_ms = ms;
             if (err != hipSuccess) return err;
         }
-        Entry e;
+        // the queue slot is taken before any work is enqueued: once a copy into
+        // host_dst is in flight, nothing below can throw
+        cq.q.emplace_back();
+        Entry &e = cq.q.back();
     #define QCHECK(x)                                                                    \
         do {                                                                             \
             err = (x);                                                                   \
@@ -191,6 +199,7 @@ hipError_t zk_copyq_push(void *q, void *host_dst, const void *dev_src, size_t by
                 if (e.stage >= 0) cq.stages[e.stage].busy = false;                       \
                 for (hipEvent_t ev : {e.ready, e.snap, e.done})                          \
                     if (ev) (void)hipEventDestroy(ev);                                   \
+                cq.q.pop_back();                                                         \
                 return err;                                                              \
             }                                                                            \
         } while (0)
@@ -212,7 +221,6 @@ hipError_t zk_copyq_push(void *q, void *host_dst, const void *dev_src, size_t by
         QCHECK(hipEventRecord(e.done, cq.cs));
     #undef QCHECK
         if (flags & ZK_COPYQ_RELEASE) e.release = (void *)dev_src;
-        cq.q.push_back(e);
         cq.pushed++;
         return hipSuccess;
     });
