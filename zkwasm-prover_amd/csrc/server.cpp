@@ -206,17 +206,24 @@ int zks_submit(void *srv, int shape, unsigned long long seed) {
     return zk::guard_or<int>(-1, [&]() -> int {
         Server *s = (Server *)srv;
         if (!s || shape < 0 || shape >= (int)s->shapes.size()) return -1;
+        // built outside the list: an allocation that fails leaves no request
+        // behind that would never complete (zks_destroy waits for all of them)
+        Request r;
+        r.shape = shape;
+        r.seed = seed;
+        r.commitments.assign((size_t)s->shapes[shape].ncommit * 64, 0);
         int id;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             id = (int)s->reqs.size();
-            s->reqs.emplace_back();
-            Request &r = s->reqs.back();
-            r.shape = shape;
-            r.seed = seed;
             r.t_submit = now_ms();
-            r.commitments.assign((size_t)s->shapes[shape].ncommit * 64, 0);
-            s->queue.push_back(id);
+            s->reqs.push_back(std::move(r));  // if this throws, nothing was added
+            try {
+                s->queue.push_back(id);
+            } catch (...) {
+                s->reqs.pop_back();
+                throw;
+            }
         }
         s->cv_work.notify_all();
         return id;
