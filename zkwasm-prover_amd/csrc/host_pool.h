@@ -51,10 +51,10 @@ class HostPool {
             Slab &s = slabs_[si];
             for (int c = 0; c < kChunksPerSlab; c++)
                 if (!(s.mask >> c & 1)) {
+                    void *p = s.base + (size_t)c * kChunkBytes;
+                    owner_[p] = si;  // first: if it throws, nothing is marked
                     s.mask |= 1u << c;
                     s.used++;
-                    void *p = s.base + (size_t)c * kChunkBytes;
-                    owner_[p] = si;
                     return p;
                 }
             return nullptr;  // unreachable: used < kChunksPerSlab
@@ -62,10 +62,10 @@ class HostPool {
         int si = free_slab();
         if (si < 0) return nullptr;
         Slab &s = slabs_[si];
+        owner_[s.base] = si;  // first: if it throws, nothing is marked
         s.whole = true;
         s.used = kChunksPerSlab;
         s.mask = (1u << kChunksPerSlab) - 1;
-        owner_[s.base] = si;
         return s.base;
     }
 
@@ -107,7 +107,12 @@ class HostPool {
     bool grow() {
         void *p = Backend::alloc(kSlabBytes);
         if (!p) return false;
-        slabs_.push_back(Slab{(uint8_t *)p, 0u, 0, false});
+        try {
+            slabs_.push_back(Slab{(uint8_t *)p, 0u, 0, false});
+        } catch (...) {
+            Backend::release(p);
+            throw;
+        }
         return true;
     }
     int free_slab() {
