@@ -70,7 +70,9 @@ hipError_t msm(void *res, const void *points, const void *scalars, void *indices
  * runs on host cores and the call returns after the results are written
  * (the reference's caller synchronises right after it, src/cuda/msm.rs:436-439);
  * with ZK_MSM_TAIL=device in the environment it stays on the GPU and is
- * asynchronous. */
+ * asynchronous.  A slice msm() did not finish (its header cleared or never
+ * written) returns hipErrorInvalidValue; on the asynchronous device path that
+ * slice's 96 result bytes are all 0xff instead (no point, not the identity). */
 hipError_t batch_msm_collect(void *remain_indices, void *remain_acc, void *next_remain_indices,
                              void *next_remain_acc, void *buckets, unsigned workers, unsigned windows,
                              unsigned window_bits, unsigned batch_size, hipStream_t stream);
@@ -359,7 +361,8 @@ size_t zk_msm_batch_scratch_bytes(const zk_msm_item *items, int count, int c);
 hipError_t zk_msm_batch(void *out, const zk_msm_item *items, int count, int c, void *scratch,
                         size_t scratch_bytes, hipStream_t stream);
 /* The same without host waits: the window combination runs on the device,
- * so the call only enqueues kernels (capturable into a HIP graph). */
+ * so the call only enqueues kernels (capturable into a HIP graph).  An item
+ * whose MSM did not finish reads as 96 bytes of 0xff (no point). */
 hipError_t zk_msm_batch_async(void *out, const zk_msm_item *items, int count, int c, void *scratch,
                               size_t scratch_bytes, hipStream_t stream);
 
@@ -458,14 +461,21 @@ void zk_transcript_free(void *t);
 int zk_transcript_common_point(void *t, const void *x, const void *y);
 int zk_transcript_common_scalar(void *t, const void *s);
 int zk_transcript_squeeze(void *t, void *mont_out, void *digest64);
+/* halo2's Fr::from_bytes_wide: 64 little-endian bytes mod r, Montgomery out */
+int zk_fr_from_bytes_wide(const void *in64, void *mont_out);
 
 /* ---- prover core: one proof's device arithmetic ------------------------ */
-/* The PLONKish benchmark circuit: a*b + c - d = 0 under a selector on the
- * usable rows; `lookups` advice columns range-checked against a fixed table of
- * 2^table_log values; `perm_cols` advice columns joined by random copy
- * cycles; `shuffles` column pairs, one a row permutation of the other on
- * 2^shuffle_log rows; `blinding` random rows at the end.  advice must be at
- * least 4 + lookups + perm_cols + 2 * shuffles. */
+/* The PLONKish benchmark circuit: a*b + c - d = 0 under a selector q (fixed
+ * column 0) on the usable rows; `fixed` - 2 coefficient gates
+ * q (f_j a - e_j) = 0 over the fixed columns 2.. (f_j) and advice columns e_j;
+ * `lookups` advice columns range-checked against a fixed table (fixed column
+ * 1) of 2^table_log values; a permutation over `perm_cols` columns -- the
+ * `instance` instance columns first, then perm_cols - instance advice
+ * columns -- joined by random copy cycles; `shuffles` column pairs, one a row
+ * permutation of the other on 2^shuffle_log rows; `blinding` random rows at
+ * the end.  Needs fixed >= 2, 0 <= instance <= perm_cols and advice >=
+ * 4 + lookups + (perm_cols - instance) + 2 * shuffles + (fixed - 2); other
+ * shapes are rejected with hipErrorInvalidValue. */
 typedef struct {
     int k, advice, fixed, instance, lookups, table_log, perm_cols, shuffles, shuffle_log, blinding;
 } zkp_circuit;
@@ -499,17 +509,26 @@ hipError_t zkp_pk_destroy(void *pk);
  * zkp_pk_srs_secret copies tau (Montgomery Fr) -- a synthetic setup whose
  * trapdoor tests use to check openings without pairings.
  * zkp_pk_fixed_commitments writes the verifying key's commitments (affine,
- * 64 B each) of q, t and the perm_cols sigma columns; returns their count
- * (2 + perm_cols) or -1 if out is too small. */
+ * 64 B each) of the fixed columns q, t, f_0.. and the perm_cols sigma
+ * columns; returns their count (fixed + perm_cols) or -1 if out is too small.
+ * zkp_pk_vk_repr writes the key's transcript representation (Montgomery Fr):
+ * Blake2b-512 "Halo2-Verify-Key" over the key's canonical description (domain,
+ * column and argument counts, every commitment) read by from_bytes_wide, the
+ * first scalar each proof absorbs (vk.hash_into, src/lib.rs:242). */
 hipError_t zkp_pk_srs_secret(void *pk, void *tau_out);
 int zkp_pk_fixed_commitments(void *pk, void *out, int max);
+hipError_t zkp_pk_vk_repr(void *pk, void *out32);
 /* Per-proof device state (witness, grand products, extended polys, h). */
 hipError_t zkp_job_create(void *pk, void **job);
 hipError_t zkp_job_destroy(void *job);
 /* One proof's arithmetic (src/lib.rs:205-1194 phase order, then the
  * multi-open of src/multiopen.rs) for seed, on stream; commitments (host,
- * room for zkp_commitment_count affine points) gets info.ncommit points,
- * h_pieces (host, 2^k Fr, may be null) the folded quotient.  flags:
+ * room for zkp_commitment_count affine points) gets info.ncommit points in
+ * transcript order: instance (absorbed, not written), advice, m, permutation
+ * z, logup z, shuffle z, the vanishing argument's random polynomial, the
+ * quotient pieces, then the opening's (GWC: one per rotation in the order
+ * last, cur, next; Shplonk: h, L); h_pieces (host, 2^k Fr, may be null) the
+ * folded quotient.  flags:
  * ZKP_BATCHED commits through zk_msm_batch (else one zk_msm per polynomial);
  * ZKP_TAMPER_GATE breaks the gate on one usable row (a self-test: the
  * quotient check must then fail); ZKP_SHPLONK opens with Shplonk
@@ -521,13 +540,22 @@ hipError_t zkp_job_destroy(void *job);
 #define ZKP_SHPLONK 4
 #define ZKP_NO_OPEN 8
 #define ZKP_TAMPER_EVAL 16
-/* The last proof's opening: its evaluations (host, Montgomery Fr, in query
- * order: the {x} set -- advice, q, t, sigmas, m, h -- then the {x, xw} set --
- * logup z, shuffle z, last permutation z -- at x then xw, then the other
- * permutation z at x, xw, xw^u) and the opening challenges [v, y', u] (y', u
- * zero for GWC).  Returns the number of evaluations, or -1 if max is too
- * small. */
+/* The last proof's opening: its evaluations (host, Montgomery Fr) in the
+ * reference's order (src/lib.rs:1025-1078): h(x) -- computed, not written --
+ * then instance, advice, fixed (q, t, f_j) and the random polynomial at x,
+ * sigmas at x, each permutation z at x, xw and (all but the last) x w^-(b+1),
+ * each lookup's m at x and z at x, xw, each shuffle z at x, xw; and the
+ * opening challenges [v, y', u] (y', u zero for GWC).  Returns the number of
+ * evaluations, or -1 if max is too small.
+ * zkp_proof_bytes: the last proof's bytes (written points, compressed to 32
+ * bytes -- canonical x little-endian, the parity of y in the top bit, the
+ * identity all zeros -- and written scalars, canonical 32 bytes) in
+ * transcript order; returns the length, -1 if max is too small; out = null
+ * returns the length only.  zkp_proof_instance: the last proof's instance
+ * columns (n Fr each, `rows` Fr apart in out). */
 int zkp_proof_openings(void *job, void *evals, int max, void *challenges3);
+long long zkp_proof_bytes(void *job, void *out, long long max);
+hipError_t zkp_proof_instance(void *job, void *out, int rows);
 hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commitments, void *h_pieces,
                      zkp_proof_info *info, hipStream_t stream);
 

@@ -391,25 +391,44 @@ static inline uint32_t get_bits(const zo_fr *c, unsigned start, unsigned bits) {
 /* Unsigned-window Pippenger: the bucket semantics of cuda/msm.cu:9-181
  * (digit d of window w adds P_i into bucket (w, d); d = 0 is skipped), then
  * sum_d d*B_d per window by running sums and Horner over windows
- * (cuda/msm.cu:246-318). */
+ * (cuda/msm.cu:246-318).  Parallel over (point range, window) tasks: each
+ * task fills its own buckets from its range and reduces them to its share
+ * S_w^(r) = sum_d d*B_{w,d}^(r); S_w = sum_r S_w^(r) (bucket sums are linear
+ * in the points), so every host core has work even when the window count is
+ * below the thread count.  The point is the same for any split. */
 void zo_msm(zo_g1j *r, const zo_g1a *p, const zo_fr *s, size_t n, int c) {
+    const int threads = zo_get_threads();
+    size_t ranges = 1;
     if (c <= 0) {
+        /* ranges so that there are >= 2 tasks per thread, then the window for
+           one range's points */
+        int c0 = 4;
+        while (c0 < 16 && ((size_t)1 << (c0 + 3)) < n) c0++;
+        const size_t w0 = (254 + c0 - 1) / c0;
+        ranges = (2 * (size_t)threads + w0 - 1) / w0;
+        if (ranges > n / 4096) ranges = n / 4096;
+        if (ranges < 1) ranges = 1;
+        const size_t nr = (n + ranges - 1) / ranges;
         c = 4;
-        while (c < 16 && ((size_t)1 << (c + 3)) < n) c++;
+        while (c < 16 && ((size_t)1 << (c + 3)) < nr) c++;
     }
     const unsigned bits = 254;
-    int windows = (bits + c - 1) / c;
+    const int windows = (bits + c - 1) / c;
     zo_fr *canon = (zo_fr *)malloc(sizeof(zo_fr) * (n ? n : 1));
     zo_fr_from_mont_n(canon, s, n);
-    zo_g1j *wsum = (zo_g1j *)malloc(sizeof(zo_g1j) * windows);
-    size_t nb = (size_t)1 << c;
+    zo_g1j *part = (zo_g1j *)malloc(sizeof(zo_g1j) * windows * ranges);
+    const size_t nb = (size_t)1 << c;
+    const long tasks = (long)windows * (long)ranges;
 #pragma omp parallel for OMP_THREADS schedule(dynamic, 1)
-    for (int w = 0; w < windows; w++) {
+    for (long t = 0; t < tasks; t++) {
+        const int w = (int)(t % windows);
+        const size_t rg = (size_t)(t / windows);
+        const size_t lo = n * rg / ranges, hi = n * (rg + 1) / ranges;
         zo_g1j *b = (zo_g1j *)malloc(sizeof(zo_g1j) * nb);
         for (size_t i = 0; i < nb; i++) g1j_set_id(&b[i]);
         unsigned start = (unsigned)w * c;
         unsigned nbits = bits - start < (unsigned)c ? bits - start : (unsigned)c;
-        for (size_t i = 0; i < n; i++) {
+        for (size_t i = lo; i < hi; i++) {
             uint32_t d = get_bits(&canon[i], start, nbits);
             if (d) zo_g1_add_mixed(&b[d], &b[d], &p[i]);
         }
@@ -418,8 +437,13 @@ void zo_msm(zo_g1j *r, const zo_g1a *p, const zo_fr *s, size_t n, int c) {
             zo_g1_add(&run, &run, &b[d]);
             zo_g1_add(&acc, &acc, &run);
         }
-        wsum[w] = acc;
+        part[t] = acc;
         free(b);
+    }
+    zo_g1j *wsum = (zo_g1j *)malloc(sizeof(zo_g1j) * windows);
+    for (int w = 0; w < windows; w++) {
+        wsum[w] = part[w];
+        for (size_t rg = 1; rg < ranges; rg++) zo_g1_add(&wsum[w], &wsum[w], &part[rg * windows + w]);
     }
     zo_g1j acc = wsum[windows - 1];
     for (int w = windows - 2; w >= 0; w--) {
@@ -427,7 +451,7 @@ void zo_msm(zo_g1j *r, const zo_g1a *p, const zo_fr *s, size_t n, int c) {
         zo_g1_add(&acc, &acc, &wsum[w]);
     }
     *r = acc;
-    free(wsum); free(canon);
+    free(wsum); free(part); free(canon);
 }
 
 /* ---- NTT ----------------------------------------------------------------- */

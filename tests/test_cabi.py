@@ -222,11 +222,19 @@ def test_prover_core_sizing_host_only(lib):
     pk20, job20 = footprint(c20)
     pk18, job18 = footprint(Circuit.benchmark(18))
     assert pk20 > pk18 > 0 and job20 > job18 > 0
-    # 40 advice + 6 m + 6 logup z + 6 permutation sets + 2 shuffle z + 3 pieces
-    # + 3 GWC quotients (points x, x w, x w^u)
-    assert lib.zkp_commitment_count(ctypes.byref(c20)) == 40 + 12 + 6 + 2 + 3 + 3
+    # 1 instance + 40 advice + 6 m + 6 logup z + 6 permutation sets + 2 shuffle
+    # z + 1 random + 3 pieces + 3 GWC quotients (rotations last, cur, next)
+    assert lib.zkp_commitment_count(ctypes.byref(c20)) == 1 + 40 + 12 + 6 + 2 + 1 + 3 + 3
     # per proof at k=20: ~ 40 * 3 n + 63 coset polys of 4n, well under 288 GB
     assert 8e9 < job20 < 64e9
     bad = Circuit.benchmark(20)
     bad.advice = 10  # fewer than the columns the roles need
     assert footprint(bad) == (0, 0)
+    # every field is honoured or the shape is rejected: no table or selector
+    # column, more instance than permutation columns, a negative count, or
+    # coefficient gates without advice columns for their outputs
+    for field, value in (("fixed", 1), ("instance", 13), ("instance", -1), ("fixed", 30)):
+        bad = Circuit.benchmark(20)
+        setattr(bad, field, value)
+        assert footprint(bad) == (0, 0), (field, value)
+        assert lib.zkp_commitment_count(ctypes.byref(bad)) == 0

@@ -269,3 +269,26 @@ def test_msm_two_entry_layouts_agree(gpu, zo):
     a = _zk_msm_dev(pts.data_ptr(), sc.data_ptr(), h)
     b = _zk_msm_dev(pts.data_ptr() + h * 64, sc.data_ptr() + h * 32, n - h)
     assert (zo.g1_to_affine(full) == zo.g1_to_affine(zo.g1_add(a, b))).all()
+
+
+def test_collect_of_unfinished_slice_is_an_error(gpu, zo, monkeypatch):
+    """batch_msm_collect over a slice msm() never finished (its header
+    cleared) reports hipErrorInvalidValue instead of a commitment; the device
+    tail marks such a slice with 96 bytes of 0xff (no point)"""
+    import torch
+    from zkprover import device as dv
+    from zkprover._lib import HipError, call
+    n = 1000
+    pts = zo.gen_bases_random(n, 17)
+    sc = zo.rand_fr(n, 18)
+    good = _zk_msm(pts, sc)  # a finished slice for comparison
+    assert (zo.g1_to_affine(good) == zo.g1_to_affine(zo.msm(pts, sc))).all()
+    slice_buf = torch.zeros(OUT_BYTES, dtype=torch.uint8, device=dv.device())
+    arr = dv.to_dev(np.array([[slice_buf.data_ptr()]], dtype=np.uint64))
+    with pytest.raises(HipError):
+        call("batch_msm_collect", None, None, None, None, arr.data_ptr(), 1, 1, 1, 1, dv.stream_handle())
+    monkeypatch.setenv("ZK_MSM_TAIL", "device")
+    call("batch_msm_collect", None, None, None, None, arr.data_ptr(), 1, 1, 1, 1, dv.stream_handle())
+    torch.cuda.synchronize()
+    monkeypatch.delenv("ZK_MSM_TAIL")
+    assert (slice_buf[:96].cpu().numpy() == 0xff).all()

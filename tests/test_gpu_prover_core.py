@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 
 def _small(k=10):
     from zkprover.prover import Circuit
-    return Circuit(k=k, advice=30, fixed=2, instance=1, lookups=3, table_log=7, perm_cols=6, shuffles=2,
+    return Circuit(k=k, advice=30, fixed=4, instance=1, lookups=3, table_log=7, perm_cols=6, shuffles=2,
                    shuffle_log=7, blinding=5)
 
 
@@ -42,15 +42,16 @@ def core(gpu):
 def test_prover_core_constraints_hold(core, zo):
     pk, pr = core
     com, pieces, info = pr.prove(11)
-    # advice, m, logup z, permutation z (3 sets), shuffle z, quotient pieces,
-    # GWC quotients at x, x w, x w^u
-    assert info.ncommit == pk.ncommit == 30 + 2 * 3 + 3 + 2 + 3 + 3
+    # instance, advice, m, permutation z (3 sets), logup z, shuffle z, the
+    # random polynomial, quotient pieces, GWC quotients at the rotations
+    # last, cur, next
+    assert info.ncommit == pk.ncommit == 1 + 30 + 3 + 3 + 3 + 2 + 1 + 3 + 3
     assert info.z_ok == 1, "grand products that did not close: %d" % info.z_fail
     assert info.quotient_ok == 1
     assert info.open_ok == 1
-    # set {x}: 30 advice, q, t, 6 sigmas, 3 m, h; {x, xw}: 3 logup z, 2
-    # shuffle z, last permutation z; {x, xw, xw^u}: 2 permutation z
-    assert info.nevals == 42 + 2 * 6 + 3 * 2
+    # h; instance, 30 advice, 4 fixed, random, 6 sigmas at x; permutation z at
+    # x, xw (and x w^-(b+1) but the last); per lookup m, z, z(xw); shuffle z, z(xw)
+    assert info.nevals == 1 + 1 + 30 + 4 + 1 + 6 + (3 * 3 - 1) + 3 * 3 + 2 * 2
     assert all(zo.g1a_is_on_curve(com[i:i + 1]) for i in range(len(com)))
     assert not (com == 0).all(axis=1).any()
     assert not (pieces == 0).all()
@@ -84,49 +85,136 @@ def test_prover_core_opening_tamper_detected(core, shplonk):
     assert info.quotient_ok == 1 and info.open_ok == 0
 
 
-def _transcript_model(zo, com, A, L, nz):
-    """beta, gamma, y, x as halo2's Blake2bWrite derives them from the
-    commitments (a model written with hashlib): advice, theta, m, beta,
-    gamma, the grand products, y, the quotient pieces, x"""
-    import hashlib
-    R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
-    h = hashlib.blake2b(digest_size=64, person=b"Halo2-Transcript")
-
-    def point(i):
-        x, y = zo.g1a_to_ints(com[i:i + 1])
-        h.update(b"\x01" + x.to_bytes(32, "little") + y.to_bytes(32, "little"))
-
-    def challenge():
-        h.update(b"\x00")
-        return int.from_bytes(h.copy().digest(), "little") % R
-
-    i = 0
-    for _ in range(A):
-        point(i); i += 1
-    challenge()  # theta
-    for _ in range(L):
-        point(i); i += 1
-    beta, gamma = challenge(), challenge()
-    for _ in range(nz):
-        point(i); i += 1
-    y = challenge()
-    for _ in range(3):
-        point(i); i += 1
-    return [beta, gamma, y, challenge()]
+P_MOD = 0x30644e72e131a029b85045b68181585d97816a916871ca8d3c208c16d87cfd47
+R_MOD = 21888242871839275222246405745257275088548364400416034343698204186575808495617
 
 
-def test_prover_core_transcript(core, zo):
-    """the challenges are the transcript's: recomputed from the returned
-    commitments with a hashlib model of halo2's Blake2bWrite"""
+class _Reader:
+    """a verifier reading a proof's bytes: points decompressed (x with the
+    parity of y in the top bit; y^2 = x^3 + 3, p = 3 mod 4), scalars
+    canonical; every item absorbed as halo2's Blake2bRead does (a model
+    written with hashlib)"""
+
+    def __init__(self, proof):
+        import hashlib
+        self.buf, self.pos = proof, 0
+        self.h = hashlib.blake2b(digest_size=64, person=b"Halo2-Transcript")
+
+    def common_point(self, x, y):
+        self.h.update(b"\x01" + x.to_bytes(32, "little") + y.to_bytes(32, "little"))
+
+    def common_scalar(self, s):
+        self.h.update(b"\x02" + s.to_bytes(32, "little"))
+
+    def read_point(self):
+        b = self.buf[self.pos:self.pos + 32]
+        self.pos += 32
+        if b == bytes(32):
+            self.common_point(0, 0)
+            return (0, 0)
+        odd = b[31] >> 7
+        x = int.from_bytes(b[:31] + bytes([b[31] & 0x7f]), "little")
+        y = pow((x * x * x + 3) % P_MOD, (P_MOD + 1) // 4, P_MOD)
+        assert y * y % P_MOD == (x * x * x + 3) % P_MOD, "not a curve point"
+        if y & 1 != odd:
+            y = P_MOD - y
+        self.common_point(x, y)
+        return (x, y)
+
+    def read_scalar(self):
+        s = int.from_bytes(self.buf[self.pos:self.pos + 32], "little")
+        self.pos += 32
+        assert s < R_MOD
+        self.common_scalar(s)
+        return s
+
+    def challenge(self):
+        self.h.update(b"\x00")
+        return int.from_bytes(self.h.copy().digest(), "little") % R_MOD
+
+
+def _verify_transcript(zo, pk, com, proof, info, open_ch, shplonk):
+    """reads the proof as the verifier does (src/lib.rs:242-1194 order): the
+    key's representation and the instance commitments absorbed, then every
+    written commitment and evaluation; returns the challenges it squeezes
+    and the points it read"""
+    c = pk.circuit
+    A, L, P, S, I = c.advice, c.lookups, c.perm_cols, c.shuffles, c.instance
+    Sp = (P + 1) // 2
+    rd = _Reader(proof)
+    rd.common_scalar(zo.from_mont(pk.vk_repr())[0])
+    for i in range(I):
+        rd.common_point(*zo.g1a_to_ints(com[i:i + 1]))
+    pts = [rd.read_point() for _ in range(A)]
+    rd.challenge()  # theta
+    pts += [rd.read_point() for _ in range(L)]
+    beta, gamma = rd.challenge(), rd.challenge()
+    pts += [rd.read_point() for _ in range(Sp + L + S + 1)]
+    y = rd.challenge()
+    pts += [rd.read_point() for _ in range(3)]
+    x = rd.challenge()
+    nev = info.nevals - 1  # h(x) is not written
+    [rd.read_scalar() for _ in range(nev)]
+    ch = [beta, gamma, y, x]
+    if shplonk:
+        ch += [rd.challenge(), rd.challenge()]  # y', v
+        pts.append(rd.read_point())
+        ch.append(rd.challenge())  # u
+        pts.append(rd.read_point())
+    else:
+        ch.append(rd.challenge())  # v
+        while rd.pos < len(proof):
+            pts.append(rd.read_point())
+    assert rd.pos == len(proof)
+    return ch, pts
+
+
+@pytest.mark.parametrize("shplonk", [False, True])
+def test_prover_core_proof_bytes_transcript(core, zo, shplonk):
+    """the proof's bytes are a halo2 transcript: a verifier holding only the
+    key's representation and the instance commitments reads every written
+    commitment and evaluation from them and squeezes the same challenges
+    (beta, gamma, y, x and the opening's) as the prover; the points it reads
+    are the prover's commitments"""
     from zkprover.prover import BATCHED, SHPLONK
     pk, pr = core
-    com, _, info = pr.prove(17)
-    exp = _transcript_model(zo, com, 30, 3, 3 + 3 + 2)
+    com, _, info = pr.prove(17, flags=BATCHED | (SHPLONK if shplonk else 0))
+    proof = pr.proof_bytes()
+    ev_m, och = pr.openings()
+    ch, pts = _verify_transcript(zo, pk, com, proof, info, och, shplonk)
     got = [zo.from_mont(np.array(info.challenges[c], dtype=np.uint64).reshape(1, 4))[0] for c in range(4)]
-    assert got == exp
-    com2, _, info2 = pr.prove(17, flags=BATCHED | SHPLONK)
-    assert bytes(info.transcript) != bytes(info2.transcript)  # the openings differ
-    assert [list(info2.challenges[c]) for c in range(4)] == [list(info.challenges[c]) for c in range(4)]
+    assert ch[:4] == got
+    oc = zo.from_mont(och)
+    if shplonk:
+        assert ch[4:] == [oc[1], oc[0], oc[2]]  # y', v, u
+    else:
+        assert ch[4] == oc[0]
+    I = pk.circuit.instance
+    exp = [tuple(zo.g1a_to_ints(com[i:i + 1])) for i in range(I, info.ncommit)]
+    assert [tuple(p) for p in pts] == exp
+    # the written evaluations are the openings but h(x), in order: they follow
+    # the advice, m, grand-product, random and quotient-piece points
+    c = pk.circuit
+    at = 32 * (c.advice + 2 * c.lookups + (c.perm_cols + 1) // 2 + c.shuffles + 1 + 3)
+    got_ev = [int.from_bytes(proof[at + 32 * i:at + 32 * i + 32], "little") for i in range(info.nevals - 1)]
+    assert got_ev == [int(v) for v in zo.from_mont(ev_m)[1:]]
+    assert len(proof) == 32 * (info.ncommit - I) + 32 * (info.nevals - 1)
+
+
+def test_prover_core_vk_repr(core, zo):
+    """the key's transcript representation: Blake2b-512 personalised
+    "Halo2-Verify-Key" of its canonical description, read mod r"""
+    pk, _ = core
+    rep = zo.from_mont(pk.vk_repr())[0]
+    assert 0 < rep < R_MOD
+    # the description covers the fixed and sigma commitments: a different key
+    # seed gives a different representation
+    from zkprover.prover import ProvingKey
+    other = ProvingKey(pk.circuit, seed=4)
+    try:
+        assert zo.from_mont(other.vk_repr())[0] != rep
+    finally:
+        other.close()
 
 
 def test_prover_core_batched_equals_plain_and_deterministic(core):

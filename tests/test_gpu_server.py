@@ -57,3 +57,32 @@ def test_server_poisson_driver(gpu):
         assert summ["all_ok"] and summ["proofs"] == 12 and summ["p99_ms"] >= summ["p50_ms"] > 0
     finally:
         s.close()
+
+
+def test_server_zkwasm_sizes_mix_equals_alone(gpu):
+    """the serving workload's circuit family at k = 18, 20 and 22: every
+    proof's commitments are the same run alone (one slot) and admitted into
+    a mix of three slots while others are on the device"""
+    from zkprover.prover import Circuit
+    from zkprover.server import Server
+    shapes = [Circuit.benchmark(k) for k in (18, 20, 22)]
+    plan = [(0, 201), (1, 202), (2, 203), (0, 204), (1, 205), (0, 206)]
+    alone = {}
+    s1 = Server(shapes, slots=1)
+    try:
+        for sh, seed in plan:
+            com, res = s1.wait(s1.submit(sh, seed), sh)
+            assert res.err == 0 and res.z_ok and res.quotient_ok, (sh, seed)
+            alone[seed] = com
+    finally:
+        s1.close()
+    s3 = Server(shapes, slots=3)
+    try:
+        ids = [(s3.submit(sh, seed), sh, seed) for sh, seed in plan]
+        for rid, sh, seed in ids:
+            com, res = s3.wait(rid, sh)
+            assert res.err == 0 and res.z_ok and res.quotient_ok, (sh, seed)
+            assert (com == alone[seed]).all(), seed
+        assert s3.stats()["peak_running"] >= 2
+    finally:
+        s3.close()

@@ -838,7 +838,7 @@ __global__ void k_slice_header(uint8_t *const *slots, uint32_t count, uint32_t c
     h->magic = kMagic;
 }
 
-__device__ __forceinline__ bool header_ok(const SliceHeader *h) {
+__host__ __device__ __forceinline__ bool header_ok(const SliceHeader *h) {
     return h->magic == kMagic && h->W >= 1 && h->W <= 128 && h->c >= 2 && h->c <= 17 && h->nbits <= 16;
 }
 
@@ -866,13 +866,11 @@ __global__ __launch_bounds__(64) void k_msm_window_finalize(uint8_t *const *__re
 __device__ void collect_slice(uint8_t *slice) {
     const SliceHeader *h = reinterpret_cast<const SliceHeader *>(slice + 64);
     if (!header_ok(h)) {
-        // no masked sums in this slice (msm() failed or was not run): emit
-        // the identity rather than reading garbage
-        G1Jac id;
-        id.x = fe_zero<FpParams>();
-        id.y = fe_one<FpParams>();
-        id.z = fe_zero<FpParams>();
-        jac_store(reinterpret_cast<G1Jac *>(slice), id);
+        // no masked sums in this slice (msm() failed or was not run): write
+        // the no-result marker (every byte 0xff: coordinates >= p, so no
+        // point and not the identity) rather than reading garbage
+        uint32_t *o = reinterpret_cast<uint32_t *>(slice);
+        for (int i = 0; i < 24; i++) o[i] = 0xffffffffu;
         return;
     }
     const uint32_t c = h->c, W = h->W;
@@ -1138,10 +1136,11 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
 // GPU wave runs in ~3 ms; host cores run it in well under 0.1 ms.  The Q
 // points are copied down, combined exactly as the device tail would, and the
 // normalised result is written back into each slice.
+// The slice's header has been checked by the caller (header_ok).
 void host_tail_one(const uint8_t *blob, uint8_t *out96) {
     const SliceHeader *h = reinterpret_cast<const SliceHeader *>(blob + 64);
     zkh::Jac res = zkh::jac_id();
-    if (h->magic == kMagic && h->W >= 1 && h->W <= 128 && h->c >= 2 && h->c <= 17 && h->nbits <= 16) {
+    {
         const uint32_t c = h->c, W = h->W, nb = h->nbits;
         const zkh::Fp *q = reinterpret_cast<const zkh::Fp *>(blob + kSliceQ);  // 4 Fp per XYZZ point
         auto Q = [&](uint32_t w, uint32_t b) {
@@ -1169,11 +1168,13 @@ hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stre
     for (uint32_t m = 0; m < batch; m++)
         ZK_CHECK(hipMemcpyAsync(&hdr[m], slices[m] + 64, sizeof(SliceHeader), hipMemcpyDeviceToHost, stream));
     ZK_CHECK(hipStreamSynchronize(stream));
+    // a slice msm() never finished (cleared or stale header) has no result:
+    // an error, not a commitment
+    for (uint32_t m = 0; m < batch; m++)
+        if (!header_ok(&hdr[m])) return hipErrorInvalidValue;
     std::vector<std::vector<uint8_t>> blob(batch);
     for (uint32_t m = 0; m < batch; m++) {
-        const SliceHeader &h = hdr[m];
-        bool ok = h.magic == kMagic && h.W >= 1 && h.W <= 128 && h.nbits <= 16;
-        size_t bytes = ok ? slice_sums_off(h.W, h.nbits) : 128;
+        const size_t bytes = slice_sums_off(hdr[m].W, hdr[m].nbits);
         blob[m].assign(bytes, 0);
         ZK_CHECK(hipMemcpyAsync(blob[m].data(), slices[m], bytes, hipMemcpyDeviceToHost, stream));
     }
@@ -1313,7 +1314,14 @@ size_t zk_msm_scratch_bytes(int n, int c) {
 static hipError_t finish_batch(uint8_t *const *slices, uint8_t **slots_dev, uint32_t count, hipStream_t stream) {
     if (device_tail_requested()) {
         ZK_CHECK(device_collect(slots_dev, count, stream));
-        return hipStreamSynchronize(stream);
+        // a slice without masked sums carries the no-result marker
+        std::vector<uint32_t> z((size_t)count * 8);
+        for (uint32_t m = 0; m < count; m++)
+            ZK_CHECK(hipMemcpyAsync(&z[(size_t)m * 8], slices[m] + 64, 32, hipMemcpyDeviceToHost, stream));
+        ZK_CHECK(hipStreamSynchronize(stream));
+        for (uint32_t m = 0; m < count; m++)
+            if (z[(size_t)m * 8] == 0xffffffffu && z[(size_t)m * 8 + 7] == 0xffffffffu) return hipErrorInvalidValue;
+        return hipSuccess;
     }
     return host_collect(slices, count, stream);
 }

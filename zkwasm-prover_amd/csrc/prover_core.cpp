@@ -37,6 +37,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <vector>
 
@@ -95,16 +96,26 @@ struct Arena {
     void *fr(size_t count) { return take(count * FR); }
 };
 
+struct Shape;
+int ptr_slots(const Shape &s);
+constexpr int kPtrSlots = 1024;  // device pointer area of a job (8 KiB)
+
 struct Shape {
-    int k, ek, A, F, I, L, P, S, Sp, tlog, slog, blind;
+    int k, ek, A, F, I, L, P, PA, S, Sp, FG, tlog, slog, blind;
     uint32_t n, N, u, tsize, R;
     // advice column roles
-    int col_gate, col_lookup, col_perm, col_shuffle;
+    int col_gate, col_lookup, col_perm, col_shuffle, col_fixed_gate;
 };
 
+// The circuit's columns: fixed 0 is the gate selector q, fixed 1 the lookup
+// table t, fixed 2.. the coefficients f_j of the gates q (f_j a - e_j) = 0;
+// the permutation spans the instance columns, then perm_cols - instance
+// advice columns.  Every field must be honoured: shapes this core cannot
+// build are rejected.
 bool make_shape(const zkp_circuit *c, Shape &s) {
-    if (!c || c->k < 4 || c->k > 24 || c->lookups < 0 || c->perm_cols < 0 || c->shuffles < 0 || c->blinding < 1 ||
-        c->table_log < 1 || c->shuffle_log < 1)
+    if (!c || c->k < 4 || c->k > 24 || c->advice < 0 || c->lookups < 0 || c->perm_cols < 0 || c->shuffles < 0 ||
+        c->blinding < 1 || c->table_log < 1 || c->shuffle_log < 1 || c->fixed < 2 || c->fixed > 64 ||
+        c->instance < 0 || c->instance > c->perm_cols)
         return false;
     s.k = c->k;
     s.ek = c->k + kExtBits;
@@ -115,8 +126,12 @@ bool make_shape(const zkp_circuit *c, Shape &s) {
     s.u = s.n - (uint32_t)(c->blinding + 1);
     s.L = c->lookups;
     s.P = c->perm_cols;
+    s.I = c->instance;
+    s.PA = s.P - s.I;
     s.S = c->shuffles;
     s.Sp = (s.P + kChunk - 1) / kChunk;
+    s.F = c->fixed;
+    s.FG = s.F - 2;
     s.tlog = c->table_log;
     s.slog = c->shuffle_log;
     s.tsize = (s.tlog < 31 && (1u << s.tlog) < s.u) ? (1u << s.tlog) : s.u;
@@ -124,12 +139,11 @@ bool make_shape(const zkp_circuit *c, Shape &s) {
     s.col_gate = 0;
     s.col_lookup = 4;
     s.col_perm = 4 + s.L;
-    s.col_shuffle = 4 + s.L + s.P;
+    s.col_shuffle = s.col_perm + s.PA;
+    s.col_fixed_gate = s.col_shuffle + 2 * s.S;
     s.A = c->advice;
-    s.F = c->fixed < 2 ? 2 : c->fixed;
-    s.I = c->instance < 0 ? 0 : c->instance;
-    if (s.A < s.col_shuffle + 2 * s.S) return false;
-    if (4 * s.P + 2 * s.Sp + 24 + 2 * s.S > 512) return false;  // device pointer area of a job
+    if (s.A < s.col_fixed_gate + s.FG) return false;
+    if (ptr_slots(s) > kPtrSlots) return false;  // device pointer area of a job
     return true;
 }
 
@@ -152,7 +166,9 @@ struct Pk {
     uint8_t *cst;  // Fr constants
     void *t_lag, *t_ext, *q_ext, *l0_ext, *llast_ext, *lact_ext, *ones_ext, *x_ext;
     void *t_coef, *q_coef;  // fixed columns opened in the multi-open step
-    std::vector<uint8_t> vk;  // commitments of q, t, sigmas (affine, host)
+    std::vector<void *> fix_lag, fix_coef, fix_ext;  // the gate coefficients f_j
+    std::vector<uint8_t> vk;  // commitments of q, t, f_j, sigmas (affine, host)
+    zkh::Fr vk_repr{};        // the verifying key's transcript representation (Montgomery)
     std::vector<void *> sig_lag, sig_coef, sig_ext;
     uint32_t *perm_head, *perm_target;
     std::vector<uint32_t *> shuf_perm;
@@ -181,6 +197,14 @@ void pk_layout(Pk &pk, Arena &a) {
     pk.lact_ext = a.fr(s.N);
     pk.ones_ext = a.fr(s.N);
     pk.x_ext = a.fr(s.N);
+    pk.fix_lag.assign(s.FG, nullptr);
+    pk.fix_coef.assign(s.FG, nullptr);
+    pk.fix_ext.assign(s.FG, nullptr);
+    for (int j = 0; j < s.FG; j++) {
+        pk.fix_lag[j] = a.fr(s.n);
+        pk.fix_coef[j] = a.fr(s.n);
+        pk.fix_ext[j] = a.fr(s.N);
+    }
     pk.sig_lag.assign(s.P, nullptr);
     pk.sig_coef.assign(s.P, nullptr);
     pk.sig_ext.assign(s.P, nullptr);
@@ -229,6 +253,48 @@ hipError_t to_coset(const Pk &pk, void *ext, const void *coef, void *tmp_e, void
     PCHECK(zk_coset_extend_batch(&it, 1, pk.cf(C_FWD0), 3, &sw, st));
     *result = sw ? tmp_e : ext;
     return hipSuccess;
+}
+
+// canonical little-endian bytes of a Montgomery Fp coordinate
+void fp_canonical(const uint8_t *mont32, uint8_t *out32) {
+    zkh::Fp v;
+    memcpy(&v, mont32, 32);
+    const zkh::Fp one_raw{{1, 0, 0, 0}};
+    v = zkh::fp_mul(v, one_raw);
+    memcpy(out32, &v, 32);
+}
+
+// The verifying key's transcript representation, absorbed first by every
+// proof (vk.hash_into, src/lib.rs:242): Blake2b-512 personalised
+// "Halo2-Verify-Key" over the pinned description of the key, read as a
+// scalar by from_bytes_wide.  halo2 hashes the Debug text of its pinned
+// key; this core hashes its own canonical encoding of the same contents:
+// the domain (k, extended k, blinding rows), the column and argument counts,
+// the table and shuffle sizes, and every fixed and sigma commitment
+// (canonical x, y; zeros for the identity).
+void vk_transcript_repr(Pk &pk) {
+    const Shape &s = pk.sh;
+    std::vector<uint8_t> b;
+    const char tag[] = "zkwasm-prover-amd pinned vk v1";
+    b.insert(b.end(), tag, tag + sizeof(tag) - 1);
+    const int32_t fields[] = {s.k, s.ek, s.blind, s.A, s.F, s.I, s.L, s.P, s.S, s.tlog, s.slog, kDegree};
+    for (int32_t f : fields)
+        for (int i = 0; i < 4; i++) b.push_back((uint8_t)((uint32_t)f >> (8 * i)));
+    for (size_t c = 0; c < pk.vk.size() / 64; c++) {
+        const uint8_t *p = pk.vk.data() + c * 64;
+        bool id = true;
+        for (int i = 0; i < 64; i++) id &= p[i] == 0;
+        uint8_t xy[64] = {0};
+        if (!id) {
+            fp_canonical(p, xy);
+            fp_canonical(p + 32, xy + 32);
+        }
+        b.insert(b.end(), xy, xy + 64);
+    }
+    static const uint8_t personal[16] = {'H', 'a', 'l', 'o', '2', '-', 'V', 'e', 'r', 'i', 'f', 'y', '-', 'K', 'e', 'y'};
+    uint8_t d[64];
+    zk_blake2b(d, 64, b.data(), b.size(), personal);
+    zk_fr_from_bytes_wide(d, &pk.vk_repr);
 }
 
 hipError_t pk_build(Pk &pk, uint64_t seed, hipStream_t st) {
@@ -320,6 +386,13 @@ hipError_t pk_build(Pk &pk, uint64_t seed, hipStream_t st) {
     PCHECK(zk::wit_indicator(lag, s.n, s.u, s.u + 1, st));
     PCHECK(lag_to_coset(lag, pk.llast_ext));
     PCHECK(zk::wit_fill_value(pk.ones_ext, pk.cf(C_ONE), s.N, st));
+    // the gate coefficients f_j: seeded values on the usable rows, zero on the
+    // rest (keygen leaves unassigned fixed cells zero)
+    for (int j = 0; j < s.FG; j++) {
+        PCHECK(zk_rand_fr(pk.fix_lag[j], (int)s.n, seed ^ (0x6666ull + (uint64_t)j), st));
+        PCHECK(zk::wit_zero_rows(pk.fix_lag[j], s.u, s.n, st));
+        PCHECK(lag_to_coset(pk.fix_lag[j], pk.fix_ext[j], pk.fix_coef[j]));
+    }
     // copy cycles over the usable cells of the permutation columns: a random
     // half of the cells, shuffled, cut into cycles of 2..4
     if (s.P > 0) {
@@ -377,37 +450,45 @@ hipError_t pk_build(Pk &pk, uint64_t seed, hipStream_t st) {
         }
         PCHECK(hipMemcpy(pk.shuf_perm[j], p.data(), (size_t)s.R * 4, hipMemcpyHostToDevice));
     }
-    // the verifying key: commitments of the fixed columns the proof opens
+    // the verifying key: commitments of the fixed columns (q, t, f_j) and of
+    // the permutation's sigma columns
     {
         std::vector<const void *> fixed = {pk.q_coef, pk.t_coef};
+        for (auto c : pk.fix_coef) fixed.push_back(c);
         for (auto c : pk.sig_coef) fixed.push_back(c);
         PCHECK(hipStreamSynchronize(st));
         pk.vk.assign(fixed.size() * 64, 0);
         PCHECK(zkp_batch_msm(pk.vk.data(), pk.g, fixed.data(), (int)fixed.size(), (int)s.n, 0));
     }
+    vk_transcript_repr(pk);
     return hipStreamSynchronize(st);
 }
 
 // ---- multi-open query layout -------------------------------------------------
-// Polynomials grouped by the points they are opened at (points: x, x w,
-// x w^u): set 0 at {x}: advice, q, t, sigmas, m, h; set 1 at {x, x w}:
-// logup z, shuffle z, the last permutation z; set 2 at {x, x w, x w^u}: the
-// other permutation z (u = the last usable row, the halo2 "last" rotation).
-int open_set_size(const Shape &s, int set) {
-    if (set == 0) return s.A + 2 + s.P + s.L + 1;
-    if (set == 1) return s.L + s.S + (s.Sp > 0 ? 1 : 0);
-    return s.Sp > 1 ? s.Sp - 1 : 0;
-}
-int open_eval_count(const Shape &s) { return open_set_size(s, 0) + 2 * open_set_size(s, 1) + 3 * open_set_size(s, 2); }
-// GWC commits one quotient per distinct point, Shplonk two polynomials
-int open_gwc_points(const Shape &s) {
-    return open_set_size(s, 2) ? 3 : open_set_size(s, 1) ? 2 : 1;
+// Rotations of the opening points in halo2's Rotation order:
+// last = -(blinding + 1) < cur = 0 < next = 1.
+enum Rot { ROT_LAST = 0, ROT_CUR = 1, ROT_NEXT = 2, ROT_COUNT = 3 };
+// Polynomials of the opening (coefficient form): instance, advice, fixed
+// (q, t, f_j), the vanishing argument's random polynomial, sigma,
+// permutation products, lookup multiplicities and sums, shuffle products,
+// and h (the quotient folded in x^n).
+struct OpenLayout {
+    std::vector<const void *> poly;
+    std::vector<std::pair<int, int>> evals;    // (poly, rot): src/lib.rs:1025-1078 order, h first
+    std::vector<std::pair<int, int>> queries;  // (poly, rot): src/lib.rs:1094-1176 order
+};
+// GWC commits one quotient per distinct rotation, Shplonk two polynomials
+int open_gwc_points(const Shape &s) { return s.Sp > 1 ? 3 : (s.Sp + s.L + s.S > 0 ? 2 : 1); }
+int open_eval_count(const Shape &s) {
+    const int zp = s.Sp ? 3 * s.Sp - 1 : 0;
+    return 1 + s.I + s.A + s.F + 1 + s.P + zp + 3 * s.L + 2 * s.S;
 }
 constexpr int kOpenCst = 64;
 
 // ---- per-proof state ---------------------------------------------------------
 // beta, gamma, y, x; x^n; logup carry; opening challenges v, y', u; opening
-// points x w, x w^u; then beta delta^j per permutation column
+// points x w, x w^u; then beta delta^j per permutation column, then the
+// gates' y powers and their negatives
 enum JConst {
     J_BETA, J_GAMMA, J_Y, J_X, J_XN, J_LASTZ, J_V, J_SY, J_U, J_XW, J_XWU, J_BDELTA,
     J_COUNT_FIXED = J_BDELTA
@@ -418,9 +499,10 @@ struct Job {
     uint64_t seed;
     uint8_t *mem = nullptr;
     size_t bytes = 0;
-    std::vector<void *> adv_lag, adv_coef, adv_tmp;
+    std::vector<void *> inst_lag, inst_coef, inst_tmp, adv_lag, adv_coef, adv_tmp;
     std::vector<void *> m_lag, m_coef, m_tmp, zl_lag, zl_coef, zl_tmp, zp_lag, zp_coef, zp_tmp, zs_lag, zs_coef,
         zs_tmp;
+    void *rnd_lag, *rnd_coef, *rnd_tmp;  // the vanishing argument's random polynomial
     uint32_t *vals;
     void *w_sum, *w_temp, *w_tab;
     std::vector<void *> ext, ext_tmp;  // coset buffers, NTT spares
@@ -434,11 +516,24 @@ struct Job {
     void *open_cst;  // opening scalars staged from the host
     std::vector<zkh::Fr> last_evals;  // the last proof's opening (host copies)
     zkh::Fr last_open_ch[3] = {};     // v, y', u
+    std::vector<uint8_t> proof;       // the last proof's bytes (written points and scalars)
     void *cf(int c) const { return cst + (size_t)c * FR; }
+    int j_ypow(int i) const { return J_COUNT_FIXED + pk->sh.P + i; }                  // y^i, i <= FG
+    int j_mypow(int i) const { return J_COUNT_FIXED + pk->sh.P + pk->sh.FG + 1 + i; }  // -y^i
 };
 
 // how many polynomials get an extended coset in phase 4
-int ext_count(const Shape &s) { return 4 + 3 * s.L + s.P + s.Sp + 3 * s.S; }
+int ext_count(const Shape &s) { return 4 + 3 * s.L + s.P + s.Sp + 3 * s.S + s.FG; }
+// gate term list of field_op_batch_mul_sum: pointers (main gate 13 entries,
+// each coefficient gate 9) and rotations (one per column entry)
+int gate_list_len(const Shape &s) { return 13 + 9 * s.FG; }
+int gate_rot_len(const Shape &s) { return 7 + 5 * s.FG; }
+// device pointer area: [values | sigmas | zs] (2P + Sp), gate list, gate
+// rotations (as pointer-size slots), permutation sets (Sp), permutation
+// values and sigma cosets (2P), shuffle inputs and tables (2S)
+int ptr_slots(const Shape &s) {
+    return 2 * s.P + s.Sp + gate_list_len(s) + (gate_rot_len(s) + 1) / 2 + s.Sp + 2 * s.P + 2 * s.S + 8;
+}
 
 size_t msm_group_bytes(const Shape &s) {
     std::vector<zk_msm_item> it(kMsmGroup, zk_msm_item{nullptr, nullptr, (int)s.n, 0});
@@ -451,6 +546,9 @@ void job_layout(Job &j, Arena &a) {
         v.assign(c, nullptr);
         for (auto &p : v) p = a.fr(len);
     };
+    cols(j.inst_lag, s.I, s.n);
+    cols(j.inst_coef, s.I, s.n);
+    cols(j.inst_tmp, s.I, s.n);
     cols(j.adv_lag, s.A, s.n);
     cols(j.adv_coef, s.A, s.n);
     cols(j.adv_tmp, s.A, s.n);
@@ -466,6 +564,9 @@ void job_layout(Job &j, Arena &a) {
     cols(j.zs_lag, s.S, s.n);
     cols(j.zs_coef, s.S, s.n);
     cols(j.zs_tmp, s.S, s.n);
+    j.rnd_lag = a.fr(s.n);
+    j.rnd_coef = a.fr(s.n);
+    j.rnd_tmp = a.fr(s.n);
     j.vals = (uint32_t *)a.take((size_t)s.u * 4);
     j.w_sum = a.fr(s.n);
     j.w_temp = a.fr(s.n);
@@ -478,8 +579,8 @@ void job_layout(Job &j, Arena &a) {
     j.r_acc = a.fr(s.N);
     j.aux0 = a.fr(s.N);
     j.aux1 = a.fr(s.N);
-    j.cst = a.take((size_t)(J_COUNT_FIXED + 3 * s.P + 8) * FR);
-    j.ptrs = (void **)a.take(4096);
+    j.cst = a.take((size_t)(J_COUNT_FIXED + s.P + 2 * (s.FG + 1) + 8) * FR);
+    j.ptrs = (void **)a.take((size_t)kPtrSlots * 8);
     j.msm_bytes = msm_group_bytes(s);
     j.msm_scratch = a.take(j.msm_bytes);
     j.msm_out = a.take((size_t)kMsmGroup * ZK_MSM_OUT_BYTES);
@@ -504,18 +605,20 @@ hipError_t commit(Job &j, const void *bases, const std::vector<const void *> &po
                 PCHECK(zk_msm((uint8_t *)j.msm_out + (size_t)m * ZK_MSM_OUT_BYTES, bases, polys[first + m],
                               (int)s.n, 0, j.msm_scratch, j.msm_bytes, st));
         }
-        for (int m = 0; m < cnt; m++) {
-            uint8_t jac[96];
-            PCHECK(hipMemcpyAsync(jac, (uint8_t *)j.msm_out + (size_t)m * ZK_MSM_OUT_BYTES, 96,
+        std::vector<uint8_t> jac((size_t)cnt * 96);
+        for (int m = 0; m < cnt; m++)
+            PCHECK(hipMemcpyAsync(&jac[(size_t)m * 96], (uint8_t *)j.msm_out + (size_t)m * ZK_MSM_OUT_BYTES, 96,
                                   hipMemcpyDeviceToHost, st));
-            PCHECK(hipStreamSynchronize(st));
+        PCHECK(hipStreamSynchronize(st));
+        for (int m = 0; m < cnt; m++) {
+            const uint8_t *q = &jac[(size_t)m * 96];
             bool id = true;
-            for (int b = 64; b < 96; b++) id &= jac[b] == 0;
+            for (int b = 64; b < 96; b++) id &= q[b] == 0;
             uint8_t *o = out + (first + m) * 64;
             if (id)
                 memset(o, 0, 64);
             else
-                memcpy(o, jac, 64);
+                memcpy(o, q, 64);
         }
     }
     return hipSuccess;
@@ -565,21 +668,32 @@ hipError_t witness(Job &j, hipStream_t st) {
     const Pk &pk = *j.pk;
     const uint64_t seed = j.seed;
     for (int c = 0; c < s.A; c++) PCHECK(zk_rand_fr(j.adv_lag[c], (int)s.n, seed * 1000003ull + c, st));
+    // instance columns: the public values of the usable rows (the caller's
+    // `instances` in the reference, padded with zeros to n, src/lib.rs:246-253)
+    for (int c = 0; c < s.I; c++) PCHECK(zk_rand_fr(j.inst_lag[c], (int)s.n, seed * 1000003ull + 0x1000 + c, st));
+    // the vanishing argument's random polynomial (src/buffer.rs:185-206),
+    // drawn as its values on the domain
+    PCHECK(zk_rand_fr(j.rnd_lag, (int)s.n, seed * 1000003ull + 0x2000, st));
     // gate: d = a * b + c on the usable rows
     PCHECK(zk::wit_gate_rows(j.adv_lag[3], j.adv_lag[0], j.adv_lag[1], j.adv_lag[2], s.u, st));
+    // coefficient gates: e_j = f_j * a on the usable rows
+    for (int g = 0; g < s.FG; g++)
+        PCHECK(zk::wit_mul_rows(j.adv_lag[s.col_fixed_gate + g], pk.fix_lag[g], j.adv_lag[s.col_gate], s.u, st));
     // lookups: inputs are table values on the usable rows
     for (int l = 0; l < s.L; l++) {
         PCHECK(zk::wit_lookup_values(j.vals, s.u, s.tsize, seed * 7919ull + l, st));
         PCHECK(zk::wit_u32_rows_to_fr(j.adv_lag[s.col_lookup + l], j.vals, s.u, st));
     }
-    // copy cycles: every cell of a cycle takes its head's value
+    // copy cycles over the permutation columns (instance first, then advice):
+    // every cell of a cycle takes its head's value
     if (s.P > 0) {
         std::vector<void *> cols(s.P);
-        for (int c = 0; c < s.P; c++) cols[c] = j.adv_lag[s.col_perm + c];
+        for (int c = 0; c < s.P; c++) cols[c] = c < s.I ? j.inst_lag[c] : j.adv_lag[s.col_perm + c - s.I];
         PCHECK(hipMemcpyAsync(j.ptrs, cols.data(), (size_t)s.P * 8, hipMemcpyHostToDevice, st));
         PCHECK(zk::wit_cycle_values(j.ptrs, pk.perm_head, (uint32_t)s.P, s.u, st));
         PCHECK(hipStreamSynchronize(st));  // cols is copied from host memory
     }
+    for (int c = 0; c < s.I; c++) PCHECK(zk::wit_zero_rows(j.inst_lag[c], s.u, s.n, st));
     // shuffles: the second column is a row permutation of the first on R rows,
     // both zero on [R, u)
     for (int t = 0; t < s.S; t++) {
@@ -616,29 +730,42 @@ std::vector<zkh::Fr> lagrange(const zkh::Fr *pts, const zkh::Fr *vals, int m) {
 }
 
 // ---- Fiat-Shamir (halo2's Blake2bWrite, csrc/transcript.cpp) -------------------
+// common_* absorb only; write_* also append to the proof: a point as its
+// compressed encoding (canonical x little-endian, the parity of canonical y
+// in the top bit of the last byte; the identity as 32 zero bytes), a scalar
+// as its canonical 32 little-endian bytes.
 struct Transcript {
     void *t = zk_transcript_new();
+    std::vector<uint8_t> *proof = nullptr;
     ~Transcript() { zk_transcript_free(t); }
     // affine commitments as commit() writes them (Montgomery x, y; zeros for
     // the identity, which halo2 cannot absorb: absorbed here as x = y = 0)
-    void points(const uint8_t *aff, int count) {
+    void points(const uint8_t *aff, int count, bool write) {
         for (int i = 0; i < count; i++) {
             zkh::Fp x, y;
             memcpy(&x, aff + (size_t)i * 64, 32);
             memcpy(&y, aff + (size_t)i * 64 + 32, 32);
             const zkh::Fp one_raw{{1, 0, 0, 0}};
-            if (!zkh::fp_is_zero(x) || !zkh::fp_is_zero(y)) {
+            const bool id = zkh::fp_is_zero(x) && zkh::fp_is_zero(y);
+            if (!id) {
                 x = zkh::fp_mul(x, one_raw);  // canonical
                 y = zkh::fp_mul(y, one_raw);
             }
             zk_transcript_common_point(t, &x, &y);
+            if (write && proof) {
+                uint8_t c[32];
+                memcpy(c, &x, 32);
+                if (!id && (y.l[0] & 1)) c[31] |= 0x80;
+                proof->insert(proof->end(), c, c + 32);
+            }
         }
     }
-    void scalars(const zkh::Fr *v, int count) {
+    void scalars(const zkh::Fr *v, int count, bool write) {
         const zkh::Fr one_raw{{1, 0, 0, 0}};
         for (int i = 0; i < count; i++) {
             const zkh::Fr c = zkh::fr_mul(v[i], one_raw);
             zk_transcript_common_scalar(t, &c);
+            if (write && proof) proof->insert(proof->end(), (const uint8_t *)&c, (const uint8_t *)&c + 32);
         }
     }
     zkh::Fr challenge() {
@@ -655,13 +782,87 @@ hipError_t put_challenges(Job &j, int first, const zkh::Fr *v, int count, hipStr
     return hipStreamSynchronize(st);
 }
 
+// The opening's polynomials with their evaluation order (src/lib.rs:1025-1078:
+// h first, then instance, advice, fixed, the random polynomial, sigma, the
+// permutation products at x, x w and -- all but the last -- x w^-(b+1), each
+// lookup's multiplicity and sum, each shuffle product) and query order
+// (src/lib.rs:1094-1176 with permutation_product_open, lookup_open and
+// shuffle_open of src/multiopen.rs:564-660).  h(x) is computed but not
+// written: the verifier derives it from the constraints.
+OpenLayout open_layout(const Job &j, const void *hpoly) {
+    const Pk &pk = *j.pk;
+    const Shape &s = pk.sh;
+    OpenLayout o;
+    auto add = [&](const void *p) {
+        o.poly.push_back(p);
+        return (int)o.poly.size() - 1;
+    };
+    const int h = add(hpoly);
+    std::vector<int> inst, adv, fix, sig, zp, m, zl, zs;
+    for (auto p : j.inst_coef) inst.push_back(add(p));
+    for (auto p : j.adv_coef) adv.push_back(add(p));
+    fix.push_back(add(pk.q_coef));
+    fix.push_back(add(pk.t_coef));
+    for (auto p : pk.fix_coef) fix.push_back(add(p));
+    const int rnd = add(j.rnd_coef);
+    for (auto p : pk.sig_coef) sig.push_back(add(p));
+    for (auto p : j.zp_coef) zp.push_back(add(p));
+    for (auto p : j.m_coef) m.push_back(add(p));
+    for (auto p : j.zl_coef) zl.push_back(add(p));
+    for (auto p : j.zs_coef) zs.push_back(add(p));
+    auto E = [&](int p, int r) { o.evals.push_back({p, r}); };
+    auto Q = [&](int p, int r) { o.queries.push_back({p, r}); };
+    E(h, ROT_CUR);
+    for (int p : inst) E(p, ROT_CUR);
+    for (int p : adv) E(p, ROT_CUR);
+    for (int p : fix) E(p, ROT_CUR);
+    E(rnd, ROT_CUR);
+    for (int p : sig) E(p, ROT_CUR);
+    for (size_t i = 0; i < zp.size(); i++) {
+        E(zp[i], ROT_CUR);
+        E(zp[i], ROT_NEXT);
+        if (i + 1 != zp.size()) E(zp[i], ROT_LAST);
+    }
+    for (int l = 0; l < s.L; l++) {
+        E(m[l], ROT_CUR);
+        E(zl[l], ROT_CUR);
+        E(zl[l], ROT_NEXT);
+    }
+    for (int p : zs) {
+        E(p, ROT_CUR);
+        E(p, ROT_NEXT);
+    }
+    for (int p : inst) Q(p, ROT_CUR);
+    for (int p : adv) Q(p, ROT_CUR);
+    for (int p : zp) {
+        Q(p, ROT_CUR);
+        Q(p, ROT_NEXT);
+    }
+    for (size_t i = zp.size(); i-- > 1;) Q(zp[i - 1], ROT_LAST);  // all but the last, in reverse
+    for (int l = 0; l < s.L; l++) {
+        Q(m[l], ROT_CUR);
+        Q(zl[l], ROT_CUR);
+        Q(zl[l], ROT_NEXT);
+    }
+    for (int p : zs) {
+        Q(p, ROT_CUR);
+        Q(p, ROT_NEXT);
+    }
+    for (int p : fix) Q(p, ROT_CUR);
+    for (int p : sig) Q(p, ROT_CUR);
+    Q(h, ROT_CUR);
+    Q(rnd, ROT_CUR);
+    return o;
+}
+
 // Multi-open of every query (src/multiopen.rs): evaluations of each
 // polynomial at its points (batch_poly_eval, src/eval_poly.rs:18), then GWC
-// (one quotient per point, :68-177) or Shplonk (h(X) and L(X), :283-561) with
+// (queries grouped by rotation, one quotient per group, :16-177) or Shplonk
+// (polynomials grouped by their rotation sets, h(X) and L(X), :178-561) with
 // the divisions by (X - z) on the device.  The challenges come from the
-// transcript.  open_ok: GWC, every batch polynomial evaluates at its point to the
-// v-combination of the evaluations; Shplonk, L(u) = 0 -- each holds only if
-// every evaluation and every division is right.
+// transcript.  open_ok: GWC, every batch polynomial evaluates at its point to
+// the v-combination of the evaluations; Shplonk, L(u) = 0 -- each holds only
+// if every evaluation and every division is right.
 hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, bool batched, bool tamper,
                       uint8_t *out, int &ncom, int &open_ok, int &nevals, hipStream_t st) {
     using namespace zkh;
@@ -676,48 +877,41 @@ hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, b
     void *ev_scratch = slot[15], *kate_tmp = slot[14];
     uint8_t *oc = (uint8_t *)j.open_cst;
     auto ocs = [&](int i) -> void * { return oc + (size_t)i * FR; };
-    std::vector<const void *> sets[3];
-    for (auto p : j.adv_coef) sets[0].push_back(p);
-    sets[0].push_back(pk.q_coef);
-    sets[0].push_back(pk.t_coef);
-    for (auto p : pk.sig_coef) sets[0].push_back(p);
-    for (auto p : j.m_coef) sets[0].push_back(p);
-    sets[0].push_back(hpoly);
-    for (auto p : j.zl_coef) sets[1].push_back(p);
-    for (auto p : j.zs_coef) sets[1].push_back(p);
-    if (s.Sp > 0) sets[1].push_back(j.zp_coef[s.Sp - 1]);
-    for (int t = 0; t + 1 < s.Sp; t++) sets[2].push_back(j.zp_coef[t]);
-    const int nsets = sets[2].empty() ? (sets[1].empty() ? 1 : 2) : 3;  // = number of distinct points
+    const OpenLayout lay = open_layout(j, hpoly);
+    const int npoly = (int)lay.poly.size();
 
-    // points x, x w, x w^u with w^u = (w^-1)^(blinding + 1)
+    // points: x w^-(blinding + 1), x, x w (the rotations last, cur, next)
     PCHECK(zk_fr_op(j.cf(J_XW), j.cf(J_X), pk.cf(C_WN), OP_MUL, 1, st));
     PCHECK(hipMemcpyAsync(j.cf(J_XWU), j.cf(J_X), FR, hipMemcpyDeviceToDevice, st));
     for (int i = 0; i <= s.blind; i++) PCHECK(zk_fr_op(j.cf(J_XWU), j.cf(J_XWU), pk.cf(C_WNINV), OP_MUL, 1, st));
-    const void *pts_dev[3] = {j.cf(J_X), j.cf(J_XW), j.cf(J_XWU)};
+    const void *pts_dev[ROT_COUNT] = {j.cf(J_XWU), j.cf(J_X), j.cf(J_XW)};
+    const int rot_value[ROT_COUNT] = {-(s.blind + 1), 0, 1};
 
-    // evaluations: set by set, polynomial by polynomial, point by point
-    int e = 0;
-    for (int set = 0; set < 3; set++)
-        for (auto p : sets[set])
-            for (int q = 0; q <= set; q++) {
-                PCHECK(poly_eval(p, ev_scratch, pts_dev[q], n, st));
-                PCHECK(hipMemcpyAsync((uint8_t *)j.evals + (size_t)e * FR, ev_scratch, FR, hipMemcpyDeviceToDevice,
-                                      st));
-                e++;
-            }
+    // evaluations in the reference's order
+    std::vector<int> eidx((size_t)npoly * ROT_COUNT, -1);
+    const int e = (int)lay.evals.size();
+    for (int i = 0; i < e; i++) {
+        const int p = lay.evals[i].first, r = lay.evals[i].second;
+        eidx[(size_t)p * ROT_COUNT + r] = i;
+        PCHECK(poly_eval(lay.poly[p], ev_scratch, pts_dev[r], n, st));
+        PCHECK(hipMemcpyAsync((uint8_t *)j.evals + (size_t)i * FR, ev_scratch, FR, hipMemcpyDeviceToDevice, st));
+    }
     nevals = e;
-    if (tamper)  // self-test of the opening check: one evaluation off by one
+    if (tamper)  // self-test of the opening check: one written evaluation off by one
         PCHECK(zk_fr_op((uint8_t *)j.evals + (size_t)(e - 1) * FR, (uint8_t *)j.evals + (size_t)(e - 1) * FR,
                         pk.cf(C_ONE), OP_ADD, 1, st));
     std::vector<Fr> ev(e);
-    Fr P[3];
+    Fr P[ROT_COUNT];
     PCHECK(hipMemcpyAsync(ev.data(), j.evals, (size_t)e * FR, hipMemcpyDeviceToHost, st));
-    for (int q = 0; q < 3; q++) PCHECK(hipMemcpyAsync(&P[q], pts_dev[q], FR, hipMemcpyDeviceToHost, st));
+    for (int q = 0; q < ROT_COUNT; q++) PCHECK(hipMemcpyAsync(&P[q], pts_dev[q], FR, hipMemcpyDeviceToHost, st));
     PCHECK(hipStreamSynchronize(st));
-    // the evaluations enter the transcript; GWC draws v, Shplonk y' then v
-    // (src/multiopen.rs:85, :302-305); Shplonk's u follows h's commitment
-    tr.scalars(ev.data(), e);
+    // every evaluation but h(x) is written (src/lib.rs:1083-1085); GWC then
+    // draws v, Shplonk y' then v
+    tr.scalars(ev.data() + 1, e - 1, true);
     j.last_evals = ev;
+    auto eval_of = [&](int p, int r) { return ev[(size_t)eidx[(size_t)p * ROT_COUNT + r]]; };
+    for (const auto &q : lay.queries)
+        if (eidx[(size_t)q.first * ROT_COUNT + q.second] < 0) return hipErrorUnknown;  // a query without its evaluation
     Fr v, y = fr_zero();
     if (shplonk) {
         y = tr.challenge();
@@ -731,127 +925,151 @@ hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, b
     j.last_open_ch[0] = v;
     j.last_open_ch[1] = y;
     j.last_open_ch[2] = fr_zero();
-    // Horner in c over the evaluations of `set` at point q, in query order
-    auto combine = [&](int set, int q, const Fr &c) {
-        int first = 0;
-        for (int t = 0; t < set; t++) first += (int)sets[t].size() * (t + 1);
-        Fr acc = fr_zero();
-        for (size_t i = 0; i < sets[set].size(); i++) acc = fr_add(fr_mul(acc, c), ev[first + i * (set + 1) + q]);
-        return acc;
-    };
 
     if (!shplonk) {
-        // one batch per point: every query at point q, sets q.. in order
+        // construct_intermediate_sets: queries grouped by rotation, groups in
+        // rotation order; a group's batch is sum_i v^(len-1-i) p_i
         std::vector<const void *> W;
-        std::vector<Fr> expect(nsets);
-        for (int q = 0; q < nsets; q++) {
-            void *B = slot[q];
-            bool first = true;
+        std::vector<Fr> expect;
+        for (int r = 0; r < ROT_COUNT; r++) {
+            std::vector<int> grp;
+            for (const auto &q : lay.queries)
+                if (q.second == r) grp.push_back(q.first);
+            if (grp.empty()) continue;
+            void *B = slot[W.size()];
             Fr acc = fr_zero();
-            for (int set = q; set < 3; set++) {
-                Fr cs = combine(set, q, v);
-                // the set's block of the Horner sum: acc v^len + (set's own sum)
-                for (size_t i = 0; i < sets[set].size(); i++) acc = fr_mul(acc, v);
-                acc = fr_add(acc, cs);
-                for (auto p : sets[set]) {
-                    if (first)
-                        PCHECK(hipMemcpyAsync(B, p, nb, hipMemcpyDeviceToDevice, st));
-                    else
-                        PCHECK(field_op(B, B, 0, j.cf(J_V), p, 0, nullptr, n, FO_ADD, st));
-                    first = false;
-                }
+            for (size_t i = 0; i < grp.size(); i++) {
+                if (i == 0)
+                    PCHECK(hipMemcpyAsync(B, lay.poly[grp[i]], nb, hipMemcpyDeviceToDevice, st));
+                else
+                    PCHECK(field_op(B, B, 0, j.cf(J_V), lay.poly[grp[i]], 0, nullptr, n, FO_ADD, st));
+                acc = fr_add(fr_mul(acc, v), eval_of(grp[i], r));
             }
-            expect[q] = acc;
-            PCHECK(zk_kate_divide(B, B, pts_dev[q], ocs(q), n, kate_tmp, st));
+            PCHECK(zk_kate_divide(B, B, pts_dev[r], ocs((int)W.size()), n, kate_tmp, st));
+            expect.push_back(acc);
             W.push_back(B);
         }
+        const int ng = (int)W.size();
         PCHECK(commit(j, pk.g, W, out + (size_t)ncom * 64, batched, st));
-        tr.points(out + (size_t)ncom * 64, nsets);
-        ncom += nsets;
-        std::vector<Fr> got(nsets);
-        PCHECK(hipMemcpyAsync(got.data(), oc, (size_t)nsets * FR, hipMemcpyDeviceToHost, st));
+        tr.points(out + (size_t)ncom * 64, ng, true);
+        ncom += ng;
+        std::vector<Fr> got(ng);
+        PCHECK(hipMemcpyAsync(got.data(), oc, (size_t)ng * FR, hipMemcpyDeviceToHost, st));
         PCHECK(hipStreamSynchronize(st));
         open_ok = 1;
-        for (int q = 0; q < nsets; q++) open_ok &= fr_eq(got[q], expect[q]) ? 1 : 0;
+        for (int q = 0; q < ng; q++) open_ok &= fr_eq(got[q], expect[q]) ? 1 : 0;
         return hipSuccess;
     }
 
-    // Shplonk.  V_s = y-combination of set s; r_s interpolates its evaluations
-    void *V[3] = {slot[0], slot[1], slot[2]};
-    void *hx = slot[3], *pb = slot[4], *tmp = slot[5], *fz = slot[6];
-    std::vector<Fr> r[3];
-    for (int set = 0; set < nsets; set++) {
-        bool first = true;
-        for (auto p : sets[set]) {
-            if (first)
-                PCHECK(hipMemcpyAsync(V[set], p, nb, hipMemcpyDeviceToDevice, st));
-            else
-                PCHECK(field_op(V[set], V[set], 0, j.cf(J_SY), p, 0, nullptr, n, FO_ADD, st));
-            first = false;
-        }
-        Fr ec[3];
-        for (int q = 0; q <= set; q++) ec[q] = combine(set, q, y);
-        r[set] = lagrange(P, ec, set + 1);
+    // Shplonk.  Each polynomial's rotation set, polynomials in order of first
+    // appearance among the queries; sets ordered as BTreeSet<Rotation>
+    // (lexicographic over the sorted rotations)
+    std::vector<int> mask(npoly, 0), order;
+    for (const auto &q : lay.queries) {
+        if (!mask[q.first]) order.push_back(q.first);
+        mask[q.first] |= 1 << q.second;
     }
-    // host staging of the scalars the device reads: [0, 9) r coefficients of
-    // the three sets, then (once u is drawn) per set u-evaluation and z_s,
-    // then Z_T(u), 1/z_0
-    std::vector<Fr> stage(kOpenCst - 32, fr_zero());
-    for (int set = 0; set < nsets; set++)
-        for (int c = 0; c <= set; c++) stage[3 * set + c] = r[set][c];
-    PCHECK(hipMemcpyAsync(ocs(32), stage.data(), 9 * FR, hipMemcpyHostToDevice, st));
-    // point arrays on the device: [8, 11) the points in order
-    for (int q = 0; q < nsets; q++) PCHECK(hipMemcpyAsync(ocs(8 + q), pts_dev[q], FR, hipMemcpyDeviceToDevice, st));
-    PCHECK(hipStreamSynchronize(st));  // the staged coefficients
-
-    PCHECK(hipMemsetAsync(hx, 0, nb, st));
-    for (int set = 0; set < nsets; set++) {
-        PCHECK(hipMemcpyAsync(pb, V[set], nb, hipMemcpyDeviceToDevice, st));
-        PCHECK(field_op(pb, pb, 0, nullptr, ocs(32 + 3 * set), 0, nullptr, set + 1, FO_SUB, st));
+    auto key_of = [&](int mk) {
+        std::vector<int> k;
+        for (int r = 0; r < ROT_COUNT; r++)
+            if (mk >> r & 1) k.push_back(rot_value[r]);
+        return k;
+    };
+    std::vector<std::pair<std::vector<int>, int>> keys;  // (sorted rotations, mask)
+    for (int p : order) {
+        bool seen = false;
+        for (auto &k : keys) seen |= k.second == mask[p];
+        if (!seen) keys.push_back({key_of(mask[p]), mask[p]});
+    }
+    std::sort(keys.begin(), keys.end());
+    const int nsets = (int)keys.size();
+    if (nsets > 4) return hipErrorNotSupported;
+    int super_mask = 0;
+    for (auto &k : keys) super_mask |= k.second;
+    std::vector<int> super_rots;  // the super point set, in rotation order
+    for (int r = 0; r < ROT_COUNT; r++)
+        if (super_mask >> r & 1) super_rots.push_back(r);
+    // staging: [8, 11) super points, [12 + 3 s) diffs of set s, [24 + 3 s) r_s
+    // coefficients, [36 + s) r_s(u), [40 + s) z_s, [44] Z_T(u), [45] 1/z_0
+    std::vector<Fr> stage(kOpenCst, fr_zero());
+    for (size_t i = 0; i < super_rots.size(); i++) stage[8 + i] = P[super_rots[i]];
+    std::vector<void *> V(nsets);
+    std::vector<std::vector<Fr>> r(nsets);
+    std::vector<std::vector<int>> set_rots(nsets), diffs(nsets);
+    for (int si = 0; si < nsets; si++) {
+        for (int rr = 0; rr < ROT_COUNT; rr++) {
+            if (keys[si].second >> rr & 1)
+                set_rots[si].push_back(rr);
+            else if (super_mask >> rr & 1)
+                diffs[si].push_back(rr);
+        }
+        for (size_t d = 0; d < diffs[si].size(); d++) stage[12 + 3 * si + d] = P[diffs[si][d]];
+        V[si] = slot[si];
+        bool first = true;
+        const int npts = (int)set_rots[si].size();
+        std::vector<Fr> ec(npts, fr_zero()), pts(npts);
+        for (int q = 0; q < npts; q++) pts[q] = P[set_rots[si][q]];
+        for (int p : order) {
+            if (mask[p] != keys[si].second) continue;
+            if (first)
+                PCHECK(hipMemcpyAsync(V[si], lay.poly[p], nb, hipMemcpyDeviceToDevice, st));
+            else
+                PCHECK(field_op(V[si], V[si], 0, j.cf(J_SY), lay.poly[p], 0, nullptr, n, FO_ADD, st));
+            first = false;
+            for (int q = 0; q < npts; q++) ec[q] = fr_add(fr_mul(ec[q], y), eval_of(p, set_rots[si][q]));
+        }
+        r[si] = lagrange(pts.data(), ec.data(), npts);
+        for (int c = 0; c < npts; c++) stage[24 + 3 * si + c] = r[si][c];
+    }
+    PCHECK(hipMemcpyAsync(ocs(8), stage.data() + 8, 28 * FR, hipMemcpyHostToDevice, st));
+    PCHECK(hipStreamSynchronize(st));  // the staged host values
+    void *hx = slot[4], *pb = slot[5], *tmp = slot[6], *fz = slot[7];
+    PCHECK(zk::fill_zero(hx, nb, st));
+    for (int si = 0; si < nsets; si++) {
+        PCHECK(hipMemcpyAsync(pb, V[si], nb, hipMemcpyDeviceToDevice, st));
+        PCHECK(field_op(pb, pb, 0, nullptr, ocs(24 + 3 * si), 0, nullptr, (int)set_rots[si].size(), FO_SUB, st));
         bool sw = false;
         PCHECK(ntt(pb, tmp, nullptr, pk.w_n, nullptr, s.k, 8, &sw, st));
-        // the points outside the set are ocs(8 + set + 1 ..)
-        PCHECK(zk_shplonk_h_x_merge_on(hx, j.cf(J_V), sw ? tmp : pb, pk.w_n, ocs(8 + set + 1), nsets - set - 1, n,
-                                       st));
+        PCHECK(zk_shplonk_h_x_merge_on(hx, j.cf(J_V), sw ? tmp : pb, pk.w_n, ocs(12 + 3 * si),
+                                       (int)diffs[si].size(), n, st));
     }
-    PCHECK(zk_shplonk_h_x_div_points_on(hx, pk.w_n, ocs(8), nsets, n, st));
+    PCHECK(zk_shplonk_h_x_div_points_on(hx, pk.w_n, ocs(8), (int)super_rots.size(), n, st));
     {
         bool sw = false;
         PCHECK(ntt(hx, tmp, nullptr, pk.w_n_inv, pk.cf(C_NINV), s.k, 8, &sw, st));
         if (sw) PCHECK(hipMemcpyAsync(hx, tmp, nb, hipMemcpyDeviceToDevice, st));
     }
     PCHECK(commit(j, pk.g, std::vector<const void *>{hx}, out + (size_t)ncom * 64, batched, st));
-    tr.points(out + (size_t)ncom * 64, 1);
+    tr.points(out + (size_t)ncom * 64, 1, true);
     ncom += 1;
     const Fr u = tr.challenge();
     PCHECK(put_challenges(j, J_U, &u, 1, st));
     j.last_open_ch[2] = u;
-    Fr zt = fr_one(), z0inv = fr_one();
-    for (int q = 0; q < nsets; q++) zt = fr_mul(zt, fr_sub(u, P[q]));
-    for (int set = 0; set < nsets; set++) {
+    Fr zt = fr_one();
+    for (int q : super_rots) zt = fr_mul(zt, fr_sub(u, P[q]));
+    for (int si = 0; si < nsets; si++) {
         Fr ru = fr_zero();  // r_s(u)
-        for (int c = set; c >= 0; c--) ru = fr_add(fr_mul(ru, u), r[set][c]);
-        Fr zs = fr_one();   // prod over the points outside the set
-        for (int q = set + 1; q < nsets; q++) zs = fr_mul(zs, fr_sub(u, P[q]));
-        stage[9 + 2 * set] = ru;
-        stage[10 + 2 * set] = zs;
-        if (set == 0) z0inv = fr_inv(zs);
+        for (int c = (int)r[si].size() - 1; c >= 0; c--) ru = fr_add(fr_mul(ru, u), r[si][c]);
+        Fr zs = fr_one();   // product over the points outside the set
+        for (int q : diffs[si]) zs = fr_mul(zs, fr_sub(u, P[q]));
+        stage[36 + si] = ru;
+        stage[40 + si] = zs;
     }
-    stage[15] = zt;
-    stage[16] = z0inv;
-    PCHECK(hipMemcpyAsync(ocs(32 + 9), stage.data() + 9, (stage.size() - 9) * FR, hipMemcpyHostToDevice, st));
+    stage[44] = zt;
+    stage[45] = fr_inv(stage[40]);
+    PCHECK(hipMemcpyAsync(ocs(36), stage.data() + 36, 10 * FR, hipMemcpyHostToDevice, st));
     PCHECK(hipStreamSynchronize(st));  // the staged scalars
     // L(X) = (sum_s v^. z_s (V_s - r_s(u)) - Z_T(u) h(X)) / z_0, then / (X - u)
-    PCHECK(hipMemsetAsync(fz, 0, nb, st));
-    for (int set = 0; set < nsets; set++) {
-        PCHECK(zk_fr_op(V[set], V[set], ocs(32 + 9 + 2 * set), OP_SUB, 1, st));
-        PCHECK(field_op(fz, fz, 0, j.cf(J_V), V[set], 0, ocs(32 + 10 + 2 * set), n, FO_ADD, st));
+    PCHECK(zk::fill_zero(fz, nb, st));
+    for (int si = 0; si < nsets; si++) {
+        PCHECK(zk_fr_op(V[si], V[si], ocs(36 + si), OP_SUB, 1, st));
+        PCHECK(field_op(fz, fz, 0, j.cf(J_V), V[si], 0, ocs(40 + si), n, FO_ADD, st));
     }
-    PCHECK(field_op(fz, fz, 0, nullptr, hx, 0, ocs(32 + 15), n, FO_SUB, st));
-    PCHECK(field_op(fz, fz, 0, ocs(32 + 16), nullptr, 0, nullptr, n, FO_LEFT, st));
+    PCHECK(field_op(fz, fz, 0, nullptr, hx, 0, ocs(44), n, FO_SUB, st));
+    PCHECK(field_op(fz, fz, 0, ocs(45), nullptr, 0, nullptr, n, FO_LEFT, st));
     PCHECK(zk_kate_divide(fz, fz, j.cf(J_U), ocs(0), n, kate_tmp, st));
     PCHECK(commit(j, pk.g, std::vector<const void *>{fz}, out + (size_t)ncom * 64, batched, st));
-    tr.points(out + (size_t)ncom * 64, 1);
+    tr.points(out + (size_t)ncom * 64, 1, true);
     ncom += 1;
     Fr lu;
     PCHECK(hipMemcpyAsync(&lu, ocs(0), FR, hipMemcpyDeviceToHost, st));
@@ -950,14 +1168,50 @@ hipError_t zkp_pk_destroy(void *pk) {
     return e;
 }
 
-// Commitments a proof writes at most: advice, m, logup z, permutation z,
-// shuffle z, quotient pieces, then the opening (GWC: one per distinct point;
+// Commitments a proof produces at most: instance (absorbed, not written),
+// advice, m, permutation z, logup z, shuffle z, the random polynomial,
+// quotient pieces, then the opening (GWC: one per distinct rotation;
 // Shplonk: h and L).
 int zkp_commitment_count(const zkp_circuit *c) {
     Shape s;
     if (!make_shape(c, s)) return 0;
     const int open = open_gwc_points(s) > 2 ? open_gwc_points(s) : 2;
-    return s.A + 2 * s.L + s.Sp + s.S + kQuot + open;
+    return s.I + s.A + 2 * s.L + s.Sp + s.S + 1 + kQuot + open;
+}
+
+// The last proof's bytes: every written point (compressed, 32 bytes) and
+// scalar (32 bytes) in transcript order.  Returns the length; copies it when
+// out has room for it (max bytes), else returns -1.  out = null: length only.
+long long zkp_proof_bytes(void *job, void *out, long long max) {
+    if (!job) return -1;
+    const Job &j = *(const Job *)job;
+    const long long len = (long long)j.proof.size();
+    if (out) {
+        if (max < len) return -1;
+        memcpy(out, j.proof.data(), j.proof.size());
+    }
+    return len;
+}
+
+// The last proof's instance columns (Lagrange values, n Fr each, Montgomery):
+// the public input the verifier commits itself.  rows: room per column.
+hipError_t zkp_proof_instance(void *job, void *out, int rows) {
+    if (!job || !out) return hipErrorInvalidValue;
+    const Job &j = *(const Job *)job;
+    const Shape &s = j.pk->sh;
+    if (rows < (int)s.n) return hipErrorInvalidValue;
+    for (int c = 0; c < s.I; c++)
+        PCHECK(hipMemcpy((uint8_t *)out + (size_t)c * rows * FR, j.inst_lag[c], (size_t)s.n * FR,
+                            hipMemcpyDeviceToHost));
+    return hipSuccess;
+}
+
+// The verifying key's transcript representation (Montgomery Fr), the
+// first scalar every proof absorbs.
+hipError_t zkp_pk_vk_repr(void *pk, void *out32) {
+    if (!pk || !out32) return hipErrorInvalidValue;
+    memcpy(out32, &((const Pk *)pk)->vk_repr, FR);
+    return hipSuccess;
 }
 
 hipError_t zkp_job_create(void *pk, void **job_out) {
@@ -1010,6 +1264,7 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         const Shape &s = pk.sh;
         hipStream_t st = stream;
         j.seed = seed;
+        j.proof.clear();
         zkp_proof_info inf;
         memset(&inf, 0, sizeof(inf));
         uint8_t *out = (uint8_t *)commitments;
@@ -1024,10 +1279,13 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         };
 
         // Fiat-Shamir: commitments and evaluations are absorbed in the order the
-        // reference writes them; every challenge is squeezed from the transcript
+        // reference writes them; every challenge is squeezed from the transcript,
+        // which starts from the verifying key (vk.hash_into, src/lib.rs:242)
         Transcript tr;
+        tr.proof = &j.proof;
+        tr.scalars(&pk.vk_repr, 1, false);
 
-        // ---- 0. witness (the caller's advice in the reference) -----------------
+        // ---- 0. witness (the caller's advice and instances in the reference) ----
         PCHECK(witness(j, st));
         if (flags & ZKP_TAMPER_GATE) {
             // self-test of the quotient check: one usable row breaks the gate
@@ -1037,14 +1295,23 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         PCHECK(mark(0));
         const double t_start = tp;
 
-        // ---- 1. advice commitments (Lagrange basis) and iNTT --------------------
+        // ---- 1. advice and instance commitments (Lagrange basis) and iNTT -------
+        // one batch over advice then instances (batch_msm_and_intt_ext,
+        // src/lib.rs:320-379); the instance commitments are absorbed first and
+        // not written (common_point), the advice ones written (:380-385)
         {
             std::vector<const void *> polys(j.adv_lag.begin(), j.adv_lag.end());
-            PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
-            tr.points(out + (size_t)ncom * 64, s.A);
-            ncom += s.A;
+            for (auto p : j.inst_lag) polys.push_back(p);
+            std::vector<uint8_t> com(polys.size() * 64);
+            PCHECK(commit(j, pk.g_lag, polys, com.data(), batched != 0, st));
+            memcpy(out, com.data() + (size_t)s.A * 64, (size_t)s.I * 64);
+            memcpy(out + (size_t)s.I * 64, com.data(), (size_t)s.A * 64);
+            tr.points(out, s.I, false);
+            tr.points(out + (size_t)s.I * 64, s.A, true);
+            ncom += s.I + s.A;
             (void)tr.challenge();  // theta (src/lib.rs:395): this circuit's lookups are single columns
             PCHECK(intt_all(j, j.adv_lag, j.adv_coef, j.adv_tmp, st));
+            PCHECK(intt_all(j, j.inst_lag, j.inst_coef, j.inst_tmp, st));
         }
         PCHECK(mark(1));
 
@@ -1063,7 +1330,7 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         if (s.L) {
             std::vector<const void *> polys(j.m_lag.begin(), j.m_lag.end());
             PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
-            tr.points(out + (size_t)ncom * 64, s.L);
+            tr.points(out + (size_t)ncom * 64, s.L, true);
             ncom += s.L;
             PCHECK(intt_all(j, j.m_lag, j.m_coef, j.m_tmp, st));
         }
@@ -1080,7 +1347,7 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         for (int l = 0; l < s.L; l++) {
             PCHECK(logup_sum_input_inv(j.w_sum, j.adv_lag[s.col_lookup + l], j.w_temp, beta, 0, (int)s.n, st));
             PCHECK(hipMemcpyAsync(j.w_tab, pk.t_lag, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
-            PCHECK(hipMemsetAsync(j.cf(J_LASTZ), 0, FR, st));
+            PCHECK(zk::fill_zero(j.cf(J_LASTZ), FR, st));
             PCHECK(eval_logup_z_compose(j.zl_lag[l], j.w_sum, j.w_tab, j.m_lag[l], beta, j.cf(J_LASTZ), (int)s.u,
                                         (int)s.n, st));
         }
@@ -1088,14 +1355,20 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         // its rotations, the permutation sets, the permutation columns' cosets and
         // the shuffle inputs' / tables' cosets; shuffle z inputs reuse the start
         void **pa = j.ptrs;
-        void **pa_gate = pa + 2 * s.P + s.Sp, **pa_sets = pa_gate + 16 + 8;
+        void **pa_gate = pa + 2 * s.P + s.Sp;
+        int *pa_rots = (int *)(pa_gate + gate_list_len(s));
+        void **pa_sets = pa_gate + gate_list_len(s) + (gate_rot_len(s) + 1) / 2;
         void **pa_pv = pa_sets + s.Sp, **pa_ps = pa_pv + s.P;
         void **pa_sh = pa_ps + s.P;
-        int *pa_rots = (int *)(pa_gate + 16);
+        // permutation columns: instance first, then the advice ones
+        auto perm_col = [&](int c, bool coef) -> void * {
+            if (c < s.I) return coef ? j.inst_coef[c] : j.inst_lag[c];
+            return coef ? j.adv_coef[s.col_perm + c - s.I] : j.adv_lag[s.col_perm + c - s.I];
+        };
         if (s.P > 0) {
             std::vector<void *> host(2 * s.P + s.Sp);
             for (int c = 0; c < s.P; c++) {
-                host[c] = j.adv_lag[s.col_perm + c];
+                host[c] = perm_col(c, false);
                 host[s.P + c] = pk.sig_lag[c];
             }
             for (int t = 0; t < s.Sp; t++) host[2 * s.P + t] = j.zp_lag[t];
@@ -1115,77 +1388,108 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
             PCHECK(hipStreamSynchronize(st));
         }
         {
-            // permutation, lookup, shuffle: the reference's transcript order
-            // (src/lib.rs:968-980)
+            // permutation, lookup, shuffle, then the vanishing argument's random
+            // polynomial: the reference's transcript order (src/lib.rs:968-980)
             std::vector<const void *> polys;
             for (auto p : j.zp_lag) polys.push_back(p);
             for (auto p : j.zl_lag) polys.push_back(p);
             for (auto p : j.zs_lag) polys.push_back(p);
-            if (!polys.empty()) PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
-            tr.points(out + (size_t)ncom * 64, (int)polys.size());
+            polys.push_back(j.rnd_lag);
+            PCHECK(commit(j, pk.g_lag, polys, out + (size_t)ncom * 64, batched != 0, st));
+            tr.points(out + (size_t)ncom * 64, (int)polys.size(), true);
             ncom += (int)polys.size();
             const zkh::Fr yc = tr.challenge();  // y (src/lib.rs:988)
             PCHECK(put_challenges(j, J_Y, &yc, 1, st));
             memcpy(inf.challenges[2], &yc, FR);
+            // the gates' y powers (the gates fold as h = h y + gate)
+            {
+                std::vector<zkh::Fr> yp(2 * (s.FG + 1));
+                zkh::Fr acc = zkh::fr_one();
+                for (int i = 0; i <= s.FG; i++) {
+                    yp[i] = acc;
+                    yp[s.FG + 1 + i] = zkh::fr_neg(acc);
+                    acc = zkh::fr_mul(acc, yc);
+                }
+                PCHECK(put_challenges(j, j.j_ypow(0), yp.data(), (int)yp.size(), st));
+            }
             // closure of every grand product at row u: logup 0, permutation (last
             // set) 1, shuffle 1
             int bad = 0;  // bit 0: a logup sum, bit 1: the permutation, bit 2: a shuffle
-            uint64_t one[4], got[4];
+            uint64_t one[4];
+            std::vector<uint64_t> got((size_t)(s.L + s.S + 1) * 4, 0);
             PCHECK(hipMemcpyAsync(one, pk.cf(C_ONE), FR, hipMemcpyDeviceToHost, st));
+            int gi = 0;
+            for (auto z : j.zl_lag)
+                PCHECK(hipMemcpyAsync(&got[(size_t)4 * gi++], (const uint8_t *)z + (size_t)s.u * FR, FR,
+                                      hipMemcpyDeviceToHost, st));
+            for (auto z : j.zs_lag)
+                PCHECK(hipMemcpyAsync(&got[(size_t)4 * gi++], (const uint8_t *)z + (size_t)s.u * FR, FR,
+                                      hipMemcpyDeviceToHost, st));
+            if (s.Sp)
+                PCHECK(hipMemcpyAsync(&got[(size_t)4 * gi], (const uint8_t *)j.zp_lag[s.Sp - 1] + (size_t)s.u * FR,
+                                      FR, hipMemcpyDeviceToHost, st));
             PCHECK(hipStreamSynchronize(st));
-            auto at_u = [&](const void *z) -> hipError_t {
-                PCHECK(hipMemcpyAsync(got, (const uint8_t *)z + (size_t)s.u * FR, FR, hipMemcpyDeviceToHost, st));
-                return hipStreamSynchronize(st);
-            };
-            for (auto z : j.zl_lag) {
-                PCHECK(at_u(z));
-                if (got[0] | got[1] | got[2] | got[3]) bad |= 1;
+            for (int l = 0; l < s.L; l++) {
+                const uint64_t *g = &got[(size_t)4 * l];
+                if (g[0] | g[1] | g[2] | g[3]) bad |= 1;
             }
-            if (s.Sp) {
-                PCHECK(at_u(j.zp_lag[s.Sp - 1]));
-                if (memcmp(got, one, FR) != 0) bad |= 2;
-            }
-            for (auto z : j.zs_lag) {
-                PCHECK(at_u(z));
-                if (memcmp(got, one, FR) != 0) bad |= 4;
-            }
+            for (int t = 0; t < s.S; t++)
+                if (memcmp(&got[(size_t)4 * (s.L + t)], one, FR) != 0) bad |= 4;
+            if (s.Sp && memcmp(&got[(size_t)4 * (s.L + s.S)], one, FR) != 0) bad |= 2;
             inf.z_ok = bad ? 0 : 1;
             inf.z_fail = bad;
             PCHECK(intt_all(j, j.zl_lag, j.zl_coef, j.zl_tmp, st));
             PCHECK(intt_all(j, j.zp_lag, j.zp_coef, j.zp_tmp, st));
             PCHECK(intt_all(j, j.zs_lag, j.zs_coef, j.zs_tmp, st));
+            std::vector<void *> rl{j.rnd_lag}, rc{j.rnd_coef}, rt{j.rnd_tmp};
+            PCHECK(intt_all(j, rl, rc, rt, st));
+            j.rnd_coef = rc[0];
+            j.rnd_tmp = rt[0];
         }
         PCHECK(mark(3));
 
         // ---- 4. h(X) on the extended coset ----------------------------------------
-        // ext slots: gate a,b,c,d | f_l | m_l | zl_l | perm v | zp | shuffle in, sh | zs
+        // ext slots: gate a,b,c,d | f_l | m_l | zl_l | perm columns (instance,
+        // advice) | zp | shuffle in, sh | zs | coefficient-gate outputs e_j
         const int E_GATE = 0, E_F = 4, E_M = E_F + s.L, E_ZL = E_M + s.L, E_V = E_ZL + s.L, E_ZP = E_V + s.P,
-                  E_IN = E_ZP + s.Sp, E_ZS = E_IN + 2 * s.S;
+                  E_IN = E_ZP + s.Sp, E_ZS = E_IN + 2 * s.S, E_EG = E_ZS + s.S;
         {
             std::vector<const void *> c;
             for (int i = 0; i < 4; i++) c.push_back(j.adv_coef[s.col_gate + i]);
             for (int l = 0; l < s.L; l++) c.push_back(j.adv_coef[s.col_lookup + l]);
             for (auto p : j.m_coef) c.push_back(p);
             for (auto p : j.zl_coef) c.push_back(p);
-            for (int i = 0; i < s.P; i++) c.push_back(j.adv_coef[s.col_perm + i]);
+            for (int i = 0; i < s.P; i++) c.push_back(perm_col(i, true));
             for (auto p : j.zp_coef) c.push_back(p);
             for (int t = 0; t < 2 * s.S; t++) c.push_back(j.adv_coef[s.col_shuffle + t]);
             for (auto p : j.zs_coef) c.push_back(p);
+            for (int g = 0; g < s.FG; g++) c.push_back(j.adv_coef[s.col_fixed_gate + g]);
             PCHECK(extend_all(j, c, 0, st));
         }
         void *y = j.cf(J_Y);
         const int rot1 = 1 << kExtBits;  // w X on the coset
         const int N = (int)s.N;
-        PCHECK(hipMemsetAsync(j.h, 0, (size_t)s.N * FR, st));
-        // gate: q a b + q c - q d
+        PCHECK(zk::fill_zero(j.h, (size_t)s.N * FR, st));
+        // gates, folded in y: y^FG q (a b + c - d) + sum_j y^(FG-1-j) q (f_j a - e_j)
         {
-            void *hv[13] = {pk.cf(C_ONE), pk.q_ext, j.ext[E_GATE], j.ext[E_GATE + 1], nullptr,
-                            pk.cf(C_ONE), pk.q_ext, j.ext[E_GATE + 2], nullptr,
-                            pk.cf(C_MONE), pk.q_ext, j.ext[E_GATE + 3], nullptr};
-            int rots[7] = {0, 0, 0, 0, 0, 0, 0};
-            PCHECK(hipMemcpyAsync(pa_gate, hv, sizeof(hv), hipMemcpyHostToDevice, st));
-            PCHECK(hipMemcpyAsync(pa_rots, rots, sizeof(rots), hipMemcpyHostToDevice, st));
-            PCHECK(field_op_batch_mul_sum(j.h, pa_gate, pa_rots, 13, N, st));
+            std::vector<void *> hv;
+            auto term = [&](void *coef, std::initializer_list<void *> cols) {
+                hv.push_back(coef);
+                for (void *c : cols) hv.push_back(c);
+                hv.push_back(nullptr);
+            };
+            term(j.cf(j.j_ypow(s.FG)), {pk.q_ext, j.ext[E_GATE], j.ext[E_GATE + 1]});
+            term(j.cf(j.j_ypow(s.FG)), {pk.q_ext, j.ext[E_GATE + 2]});
+            term(j.cf(j.j_mypow(s.FG)), {pk.q_ext, j.ext[E_GATE + 3]});
+            for (int g = 0; g < s.FG; g++) {
+                term(j.cf(j.j_ypow(s.FG - 1 - g)), {pk.q_ext, pk.fix_ext[g], j.ext[E_GATE]});
+                term(j.cf(j.j_mypow(s.FG - 1 - g)), {pk.q_ext, j.ext[E_EG + g]});
+            }
+            if ((int)hv.size() != gate_list_len(s)) return hipErrorUnknown;
+            std::vector<int> rots(gate_rot_len(s), 0);
+            PCHECK(hipMemcpyAsync(pa_gate, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, st));
+            PCHECK(hipMemcpyAsync(pa_rots, rots.data(), rots.size() * 4, hipMemcpyHostToDevice, st));
+            PCHECK(field_op_batch_mul_sum(j.h, pa_gate, pa_rots, (int)hv.size(), N, st));
             PCHECK(hipStreamSynchronize(st));
         }
         // permutation
@@ -1246,7 +1550,7 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
             void *c = sw ? j.h_tmp : j.h;
             PCHECK(extended_prepare(c, pk.cf(C_INV0), 3, (int)s.n, N, 1, st));
             // all constraints hold iff h is a polynomial of degree < 3n
-            PCHECK(hipMemsetAsync(j.flag, 0, 4, st));
+            PCHECK(zk::fill_zero(j.flag, 16, st));
             PCHECK(zk::wit_any_nonzero(c, (uint64_t)kQuot * s.n, s.N, j.flag, st));
             uint32_t flag = 1;
             PCHECK(hipMemcpyAsync(&flag, j.flag, 4, hipMemcpyDeviceToHost, st));
@@ -1255,7 +1559,7 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
             std::vector<const void *> pieces;
             for (int i = 0; i < kQuot; i++) pieces.push_back((uint8_t *)c + (size_t)i * s.n * FR);
             PCHECK(commit(j, pk.g, pieces, out + (size_t)ncom * 64, batched != 0, st));
-            tr.points(out + (size_t)ncom * 64, kQuot);
+            tr.points(out + (size_t)ncom * 64, kQuot, true);
             ncom += kQuot;
             const zkh::Fr xc = tr.challenge();  // x
             PCHECK(put_challenges(j, J_X, &xc, 1, st));

@@ -163,17 +163,12 @@ int zk_transcript_common_scalar(void *t, const void *s) {
     b->update((const uint8_t *)s, 32);
     return 0;
 }
-// challenge: Montgomery-form Fr (4 x u64) in mont_out, the 64 digest bytes
-// in digest64 (either may be null)
-int zk_transcript_squeeze(void *t, void *mont_out, void *digest64) {
-    if (!t) return -1;
-    const uint8_t tag = 0;
-    Blake2b *b = (Blake2b *)t;
-    b->update(&tag, 1);
-    uint8_t d[64];
-    b->final(d);
-    if (digest64) memcpy(digest64, d, 64);
-    if (mont_out) {
+// 64 little-endian bytes reduced mod r (halo2's Fr::from_bytes_wide), as a
+// Montgomery-form Fr (4 x u64) in mont_out
+int zk_fr_from_bytes_wide(const void *in64, void *mont_out) {
+    if (!in64 || !mont_out) return -1;
+    const uint8_t *d = (const uint8_t *)in64;
+    {
         zkh::Fr lo, hi;
         for (int i = 0; i < 4; i++) {
             uint64_t a = 0, c = 0;
@@ -189,6 +184,20 @@ int zk_transcript_squeeze(void *t, void *mont_out, void *digest64) {
         zkh::Fr v = zkh::fr_add(zkh::fr_mul(lo, r2), zkh::fr_mul(hi, r3));
         memcpy(mont_out, v.l, 32);
     }
+    return 0;
+}
+
+// challenge: Montgomery-form Fr (4 x u64) in mont_out, the 64 digest bytes
+// in digest64 (either may be null)
+int zk_transcript_squeeze(void *t, void *mont_out, void *digest64) {
+    if (!t) return -1;
+    const uint8_t tag = 0;
+    Blake2b *b = (Blake2b *)t;
+    b->update(&tag, 1);
+    uint8_t d[64];
+    b->final(d);
+    if (digest64) memcpy(digest64, d, 64);
+    if (mont_out) zk_fr_from_bytes_wide(d, mont_out);
     return 0;
 }
 

@@ -45,6 +45,11 @@ __global__ void k_gate_rows(Fr *d, const Fr *__restrict__ a, const Fr *__restric
     ROWS(i, u) fe_store<FrParams>(&d[i], fe_load<FrParams>(&a[i]) * fe_load<FrParams>(&b[i]) + fe_load<FrParams>(&c[i]));
 }
 
+// d[i] = x[i] * y[i] for i < u (the fixed-coefficient gates' outputs)
+__global__ void k_mul_rows(Fr *d, const Fr *__restrict__ x, const Fr *__restrict__ y, uint32_t u) {
+    ROWS(i, u) fe_store<FrParams>(&d[i], fe_load<FrParams>(&x[i]) * fe_load<FrParams>(&y[i]));
+}
+
 // out[i] = 1 on [lo, hi), 0 elsewhere
 __global__ void k_indicator(Fr *out, uint32_t n, uint32_t lo, uint32_t hi) {
     ROWS(i, n) fe_store<FrParams>(&out[i], (i >= lo && i < hi) ? fe_one<FrParams>() : fe_zero<FrParams>());
@@ -121,6 +126,11 @@ hipError_t wit_table_rows(void *col, uint32_t u, uint32_t tsize, hipStream_t s) 
 hipError_t wit_gate_rows(void *d, const void *a, const void *b, const void *c, uint32_t u, hipStream_t s) {
     if (!u) return hipSuccess;
     k_gate_rows<<<blocks_for(u), kT, 0, s>>>((Fr *)d, (const Fr *)a, (const Fr *)b, (const Fr *)c, u);
+    return hipGetLastError();
+}
+hipError_t wit_mul_rows(void *d, const void *x, const void *y, uint32_t u, hipStream_t s) {
+    if (!u) return hipSuccess;
+    k_mul_rows<<<blocks_for(u), kT, 0, s>>>((Fr *)d, (const Fr *)x, (const Fr *)y, u);
     return hipGetLastError();
 }
 hipError_t wit_indicator(void *out, uint32_t n, uint32_t lo, uint32_t hi, hipStream_t s) {

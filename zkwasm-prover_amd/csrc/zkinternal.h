@@ -6,11 +6,15 @@
 
 namespace zk {
 
+// zero-fill by kernel (no memset node in a captured graph) (ntt.hip)
+hipError_t fill_zero(void *p, size_t bytes, hipStream_t stream);
+
 // witness synthesis and helpers for the prover core (witness.hip)
 hipError_t wit_u32_rows_to_fr(void *col, const uint32_t *v, uint32_t rows, hipStream_t s);
 hipError_t wit_lookup_values(uint32_t *vals, uint32_t u, uint32_t tsize, uint64_t seed, hipStream_t s);
 hipError_t wit_table_rows(void *col, uint32_t u, uint32_t tsize, hipStream_t s);
 hipError_t wit_gate_rows(void *d, const void *a, const void *b, const void *c, uint32_t u, hipStream_t s);
+hipError_t wit_mul_rows(void *d, const void *x, const void *y, uint32_t u, hipStream_t s);
 hipError_t wit_indicator(void *out, uint32_t n, uint32_t lo, uint32_t hi, hipStream_t s);
 hipError_t wit_zero_rows(void *col, uint32_t lo, uint32_t hi, hipStream_t s);
 hipError_t wit_gather(void *dst, const void *src, const uint32_t *idx, uint32_t rows, hipStream_t s);

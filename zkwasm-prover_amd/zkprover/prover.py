@@ -65,6 +65,10 @@ def _decl():
     L.zkp_pk_fixed_commitments.restype = ctypes.c_int
     L.zkp_proof_openings.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     L.zkp_proof_openings.restype = ctypes.c_int
+    L.zkp_proof_bytes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong]
+    L.zkp_proof_bytes.restype = ctypes.c_longlong
+    L.zkp_proof_instance.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    L.zkp_pk_vk_repr.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     return L
 
 
@@ -88,9 +92,16 @@ class ProvingKey:
         check(self.L.zkp_pk_srs_secret(self.handle, out.ctypes.data), "zkp_pk_srs_secret")
         return out
 
+    def vk_repr(self):
+        """the verifying key's transcript representation (Montgomery Fr, (1, 4) u64)"""
+        out = np.zeros((1, 4), dtype=np.uint64)
+        check(self.L.zkp_pk_vk_repr(self.handle, out.ctypes.data), "zkp_pk_vk_repr")
+        return out
+
     def fixed_commitments(self):
-        """verifying-key commitments of q, t and the sigma columns, (2 + P, 8) affine"""
-        out = np.zeros((2 + self.circuit.perm_cols, 8), dtype=np.uint64)
+        """verifying-key commitments of the fixed columns (q, t, f_j) and the
+        sigma columns, (fixed + perm_cols, 8) affine"""
+        out = np.zeros((self.circuit.fixed + self.circuit.perm_cols, 8), dtype=np.uint64)
         cnt = self.L.zkp_pk_fixed_commitments(self.handle, out.ctypes.data, len(out))
         if cnt < 0:
             raise HipError("zkp_pk_fixed_commitments")
@@ -110,7 +121,9 @@ class Prover:
         check(self.L.zkp_job_create(pk.handle, ctypes.byref(self.handle)), "zkp_job_create")
 
     def prove(self, seed, flags=BATCHED, stream=0, want_pieces=True, pieces_out=None):
-        """returns (commitments (ncommit, 8) u64 affine, h_pieces (n, 4) or None, ProofInfo);
+        """returns (commitments (ncommit, 8) u64 affine in transcript order -- instance,
+        advice, m, permutation z, logup z, shuffle z, random, quotient pieces, opening --
+        h_pieces (n, 4) or None, ProofInfo);
         pieces_out: a caller's (n, 4) u64 array to receive h_pieces (e.g. pinned memory)"""
         n = 1 << self.pk.circuit.k
         com = np.zeros((self.pk.ncommit, 8), dtype=np.uint64)
@@ -126,9 +139,27 @@ class Prover:
               "zkp_prove")
         return com, pieces, info
 
+    def proof_bytes(self):
+        """the last proof's bytes (written points compressed, written scalars)"""
+        ln = self.L.zkp_proof_bytes(self.handle, None, 0)
+        if ln < 0:
+            raise HipError("zkp_proof_bytes")
+        out = np.zeros(max(ln, 1), dtype=np.uint8)
+        if self.L.zkp_proof_bytes(self.handle, out.ctypes.data, len(out)) != ln:
+            raise HipError("zkp_proof_bytes")
+        return out[:ln].tobytes()
+
+    def instance(self):
+        """the last proof's instance columns, (instance, n, 4) Montgomery"""
+        n = 1 << self.pk.circuit.k
+        out = np.zeros((max(self.pk.circuit.instance, 1), n, 4), dtype=np.uint64)
+        check(self.L.zkp_proof_instance(self.handle, out.ctypes.data, n), "zkp_proof_instance")
+        return out[:self.pk.circuit.instance]
+
     def openings(self):
-        """the last proof's evaluations ((count, 4) Montgomery, query order)
-        and its opening challenges [v, y', u] ((3, 4))"""
+        """the last proof's evaluations ((count, 4) Montgomery; h(x) first, then
+        the written ones in the reference's order) and its opening challenges
+        [v, y', u] ((3, 4))"""
         cnt = self.L.zkp_proof_openings(self.handle, None, 0, None)
         ev = np.zeros((max(cnt, 1), 4), dtype=np.uint64)
         ch = np.zeros((3, 4), dtype=np.uint64)
