@@ -593,19 +593,88 @@ def _pmc_traffic(kernel):
     return None
 
 
+def _cpu_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
 def cpu_baseline(n):
-    """The C oracle's Pippenger MSM on host cores, same shape (n = 2^22)."""
+    """The C oracle on the host cores this job is given (OMP_NUM_THREADS, else
+    every usable CPU), one full run of each primitive at the zkWasm shape:
+    G1 MSM n = 2^22 (Pippenger parallel over point ranges x windows), Fr NTT
+    2^22, coset NTT 2^22 -> 2^24, batch inversion 2^22 (parallel chunks); and
+    a lower bound for one k=20 benchmark proof's arithmetic: the proof's
+    primitive counts times the oracle's time for each primitive at k=20 (it
+    leaves out the h(X) evaluation, the grand products' element-wise passes
+    and the openings' scans)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import zkoracle as zo
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
-    zo.set_threads(cores)
-    pts = zo.gen_bases_ap(n, 4)
-    sc = zo.rand_fr(n, 3)
-    t0 = time.perf_counter()
-    zo.msm(pts, sc)
-    dt = time.perf_counter() - t0
-    return {"value": round(dt * 1e3, 1), "unit": "ms (G1 MSM, n=2^22)", "cores": cores, "kind": "port",
-            "sample": "one full n=2^22 MSM (C oracle Pippenger, unsigned windows, one thread per window)"}
+    info = _cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["usable_cpus"] or 1
+    zo.set_threads(threads)
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return (time.perf_counter() - t0) * 1e3
+
+    def prims(log_n):
+        m = 1 << log_n
+        pts = zo.gen_bases_ap(m, 4)
+        sc = zo.rand_fr(m, 3)
+        x = zo.rand_fr(m, 5)
+        w = zo.root_of_unity(log_n)
+        we = zo.root_of_unity(log_n + EXT)
+        zeta = pow(7, (_R - 1) // 3, _R)
+        cp = zo.to_mont([zeta, zeta * zeta % _R])
+        ext = zo.fr_zeros(m << EXT)
+        inv_in = zo.rand_fr_nonzero(m, 7)
+        L = zo.lib()
+        xa = np.array(x, copy=True)
+        out = {"msm": timed(lambda: zo.msm(pts, sc)),
+               "ntt": timed(lambda: L.zo_ntt(zo._p(xa), ctypes.c_size_t(log_n), zo._p(w), None))}
+        ext[:m] = x
+
+        def coset():
+            # in place: scale by the coset powers, then the 2^(log_n + EXT) NTT
+            L.zo_extended_prepare(zo._p(ext), zo._p(cp), ctypes.c_int(3), ctypes.c_size_t(m),
+                                  ctypes.c_size_t(m << EXT), ctypes.c_int(1))
+            L.zo_ntt(zo._p(ext), ctypes.c_size_t(log_n + EXT), zo._p(we), None)
+        out["coset"] = timed(coset)
+        out["batch_inversion"] = timed(lambda: L.zo_batch_invert(zo._p(inv_in), ctypes.c_size_t(m)))
+        return out
+
+    p22 = prims(LOG_N)
+    p20 = prims(20)
+    from zkprover.prover import Circuit
+    c = Circuit.benchmark(20)
+    Sp = (c.perm_cols + 1) // 2
+    counts = {"msm": c.instance + c.advice + 2 * c.lookups + Sp + c.shuffles + 1 + 3 + 3,
+              "ntt": c.instance + c.advice + 2 * c.lookups + Sp + c.shuffles + 1,
+              "coset": 4 + 3 * c.lookups + c.perm_cols + Sp + 3 * c.shuffles + (c.fixed - 2),
+              "batch_inversion": 2 * c.lookups + Sp + c.shuffles}
+    est = sum(counts[k] * p20[k] for k in counts)
+    return {"value": round(p22["msm"], 1), "unit": "ms (G1 MSM, n=2^22)", "cores": threads, "kind": "port",
+            "cpu": info,
+            "sample": "one full run of each primitive at n=2^22 on the C oracle (the MSM is `value`)",
+            "primitives_ms": {"msm_2^22": round(p22["msm"], 1), "ntt_2^22": round(p22["ntt"], 1),
+                              "coset_ntt_2^22_to_2^24": round(p22["coset"], 1),
+                              "batch_inversion_2^22": round(p22["batch_inversion"], 1)},
+            "primitives_k20_ms": {k: round(v, 1) for k, v in p20.items()},
+            "proof_k20_lower_bound_s": round(est / 1e3, 2),
+            "proof_k20_counts": counts}
 
 
 if __name__ == "__main__":

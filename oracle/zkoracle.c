@@ -547,19 +547,28 @@ void zo_extended_prepare(zo_fr *s, const zo_fr *coset_powers, int coset_powers_n
 }
 
 /* ---- batch inversion / logup --------------------------------------------- */
+/* Montgomery's trick over contiguous chunks, one chunk per task (halo2's
+ * BatchInvert run per parallel chunk, as src/permutation.rs:73-75 and
+ * src/shuffle.rs:163-165 do); zeros stay zero.  The inverses do not depend
+ * on the split. */
 void zo_batch_invert(zo_fr *a, size_t n) {
     zo_fr *pre = (zo_fr *)malloc(sizeof(zo_fr) * (n ? n : 1));
-    zo_fr acc; fr_one(&acc);
-    for (size_t i = 0; i < n; i++) {
-        pre[i] = acc;
-        if (!is_zero4(a[i].l)) fr_mul(&acc, &acc, &a[i]);
-    }
-    zo_fr inv; zo_fr_inv(&inv, &acc);
-    for (size_t i = n; i-- > 0;) {
-        if (is_zero4(a[i].l)) continue;
-        zo_fr t; fr_mul(&t, &inv, &pre[i]);
-        fr_mul(&inv, &inv, &a[i]);
-        a[i] = t;
+    const size_t chunks = n >= 65536 ? (size_t)zo_get_threads() * 4 : 1;
+#pragma omp parallel for OMP_THREADS schedule(dynamic, 1) if (chunks > 1)
+    for (long c = 0; c < (long)chunks; c++) {
+        const size_t lo = n * (size_t)c / chunks, hi = n * (size_t)(c + 1) / chunks;
+        zo_fr acc; fr_one(&acc);
+        for (size_t i = lo; i < hi; i++) {
+            pre[i] = acc;
+            if (!is_zero4(a[i].l)) fr_mul(&acc, &acc, &a[i]);
+        }
+        zo_fr inv; zo_fr_inv(&inv, &acc);
+        for (size_t i = hi; i-- > lo;) {
+            if (is_zero4(a[i].l)) continue;
+            zo_fr t; fr_mul(&t, &inv, &pre[i]);
+            fr_mul(&inv, &inv, &a[i]);
+            a[i] = t;
+        }
     }
     free(pre);
 }

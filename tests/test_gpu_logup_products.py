@@ -27,8 +27,12 @@ def _arena():
     return Arena()
 
 
-@pytest.mark.parametrize("n", [1, 7, 512, 513, 4097, 1 << 16, (1 << 20) + 3, 1 << 22])
+@pytest.mark.parametrize("n", [1, 7, 255, 256, 257, 512, 513, 4095, 4096, 4097, 1 << 16, (1 << 20) + 3, 1 << 22,
+                               1 << 24, (1 << 24) + 5])
 def test_batch_invert(gpu, zo, n):
+    """tiles of 4096 (partial last tiles, one to sixteen tile totals per
+    thread of the top workgroup at 2^24) and, past 2^24, the multi-level
+    hierarchy; zeros stay zero"""
     dv, call = _dv()
     A = _arena()
     x = zo.rand_fr(n, 50 + n)
@@ -47,6 +51,21 @@ def test_batch_invert(gpu, zo, n):
         one = zo.to_mont([1])[0]
         prod = zo.field_op(x, 0, None, got, 0, None, n, 1)
         assert (prod[nz] == one).all() and (got[~nz] == 0).all()
+
+
+def test_batch_invert_zero_tile(gpu, zo):
+    """a tile of 4096 zeros between two tiles of values: its product is the
+    empty product, its zeros stay zero, the other tiles are unaffected"""
+    dv, call = _dv()
+    n = 3 * 4096 + 100
+    x = zo.rand_fr(n, 77)
+    x[4096:8192] = 0
+    d = dv.to_dev(x)
+    tmp = dv.empty(n)
+    call("zk_batch_invert", dv.ptr(d), dv.ptr(tmp), n, dv.stream_handle())
+    got = dv.from_dev(d)
+    assert (got == zo.batch_invert(x)).all()
+    assert (got[4096:8192] == 0).all()
 
 
 @pytest.mark.parametrize("log_n", [10, 16, 20])

@@ -4,14 +4,15 @@
 // `eval_logup_z_pure` (cuda/bn254.cu:19-220, :1914-1993; caller
 // src/lib.rs:590-745), plus zk_batch_invert.
 //
-// Batch inversion is a hierarchical Montgomery trick.  Level 0: thread t owns
-// the m elements t, t + T, t + 2T, ... (T = n/m, so every load is coalesced)
-// and writes only the product of its elements.  The T products are inverted
-// recursively the same way; the last <= 512 values are inverted by one
-// workgroup with a product tree in LDS and a single binary-Euclid inverse.
-// On the way down each thread reloads its m elements, rebuilds the prefix
-// products in registers and emits the inverses.  Nothing of size n is stored
-// besides the output: traffic is two reads and one write per element.
+// Batch inversion is Montgomery's trick in three launches over tiles of 4096
+// elements (prefix products into the caller's temp, one workgroup inverting
+// the tile totals with a single field inversion, then two products per
+// element on the way back; details at k_binv_up below).  Above 2^24 elements
+// a multi-level hierarchy takes over: level 0 thread t owns the m elements
+// t, t + T, t + 2T, ... (T = n/m, coalesced) and writes only their product;
+// the T products are inverted recursively the same way, the last <= 512 by
+// one workgroup with a product tree in LDS; on the way down each thread
+// rebuilds its prefix products in registers and emits the inverses.
 // A zero input stays zero and does not disturb the others (halo2
 // BatchInvert semantics); the reference's chunked kernel would zero its chunk
 // instead, a case its callers never produce (beta is random).
@@ -121,6 +122,142 @@ __global__ __launch_bounds__(256) void k_binv_top(const Fr *__restrict__ x, cons
     }
     for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
         Fr r = zero[i] ? fe_zero<FrParams>() : tree[P + i];
+        if (MODE == OUT_ADD) r = fe_load<FrParams>(&out[i]) + r;
+        fe_store<FrParams>(&out[i], r);
+    }
+}
+
+// ---- batch inversion in three launches (n <= 2^24) ------------------------
+// Tile b = 4096 elements, 256 threads x 16; thread t owns the elements
+// b * 4096 + k * 256 + t (k < 16), so every load and store is coalesced.
+//   up:   each thread's exclusive prefix products go to pre[] (the slot of its
+//         first element is not needed: that prefix is 1), the 256 thread
+//         products into an LDS product tree; the tile's total goes into the
+//         free slot pre[b * 4096].
+//   top:  one workgroup inverts the T tile totals (per-thread prefixes in
+//         registers, a product tree, one field inversion) and writes each
+//         inverse back into its tile's slot.
+//   down: each tile rebuilds its thread products (pre of the last element
+//         times that element), the tree, walks it down from the tile's
+//         inverse, and each thread emits its elements' inverses backwards
+//         (two products per element).
+// Three Montgomery products per element in all; traffic 5 x 32 B per element
+// (x twice, pre written and read, out written).
+constexpr uint32_t kBT = 256;
+constexpr int kBM = 16;
+constexpr uint32_t kTile = kBT * kBM;
+constexpr uint32_t kTopMax = kBT * kBM;  // tile totals one workgroup inverts
+
+// LDS product tree over kBT leaves at [kBT, 2 kBT): tree[1] = their product
+__device__ __forceinline__ void tree_up(Fr *tree) {
+    for (uint32_t w = kBT / 2; w >= 1; w >>= 1) {
+        if (threadIdx.x < w) {
+            const uint32_t p = w + threadIdx.x;
+            tree[p] = tree[2 * p] * tree[2 * p + 1];
+        }
+        __syncthreads();
+    }
+}
+// from tree[1] = the inverse of the product: each leaf becomes its inverse
+__device__ __forceinline__ void tree_down(Fr *tree) {
+    for (uint32_t w = 1; w < kBT; w <<= 1) {
+        if (threadIdx.x < w) {
+            const uint32_t p = w + threadIdx.x;
+            const Fr ip = tree[p], l = tree[2 * p], r = tree[2 * p + 1];
+            tree[2 * p] = ip * r;
+            tree[2 * p + 1] = ip * l;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_binv_up(const Fr *__restrict__ x, const Fr *__restrict__ beta,
+                                                 Fr *__restrict__ pre, uint32_t n) {
+    __shared__ Fr tree[2 * kBT];
+    const uint32_t t = threadIdx.x, base = blockIdx.x * kTile;
+    const bool hb = beta != nullptr;
+    const Fr b = hb ? fe_load<FrParams>(beta) : fe_zero<FrParams>();
+    Fr acc = fe_one<FrParams>();
+#pragma unroll 4
+    for (int k = 0; k < kBM; k++) {
+        const uint32_t i = base + (uint32_t)k * kBT + t;
+        if (i >= n) break;
+        const Fr v = load_plus(x, i, beta, hb, b);
+        if (k) fe_store<FrParams>(&pre[i], acc);
+        if (!fe_is_zero<FrParams>(v)) acc = acc * v;
+    }
+    tree[kBT + t] = acc;
+    __syncthreads();
+    tree_up(tree);
+    if (t == 0) fe_store<FrParams>(&pre[base], tree[1]);
+}
+
+// T tile totals at pre[b * kTile] -> their inverses, in place; c = ceil(T / kBT)
+__global__ __launch_bounds__(kBT) void k_binv_top2(Fr *pre, uint32_t T, uint32_t c) {
+    __shared__ Fr tree[2 * kBT];
+    const uint32_t t = threadIdx.x;
+    Fr v[kBM], pr[kBM];
+    Fr acc = fe_one<FrParams>();
+#pragma unroll
+    for (int k = 0; k < kBM; k++) {
+        const uint32_t b = t * c + (uint32_t)k;
+        v[k] = fe_one<FrParams>();
+        pr[k] = acc;
+        if ((uint32_t)k < c && b < T) {
+            v[k] = fe_load<FrParams>(&pre[(size_t)b * kTile]);
+            acc = acc * v[k];
+        }
+    }
+    tree[kBT + t] = acc;
+    __syncthreads();
+    tree_up(tree);
+    if (t == 0) tree[1] = fe_inv<FrParams>(tree[1]);
+    __syncthreads();
+    tree_down(tree);
+    acc = tree[kBT + t];
+#pragma unroll
+    for (int k = kBM - 1; k >= 0; k--) {
+        const uint32_t b = t * c + (uint32_t)k;
+        if ((uint32_t)k < c && b < T) {
+            fe_store<FrParams>(&pre[(size_t)b * kTile], acc * pr[k]);
+            acc = acc * v[k];
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBT) void k_binv_down(const Fr *x, const Fr *__restrict__ beta,
+                                                   const Fr *__restrict__ pre, Fr *out, uint32_t n) {
+    __shared__ Fr tree[2 * kBT];
+    const uint32_t t = threadIdx.x, base = blockIdx.x * kTile;
+    const bool hb = beta != nullptr;
+    const Fr b = hb ? fe_load<FrParams>(beta) : fe_zero<FrParams>();
+    // this thread's elements: k < kl
+    int kl = 0;
+    while (kl < kBM && base + (uint32_t)kl * kBT + t < n) kl++;
+    Fr prod = fe_one<FrParams>();
+    if (kl) {
+        const uint32_t i = base + (uint32_t)(kl - 1) * kBT + t;
+        const Fr v = load_plus(x, i, beta, hb, b);
+        if (kl > 1) prod = fe_load<FrParams>(&pre[i]);
+        if (!fe_is_zero<FrParams>(v)) prod = prod * v;
+    }
+    tree[kBT + t] = prod;
+    __syncthreads();
+    const Fr inv_tile = fe_load<FrParams>(&pre[base]);
+    tree_up(tree);
+    if (t == 0) tree[1] = inv_tile;
+    __syncthreads();
+    tree_down(tree);
+    Fr acc = tree[kBT + t];
+    for (int k = kl - 1; k >= 0; k--) {
+        const uint32_t i = base + (uint32_t)k * kBT + t;
+        const Fr v = load_plus(x, i, beta, hb, b);
+        Fr r = fe_zero<FrParams>();
+        if (!fe_is_zero<FrParams>(v)) {
+            r = k ? acc * fe_load<FrParams>(&pre[i]) : acc;
+            acc = acc * v;
+        }
         if (MODE == OUT_ADD) r = fe_load<FrParams>(&out[i]) + r;
         fe_store<FrParams>(&out[i], r);
     }
@@ -238,8 +375,10 @@ __global__ __launch_bounds__(256) void k_fscan_apply(Fr *z, uint32_t n, const Fr
 
 namespace zk {
 
-// scratch Fr needed by batch_invert_ex for n elements (0 when n <= kTop)
+// scratch Fr needed by batch_invert_ex for n elements: n (the prefix
+// products) up to 2^24 elements, beyond that the multi-level hierarchy's
 size_t binv_scratch_elems(uint32_t n) {
+    if (n <= (size_t)kTopMax * kTile) return n;
     size_t s = 0;
     uint32_t cnt = n;
     while (cnt > kTop) {
@@ -255,6 +394,18 @@ size_t binv_scratch_elems(uint32_t n) {
 hipError_t batch_invert_ex(const Fr *x, const Fr *beta, Fr *out, int mode, uint32_t n, Fr *temp,
                            hipStream_t stream) {
     if (n == 0) return hipSuccess;
+    if (n <= (size_t)kTopMax * kTile) {
+        const uint32_t T = (n + kTile - 1) / kTile;
+        k_binv_up<<<T, kBT, 0, stream>>>(x, beta, temp, n);
+        ZK_CHECK(hipGetLastError());
+        k_binv_top2<<<1, kBT, 0, stream>>>(temp, T, (T + kBT - 1) / kBT);
+        ZK_CHECK(hipGetLastError());
+        if (mode == OUT_ADD)
+            k_binv_down<OUT_ADD><<<T, kBT, 0, stream>>>(x, beta, temp, out, n);
+        else
+            k_binv_down<OUT_SET><<<T, kBT, 0, stream>>>(x, beta, temp, out, n);
+        return hipGetLastError();
+    }
     if (n <= kTop) {
         if (mode == OUT_ADD)
             k_binv_top<OUT_ADD><<<1, 256, 0, stream>>>(x, beta, out, n);
@@ -332,7 +483,7 @@ hipError_t field_scan(Fr *z, uint32_t n, bool mul, const Fr *carry, Fr *last_z, 
 
 extern "C" {
 
-// In-place batch inversion; temp holds n Fr (the hierarchy needs < n/3).
+// In-place batch inversion; temp holds n Fr.
 hipError_t zk_batch_invert(void *a, void *temp, int n, hipStream_t stream) {
     if (n < 0 || (n > 0 && !a) || (binv_scratch_elems((uint32_t)n) && !temp)) return hipErrorInvalidValue;
     return batch_invert_ex((const Fr *)a, nullptr, (Fr *)a, OUT_SET, (uint32_t)n, (Fr *)temp, stream);
