@@ -7,11 +7,17 @@ One step = one pass over the primitive set, inputs already resident in HBM:
 `value` is the 2^22 MSM time in ms (lower is better); every primitive's time
 is reported beside it.  The roofline object covers the dominant kernel (the
 MSM bucket accumulation), timed with HIP events on its own stream.
-The CPU baseline is the C oracle's MSM on the same shape, on host cores.
+The CPU baseline is the C oracle on the host cores given to the job: every
+primitive once at n = 2^22 (the MSM is its `value`) and a k=20 proof lower
+bound.
 
-Beside it: 64 uneven MSMs in one batched pass against the plain per-MSM path.
+Beside it (the metric's other two terms): 64 uneven MSMs and 64 NTTs in one
+batched pass against the plain per-call path; the prover core's k=20 and
+k=22 proofs (median of 5 per mode, device and host-resident witness); and
+the serving leg, plain one-at-a-time against the server at four offered
+loads, p50 / p99 latency and proofs per hour.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-proof] [--no-serve]
 """
 import argparse
 import ctypes
@@ -46,8 +52,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-proof", action="store_true")
     ap.add_argument("--proof-ks", default="20,22")
-    ap.add_argument("--serve", action="store_true")
-    ap.add_argument("--serve-count", type=int, default=16)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-count", type=int, default=24)
     return ap.parse_args()
 
 
@@ -288,7 +294,7 @@ def main():
                           "ntt": bench_batched_ntt(L, args.steps)}
     if not args.no_proof:
         out["proof_core"] = bench_proof_core([int(k) for k in args.proof_ks.split(",") if k])
-    if args.serve:
+    if not args.no_serve:
         out["serving"] = bench_serving(args.serve_count)
     out["host_copy"] = bench_host_copy(prim_ms["msm_2^22"])
     if not args.no_cpu:
@@ -404,59 +410,92 @@ def bench_batched_msm(L, points, reps):
             "speedup": round(res["plain"] / res["batched"], 3), "identical_results": same}
 
 
-def bench_proof_core(ks):
+def bench_proof_core(ks, reps=5):
     """One proof's device arithmetic (prover core) for the benchmark circuit
-    family (40 advice, 6 range lookups into a 2^16 table, 12 permutation
-    columns, 2 shuffles of up to 2^18 rows) at each k: one warm-up proof, then
-    one timed proof through the batched MSM and one through one MSM call per
-    polynomial (both opened with GWC), and one batched proof opened with
-    Shplonk.  Witness synthesis is excluded (the reference takes the advice as
-    input)."""
+    family (40 advice, 10 fixed, 1 instance column, 6 range lookups into a
+    2^16 table, 12 permutation columns, 2 shuffles of up to 2^18 rows) at
+    each k, after one warm-up proof: the median of `reps` proofs through the
+    batched MSM, through one MSM call per polynomial (both opened with GWC),
+    batched and opened with Shplonk, and batched with the advice and instance
+    columns uploaded from pinned host memory during the commitment phase (as
+    create_proof_from_advices receives them).  The device-witness modes
+    exclude the witness synthesis (the reference takes the advice as input)."""
+    import statistics
+
     import torch
-    from zkprover.prover import BATCHED, SHPLONK, Circuit, Prover, ProvingKey, footprint
+    from zkprover import hostmem
+    from zkprover.prover import BATCHED, HOST_WITNESS, SHPLONK, Circuit, Prover, ProvingKey, footprint
     res = {}
     for k in ks:
         circ = Circuit.benchmark(k)
         pk_b, job_b = footprint(circ)
         pk = ProvingKey(circ, seed=7)
         pr = Prover(pk)
+        bufs = []
         try:
             pr.prove(1, flags=BATCHED, want_pieces=False)
-            _, _, ib = pr.prove(2, flags=BATCHED, want_pieces=False)
-            _, _, ip = pr.prove(2, flags=0, want_pieces=False)
-            _, _, ish = pr.prove(2, flags=BATCHED | SHPLONK, want_pieces=False)
+            out, ok = {}, True
+            for mode, flags in (("batched", BATCHED), ("plain", 0), ("shplonk_batched", BATCHED | SHPLONK)):
+                t, info = [], None
+                for r in range(reps):
+                    _, _, info = pr.prove(2 + r, flags=flags, want_pieces=False)
+                    t.append(info.total_ms)
+                    ok = ok and bool(info.z_ok and info.quotient_ok and info.open_ok)
+                out[mode] = (statistics.median(t), info)
+            n = 1 << k
+            bufs = [hostmem.PinnedBuffer(n * 32) for _ in range(circ.advice + circ.instance)]
+            adv = [b.ptr for b in bufs[:circ.advice]]
+            ins = [b.ptr for b in bufs[circ.advice:]]
+            pr.prove(2, flags=BATCHED, want_pieces=False)
+            pr.witness_to_host(adv, ins)
+            pr.set_host_witness(adv, ins)
+            t = []
+            for r in range(reps):
+                _, _, info = pr.prove(2, flags=BATCHED | HOST_WITNESS, want_pieces=False)
+                t.append(info.total_ms)
+                ok = ok and bool(info.z_ok and info.quotient_ok and info.open_ok)
+            out["host_witness"] = (statistics.median(t), info)
         finally:
+            for b in bufs:
+                b.close()
             pr.close()
             pk.close()
             torch.cuda.synchronize()
-        res["k%d" % k] = {"batched_s": round(ib.total_ms / 1e3, 4), "plain_s": round(ip.total_ms / 1e3, 4),
-                          "speedup": round(ip.total_ms / ib.total_ms, 3),
-                          "shplonk_batched_s": round(ish.total_ms / 1e3, 4),
+        ib = out["batched"][1]
+        res["k%d" % k] = {"batched_s": round(out["batched"][0] / 1e3, 4), "plain_s": round(out["plain"][0] / 1e3, 4),
+                          "speedup": round(out["plain"][0] / out["batched"][0], 3),
+                          "shplonk_batched_s": round(out["shplonk_batched"][0] / 1e3, 4),
+                          "host_witness_batched_s": round(out["host_witness"][0] / 1e3, 4),
+                          "proofs_per_mode": reps, "statistic": "median",
                           "phases_ms": ib.as_dict()["phase_ms"],
-                          "shplonk_open_ms": round(ish.phase_ms[6], 3),
-                          "z_ok": bool(ib.z_ok and ip.z_ok and ish.z_ok),
-                          "quotient_ok": bool(ib.quotient_ok and ip.quotient_ok and ish.quotient_ok),
-                          "open_ok": bool(ib.open_ok and ip.open_ok and ish.open_ok),
-                          "device_bytes": {"key": pk_b, "per_proof": job_b}}
+                          "host_witness_phases_ms": out["host_witness"][1].as_dict()["phase_ms"],
+                          "shplonk_open_ms": round(out["shplonk_batched"][1].phase_ms[6], 3),
+                          "all_checks_ok": ok,
+                          "device_bytes": {"key": pk_b, "per_proof": job_b},
+                          "host_witness_bytes": (circ.advice + circ.instance) * (1 << k) * 32}
     return res
 
 
 def bench_serving(count):
     """Proofs of k in {18, 20, 22} (weights 0.5 / 0.3 / 0.2) arriving as a
-    Poisson stream, offered at 0.5, 0.9, 1.5 and 4x the one-at-a-time
-    capacity: the plain path (one slot, FIFO) against the server (3 slots,
-    admission by per-proof HBM footprint), same arrivals and seeds.  For each
-    plain operating point, the served throughput reachable at the same or a
-    lower p99 latency is reported beside it."""
-    from zkprover.prover import Circuit
-    from zkprover.server import Server, run_poisson, summarize
+    Poisson stream, offered at 0.5, 1, 2 and 4x the one-at-a-time capacity:
+    the plain path (one slot, FIFO: one proof at a time, each with its own
+    batched MSMs) against the server (3 slots, admission by per-proof HBM
+    footprint, overlap when the backlog exceeds the proofs in flight, the
+    commitment MSMs of concurrent proofs merged into shared batched passes),
+    same arrivals and seeds.  For each plain operating point, the served
+    throughput reachable at the same or a lower p99 latency is reported
+    beside it."""
+    from zkprover.prover import BATCHED, Circuit
+    from zkprover.server import CROSS_BATCH, Server, run_poisson, summarize
     shapes = [Circuit.benchmark(k) for k in (18, 20, 22)]
     weights = [0.5, 0.3, 0.2]
-    loads = [0.5, 0.9, 1.5, 4.0]
-    out = {"workload": "Poisson arrivals of k=18/20/22 proofs (weights 0.5/0.3/0.2), %d requests per load" % count}
+    loads = [0.5, 1.0, 2.0, 4.0]
+    out = {"workload": "Poisson arrivals of k=18/20/22 proofs of the benchmark circuit (weights 0.5/0.3/0.2), "
+                       "%d requests per load" % count}
     res = {}
-    for mode, slots in (("plain", 1), ("served", 3)):
-        srv = Server(shapes, slots=slots)
+    for mode, slots, flags in (("plain", 1, BATCHED), ("served", 3, BATCHED | CROSS_BATCH)):
+        srv = Server(shapes, slots=slots, flags=flags)
         try:
             t = []
             for sh in range(3):
@@ -472,6 +511,7 @@ def bench_serving(count):
                 recs, wall = run_poisson(srv, ld * 1000.0 / mean_ms, count, weights, seed=10)
                 res[mode]["load_%.1f" % ld] = summarize(recs, wall)
             res[mode]["server"] = srv.stats()
+            res[mode]["batcher"] = srv.batch_stats()
         finally:
             srv.close()
     out.update(res)
@@ -483,7 +523,7 @@ def bench_serving(count):
             continue
         ok = [w["proofs_per_hour"] for kk, w in res["served"].items()
               if kk.startswith("load_") and w["p99_ms"] <= v["p99_ms"]]
-        cmp.append({"p99_ms": v["p99_ms"], "plain_proofs_per_hour": v["proofs_per_hour"],
+        cmp.append({"load": k, "p99_ms": v["p99_ms"], "plain_proofs_per_hour": v["proofs_per_hour"],
                     "served_proofs_per_hour": max(ok) if ok else None})
     out["at_equal_p99"] = cmp
     return out

@@ -540,6 +540,10 @@ hipError_t zkp_job_destroy(void *job);
 #define ZKP_SHPLONK 4
 #define ZKP_NO_OPEN 8
 #define ZKP_TAMPER_EVAL 16
+/* ZKP_HOST_WITNESS: the advice and instance columns come from host memory
+ * (zkp_set_host_witness) and are uploaded during the commitment phase,
+ * overlapped with the MSMs, instead of being synthesised on the device. */
+#define ZKP_HOST_WITNESS 32
 /* The last proof's opening: its evaluations (host, Montgomery Fr) in the
  * reference's order (src/lib.rs:1025-1078): h(x) -- computed, not written --
  * then instance, advice, fixed (q, t, f_j) and the random polynomial at x,
@@ -554,6 +558,16 @@ hipError_t zkp_job_destroy(void *job);
  * returns the length only.  zkp_proof_instance: the last proof's instance
  * columns (n Fr each, `rows` Fr apart in out). */
 int zkp_proof_openings(void *job, void *evals, int max, void *challenges3);
+/* Host-resident witness for the next ZKP_HOST_WITNESS proof, as
+ * create_proof_from_advices receives it (src/lib.rs:205-257): advice_count =
+ * advice and instance_count = instance columns of 2^k Fr (Lagrange values,
+ * Montgomery) in host memory; the pointers are kept, not copied, until that
+ * proof returns; pinned memory (zk_pinned_alloc) makes the uploads
+ * asynchronous.  zkp_proof_witness copies the last proof's witness (advice,
+ * instance) to host memory. */
+hipError_t zkp_set_host_witness(void *job, const void *const *advice, int advice_count, const void *const *instance,
+                                int instance_count);
+hipError_t zkp_proof_witness(void *job, void *const *advice, void *const *instance);
 long long zkp_proof_bytes(void *job, void *out, long long max);
 hipError_t zkp_proof_instance(void *job, void *out, int rows);
 hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commitments, void *h_pieces,
@@ -561,10 +575,18 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
 
 /* ---- proof server: continuous admission by HBM footprint -------------- */
 /* Keys of every shape stay resident; `slots` workers (one stream each) start
- * queued requests as soon as a slot is free and the proof's device state fits
- * hbm_budget (0: free memory less 4 GiB), FIFO with backfill; per-proof
- * states are pooled per shape.  flags: ZKP_BATCHED or 0.  No reference
+ * queued requests FIFO as slots free up and the proof's device state fits
+ * hbm_budget (0: free memory less 4 GiB), with backfill: a request that does
+ * not fit lets smaller ones pass until it has been passed 8 times.  A proof
+ * joins others in flight only when more requests wait than are running
+ * (ZKS_EAGER: whenever a slot and memory allow); per-proof states are pooled
+ * per shape.  flags: ZKP_BATCHED / ZKP_SHPLONK / ZKP_NO_OPEN for every proof;
+ * ZKS_CROSS_BATCH (with ZKP_BATCHED) merges the commitment MSMs of proofs in
+ * flight into shared zk_msm_batch passes.  zks_batch_stats: merged passes,
+ * passes holding more than one proof's groups, MSM items.  No reference
  * counterpart (src/lib.rs proves one circuit at a time). */
+#define ZKS_EAGER 0x100
+#define ZKS_CROSS_BATCH 0x200
 typedef struct {
     int shape, err, z_ok, quotient_ok, ncommit, reserved;
     double queue_ms, run_ms, latency_ms, prove_ms;
@@ -574,6 +596,8 @@ hipError_t zks_create(const zkp_circuit *shapes, int nshapes, int slots, size_t 
 int zks_submit(void *srv, int shape, unsigned long long seed);
 hipError_t zks_wait(void *srv, int id, void *commitments, zks_result *res);
 hipError_t zks_stats(void *srv, int *peak_running, size_t *peak_bytes, size_t *budget);
+hipError_t zks_batch_stats(void *srv, unsigned long long *launches, unsigned long long *merged,
+                           unsigned long long *items);
 hipError_t zks_destroy(void *srv);
 
 /* ---- host residency: pinned pool and device-to-host copy queue --------- */

@@ -245,3 +245,36 @@ def test_prover_core_k14(gpu):
     finally:
         pr.close()
         pk.close()
+
+
+def test_prover_core_host_witness_same_proof(core):
+    """a proof whose advice and instance columns come from host memory
+    (ZKP_HOST_WITNESS, uploaded during the commitment phase) equals the proof
+    that synthesised the same witness on the device: same commitments,
+    quotient and transcript"""
+    from zkprover import hostmem
+    from zkprover.prover import BATCHED, HOST_WITNESS
+    pk, pr = core
+    n = 1 << pk.circuit.k
+    c1, p1, i1 = pr.prove(23, flags=BATCHED)
+    bufs = [hostmem.PinnedBuffer(n * 32) for _ in range(pk.circuit.advice + pk.circuit.instance)]
+    try:
+        adv = [b.ptr for b in bufs[:pk.circuit.advice]]
+        ins = [b.ptr for b in bufs[pk.circuit.advice:]]
+        pr.witness_to_host(adv, ins)
+        pr.set_host_witness(adv, ins)
+        c2, p2, i2 = pr.prove(23, flags=BATCHED | HOST_WITNESS)
+        assert i2.z_ok and i2.quotient_ok and i2.open_ok
+        assert (c1 == c2).all() and (p1 == p2).all()
+        assert bytes(i1.transcript) == bytes(i2.transcript)
+        # pageable host memory works too (synchronous uploads)
+        host = [np.zeros((n, 4), dtype=np.uint64) for _ in bufs]
+        for h, b in zip(host, bufs):
+            h[:] = b.array(np.uint64).reshape(n, 4)
+        pr.set_host_witness([h.ctypes.data for h in host[:pk.circuit.advice]],
+                            [h.ctypes.data for h in host[pk.circuit.advice:]])
+        c3, _, i3 = pr.prove(23, flags=BATCHED | HOST_WITNESS)
+        assert (c1 == c3).all() and bytes(i1.transcript) == bytes(i3.transcript)
+    finally:
+        for b in bufs:
+            b.close()

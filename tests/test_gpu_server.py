@@ -16,10 +16,12 @@ def _shapes():
     return [a, b]
 
 
-def test_server_mix_equals_alone(gpu):
-    from zkprover.prover import footprint
-    from zkprover.server import Server
+@pytest.mark.parametrize("mode", ["eager", "cross_batch"])
+def test_server_mix_equals_alone(gpu, mode):
+    from zkprover.prover import BATCHED, footprint
+    from zkprover.server import CROSS_BATCH, EAGER, Server
     shapes = _shapes()
+    flags = BATCHED | EAGER | (CROSS_BATCH if mode == "cross_batch" else 0)
     plan = [(0, 101), (1, 102), (0, 103), (1, 104), (1, 105), (0, 106), (0, 107), (1, 108)]
     # alone: one slot, one request at a time
     alone = {}
@@ -34,7 +36,7 @@ def test_server_mix_equals_alone(gpu):
     # in the mix: 3 slots, everything submitted at once, budget for 2 of the
     # larger per-proof states
     job_b = footprint(shapes[1])[1]
-    s3 = Server(shapes, slots=3, hbm_budget=2 * job_b + job_b // 2)
+    s3 = Server(shapes, slots=3, hbm_budget=2 * job_b + job_b // 2, flags=flags)
     try:
         ids = [(s3.submit(sh, seed), sh, seed) for sh, seed in plan]
         for rid, sh, seed in ids:
@@ -44,6 +46,14 @@ def test_server_mix_equals_alone(gpu):
         st = s3.stats()
         assert st["peak_running"] >= 2          # work overlapped
         assert st["peak_bytes"] <= st["budget_bytes"]
+        bs = s3.batch_stats()
+        if mode == "cross_batch":
+            # every commitment group went through the batcher, and some
+            # passes held groups of several proofs (all 8 were queued at once)
+            assert bs["msm_items"] == sum(s3.ncommit[sh] for sh, _ in plan)
+            assert bs["passes_with_several_proofs"] >= 1
+        else:
+            assert bs["msm_items"] == 0
     finally:
         s3.close()
 
@@ -62,7 +72,8 @@ def test_server_poisson_driver(gpu):
 def test_server_zkwasm_sizes_mix_equals_alone(gpu):
     """the serving workload's circuit family at k = 18, 20 and 22: every
     proof's commitments are the same run alone (one slot) and admitted into
-    a mix of three slots while others are on the device"""
+    a mix of three slots while others are on the device, their commitment
+    MSMs merged across proofs"""
     from zkprover.prover import Circuit
     from zkprover.server import Server
     shapes = [Circuit.benchmark(k) for k in (18, 20, 22)]
@@ -76,7 +87,9 @@ def test_server_zkwasm_sizes_mix_equals_alone(gpu):
             alone[seed] = com
     finally:
         s1.close()
-    s3 = Server(shapes, slots=3)
+    from zkprover.prover import BATCHED
+    from zkprover.server import CROSS_BATCH
+    s3 = Server(shapes, slots=3, flags=BATCHED | CROSS_BATCH)
     try:
         ids = [(s3.submit(sh, seed), sh, seed) for sh, seed in plan]
         for rid, sh, seed in ids:

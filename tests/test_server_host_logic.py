@@ -1,5 +1,6 @@
 """CPU: the proof server's scheduling (admission by HBM budget, per-shape
-pools, eviction, requests submitted while workers run) built against CPU
+pools, eviction, requests submitted while workers run, overlap only on
+backlog, the bypass limit, the cross-proof MSM batcher) built against CPU
 stand-ins for the HIP runtime and the prover core, under AddressSanitizer and
 ThreadSanitizer.  Regression: requests used to live in a std::vector that
 zks_submit grew while workers held references into it, so a worker could mark
@@ -21,7 +22,8 @@ def test_server_scheduling_under_sanitizer(tmp_path, san):
         pytest.skip("needs g++ and the ROCm headers")
     exe = str(tmp_path / ("srv_" + san))
     cmd = ["g++", "-std=c++17", "-g", "-O1", "-fsanitize=" + san, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-           "-I" + os.path.join(ROOT, "include")] + SRC + ["-o", exe, "-lpthread"]
+           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "zkwasm-prover_amd", "csrc")] + SRC + \
+        ["-o", exe, "-lpthread"]
     subprocess.run(cmd, check=True, capture_output=True)
     for _ in range(3):
         r = subprocess.run([exe], capture_output=True, text=True, timeout=120)

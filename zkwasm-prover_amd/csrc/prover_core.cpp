@@ -517,6 +517,13 @@ struct Job {
     std::vector<zkh::Fr> last_evals;  // the last proof's opening (host copies)
     zkh::Fr last_open_ch[3] = {};     // v, y', u
     std::vector<uint8_t> proof;       // the last proof's bytes (written points and scalars)
+    // host-resident witness (ZKP_HOST_WITNESS): columns the next proof uploads,
+    // the upload stream and one event per commitment group
+    std::vector<const void *> host_adv, host_inst;
+    hipStream_t upload = nullptr;
+    std::vector<hipEvent_t> upload_ev;
+    zk::CommitFn commit_fn = nullptr;  // batched commitments through a hook (the proof server's)
+    void *commit_ctx = nullptr;
     void *cf(int c) const { return cst + (size_t)c * FR; }
     int j_ypow(int i) const { return J_COUNT_FIXED + pk->sh.P + i; }                  // y^i, i <= FG
     int j_mypow(int i) const { return J_COUNT_FIXED + pk->sh.P + pk->sh.FG + 1 + i; }  // -y^i
@@ -592,10 +599,17 @@ void job_layout(Job &j, Arena &a) {
 // commitments of polys (device Lagrange or coefficient values of n) against
 // bases; out: affine points (host, 64 B each)
 hipError_t commit(Job &j, const void *bases, const std::vector<const void *> &polys, uint8_t *out, bool batched,
-                  hipStream_t st) {
+                  hipStream_t st, bool uploaded = false) {
     const Shape &s = j.pk->sh;
     for (size_t first = 0; first < polys.size(); first += kMsmGroup) {
         int cnt = (int)(polys.size() - first < (size_t)kMsmGroup ? polys.size() - first : kMsmGroup);
+        if (uploaded) PCHECK(hipStreamWaitEvent(st, j.upload_ev[first / kMsmGroup], 0));
+        if (batched && j.commit_fn) {
+            std::vector<zk_msm_item> it(cnt);
+            for (int m = 0; m < cnt; m++) it[m] = zk_msm_item{bases, polys[first + m], (int)s.n, 0};
+            PCHECK(j.commit_fn(j.commit_ctx, it.data(), cnt, out + first * 64, st));
+            continue;
+        }
         if (batched) {
             std::vector<zk_msm_item> it(cnt);
             for (int m = 0; m < cnt; m++) it[m] = zk_msm_item{bases, polys[first + m], (int)s.n, 0};
@@ -620,6 +634,31 @@ hipError_t commit(Job &j, const void *bases, const std::vector<const void *> &po
             else
                 memcpy(o, q, 64);
         }
+    }
+    return hipSuccess;
+}
+
+// Uploads the host witness (advice then instance columns) into the job's
+// Lagrange buffers on the upload stream, recording one event per commitment
+// group; the upload starts after the work already on st (the previous
+// proof's reads of these buffers).
+hipError_t upload_witness(Job &j, size_t ncols, hipStream_t st) {
+    const Shape &s = j.pk->sh;
+    if (!j.upload) PCHECK(hipStreamCreateWithFlags(&j.upload, hipStreamNonBlocking));
+    const size_t groups = (ncols + kMsmGroup - 1) / kMsmGroup;
+    while (j.upload_ev.size() < groups + 1) {
+        hipEvent_t e;
+        PCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        j.upload_ev.push_back(e);
+    }
+    PCHECK(hipEventRecord(j.upload_ev[groups], st));
+    PCHECK(hipStreamWaitEvent(j.upload, j.upload_ev[groups], 0));
+    for (size_t c = 0; c < ncols; c++) {
+        const bool adv = c < (size_t)s.A;
+        void *dst = adv ? j.adv_lag[c] : j.inst_lag[c - s.A];
+        const void *src = adv ? j.host_adv[c] : j.host_inst[c - s.A];
+        PCHECK(hipMemcpyAsync(dst, src, (size_t)s.n * FR, hipMemcpyHostToDevice, j.upload));
+        if ((c + 1) % kMsmGroup == 0 || c + 1 == ncols) PCHECK(hipEventRecord(j.upload_ev[c / kMsmGroup], j.upload));
     }
     return hipSuccess;
 }
@@ -663,10 +702,12 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-hipError_t witness(Job &j, hipStream_t st) {
+hipError_t witness(Job &j, bool from_host, hipStream_t st) {
     const Shape &s = j.pk->sh;
     const Pk &pk = *j.pk;
     const uint64_t seed = j.seed;
+    if (from_host)  // the caller's columns are uploaded with the commitments
+        return zk_rand_fr(j.rnd_lag, (int)s.n, seed * 1000003ull + 0x2000, st);
     for (int c = 0; c < s.A; c++) PCHECK(zk_rand_fr(j.adv_lag[c], (int)s.n, seed * 1000003ull + c, st));
     // instance columns: the public values of the usable rows (the caller's
     // `instances` in the reference, padded with zeros to n, src/lib.rs:246-253)
@@ -1246,9 +1287,67 @@ hipError_t zkp_job_create(void *pk, void **job_out) {
 hipError_t zkp_job_destroy(void *job) {
     if (!job) return hipSuccess;
     Job *j = (Job *)job;
+    if (j->upload) (void)hipStreamSynchronize(j->upload);
+    for (hipEvent_t e : j->upload_ev) (void)hipEventDestroy(e);
+    if (j->upload) (void)hipStreamDestroy(j->upload);
     hipError_t e = hipFree(j->mem);
     delete j;
     return e;
+}
+
+}  // extern "C"
+
+namespace zk {
+hipError_t job_set_commit_fn(void *job, CommitFn fn, void *ctx) {
+    if (!job) return hipErrorInvalidValue;
+    Job &j = *(Job *)job;
+    j.commit_fn = fn;
+    j.commit_ctx = ctx;
+    return hipSuccess;
+}
+}  // namespace zk
+
+extern "C" {
+
+// Host-resident witness for the next ZKP_HOST_WITNESS proof: advice and
+// instance columns (2^k Fr each, Lagrange values, Montgomery form) in host
+// memory, as create_proof_from_advices receives them (src/lib.rs:205-257).
+// The pointers are kept, not copied, and must stay valid until that proof
+// returns; pinned memory (zk_pinned_alloc) lets the uploads overlap the
+// commitments.
+hipError_t zkp_set_host_witness(void *job, const void *const *advice, int advice_count, const void *const *instance,
+                                int instance_count) {
+    return zk::guard([&]() -> hipError_t {
+        if (!job) return hipErrorInvalidValue;
+        Job &j = *(Job *)job;
+        const Shape &s = j.pk->sh;
+        if (advice_count != s.A || instance_count != s.I || (s.A && !advice) || (s.I && !instance))
+            return hipErrorInvalidValue;
+        for (int c = 0; c < s.A; c++)
+            if (!advice[c]) return hipErrorInvalidValue;
+        for (int c = 0; c < s.I; c++)
+            if (!instance[c]) return hipErrorInvalidValue;
+        j.host_adv.assign(advice, advice + s.A);
+        j.host_inst.assign(instance, instance + s.I);
+        return hipSuccess;
+    });
+}
+
+// The last proof's witness (advice then instance columns, 2^k Fr each) into
+// host memory.
+hipError_t zkp_proof_witness(void *job, void *const *advice, void *const *instance) {
+    if (!job) return hipErrorInvalidValue;
+    const Job &j = *(const Job *)job;
+    const Shape &s = j.pk->sh;
+    for (int c = 0; c < s.A; c++) {
+        if (!advice || !advice[c]) return hipErrorInvalidValue;
+        PCHECK(hipMemcpy(advice[c], j.adv_lag[c], (size_t)s.n * FR, hipMemcpyDeviceToHost));
+    }
+    for (int c = 0; c < s.I; c++) {
+        if (!instance || !instance[c]) return hipErrorInvalidValue;
+        PCHECK(hipMemcpy(instance[c], j.inst_lag[c], (size_t)s.n * FR, hipMemcpyDeviceToHost));
+    }
+    return hipSuccess;
 }
 
 // One proof's device arithmetic for the given seed on `stream`.  commitments
@@ -1286,8 +1385,11 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         tr.scalars(&pk.vk_repr, 1, false);
 
         // ---- 0. witness (the caller's advice and instances in the reference) ----
-        PCHECK(witness(j, st));
-        if (flags & ZKP_TAMPER_GATE) {
+        const bool from_host = (flags & ZKP_HOST_WITNESS) != 0;
+        if (from_host && ((int)j.host_adv.size() != s.A || (int)j.host_inst.size() != s.I))
+            return hipErrorInvalidValue;
+        PCHECK(witness(j, from_host, st));
+        if ((flags & ZKP_TAMPER_GATE) && !from_host) {
             // self-test of the quotient check: one usable row breaks the gate
             uint8_t *d = (uint8_t *)j.adv_lag[s.col_gate + 3] + (size_t)(s.u / 2) * FR;
             PCHECK(zk_fr_op(d, d, pk.cf(C_ONE), OP_ADD, 1, st));
@@ -1298,12 +1400,15 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         // ---- 1. advice and instance commitments (Lagrange basis) and iNTT -------
         // one batch over advice then instances (batch_msm_and_intt_ext,
         // src/lib.rs:320-379); the instance commitments are absorbed first and
-        // not written (common_point), the advice ones written (:380-385)
+        // not written (common_point), the advice ones written (:380-385).  A
+        // host-resident witness is uploaded on the job's upload stream, one
+        // commitment group ahead of the MSMs that read it.
         {
             std::vector<const void *> polys(j.adv_lag.begin(), j.adv_lag.end());
             for (auto p : j.inst_lag) polys.push_back(p);
+            if (from_host) PCHECK(upload_witness(j, polys.size(), st));
             std::vector<uint8_t> com(polys.size() * 64);
-            PCHECK(commit(j, pk.g_lag, polys, com.data(), batched != 0, st));
+            PCHECK(commit(j, pk.g_lag, polys, com.data(), batched != 0, st, from_host));
             memcpy(out, com.data() + (size_t)s.A * 64, (size_t)s.I * 64);
             memcpy(out + (size_t)s.I * 64, com.data(), (size_t)s.A * 64);
             tr.points(out, s.I, false);

@@ -3,30 +3,51 @@
 //
 // Each request names a circuit shape (one of the shapes the server was built
 // with; their keys stay resident) and a witness seed.  `slots` worker threads,
-// each with its own stream (default priority), take requests as soon as a slot
-// is free and the request's per-proof device state fits the HBM budget: FIFO
-// order, but a request that does not fit yet does not block smaller ones
-// behind it (backfill).  Per-proof device states are kept in per-shape pools
-// and reused; idle states of other shapes are freed when a request needs the
-// room.  Requests in flight overlap on the device, since their kernels go to
-// different streams.  The reference proves one circuit at a time
-// (src/lib.rs:205), so this has no counterpart there; each proof's arithmetic
-// is zkp_prove, unchanged, so a result does not depend on what else runs.
+// each with its own stream (default priority), take requests FIFO as slots
+// and HBM free up: a request that does not fit yet does not block smaller
+// ones behind it (backfill), until it has been passed over kMaxBypass times;
+// then nothing starts ahead of it.  A second proof joins one in flight only
+// when the backlog is longer than the number of proofs in flight (with
+// ZKS_EAGER: whenever a slot and memory are free), so at light load a proof
+// runs alone, as on the one-at-a-time path, and under load proofs overlap.
+// Per-proof device states are kept in per-shape pools and reused; idle states
+// of other shapes are freed when a request needs the room.
+//
+// With ZKS_CROSS_BATCH the commitment MSMs of the proofs in flight go
+// through one batcher: a group submitted while the batcher's stream is busy
+// waits, and every group waiting when it frees up runs in ONE zk_msm_batch
+// pass (items of several proofs and shapes, uneven sizes), after the
+// batcher's stream has waited for each submitting proof's stream to reach
+// its commit.  The reference proves one circuit at a time (src/lib.rs:205),
+// so none of this has a counterpart there; each proof's arithmetic is
+// zkp_prove, unchanged, so a result does not depend on what else runs.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
 
 #include "guard.h"
+#include "zkinternal.h"
 #include "zkwasm_prover_kernel.h"
 
 namespace {
+
+#define SCHECK(x)                             \
+    do {                                      \
+        hipError_t e_ = (x);                  \
+        if (e_ != hipSuccess) return e_;      \
+    } while (0)
+
+constexpr int kMaxBypass = 8;        // later requests started past a waiting head
+constexpr int kGroupItems = 16;      // MSM items of one prover-core commitment group
 
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -44,11 +65,159 @@ struct Request {
     int shape = 0;
     uint64_t seed = 0;
     double t_submit = 0, t_start = 0, t_done = 0;
+    int bypassed = 0;
     std::vector<uint8_t> commitments;
     zkp_proof_info info{};
     hipError_t err = hipSuccess;
     bool done = false;
 };
+
+// Cross-proof MSM batcher.  A caller's ticket waits until a leader (the first
+// waiting caller while no batch runs) takes every pending ticket that fits
+// the scratch and runs them as one zk_msm_batch pass on the batcher's stream.
+struct Batcher {
+    std::mutex mu;
+    std::condition_variable cv;
+    struct Ticket {
+        const zk_msm_item *items;
+        int count;
+        uint8_t *out;
+        hipEvent_t ready;
+        bool done;
+        hipError_t err;
+    };
+    std::deque<Ticket *> pending;
+    bool running = false;
+    hipStream_t st = nullptr;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    uint8_t *out_dev = nullptr;
+    int max_items = 0;
+    unsigned long long launches = 0, merged = 0, items = 0;
+
+    hipError_t init(size_t bytes, int max_it) {
+        scratch_bytes = bytes;
+        max_items = max_it;
+        SCHECK(hipStreamCreate(&st));
+        SCHECK(hipMalloc(&scratch, bytes));
+        return hipMalloc((void **)&out_dev, (size_t)max_it * ZK_MSM_OUT_BYTES);
+    }
+    void release() {
+        if (st) (void)hipStreamSynchronize(st);
+        if (scratch) (void)hipFree(scratch);
+        if (out_dev) (void)hipFree(out_dev);
+        if (st) (void)hipStreamDestroy(st);
+        scratch = nullptr;
+        out_dev = nullptr;
+        st = nullptr;
+    }
+
+    // one merged pass over the given tickets (no lock held)
+    hipError_t run(const std::vector<Ticket *> &batch) {
+        std::vector<zk_msm_item> all;
+        for (Ticket *t : batch) {
+            SCHECK(hipStreamWaitEvent(st, t->ready, 0));
+            all.insert(all.end(), t->items, t->items + t->count);
+        }
+        SCHECK(zk_msm_batch(out_dev, all.data(), (int)all.size(), 0, scratch, scratch_bytes, st));
+        std::vector<uint8_t> jac(all.size() * 96);
+        for (size_t m = 0; m < all.size(); m++)
+            SCHECK(hipMemcpyAsync(&jac[m * 96], out_dev + m * ZK_MSM_OUT_BYTES, 96, hipMemcpyDeviceToHost, st));
+        SCHECK(hipStreamSynchronize(st));
+        size_t m = 0;
+        for (Ticket *t : batch)
+            for (int i = 0; i < t->count; i++, m++) {
+                const uint8_t *q = &jac[m * 96];
+                bool id = true;
+                for (int b = 64; b < 96; b++) id &= q[b] == 0;
+                if (id)
+                    memset(t->out + (size_t)i * 64, 0, 64);
+                else
+                    memcpy(t->out + (size_t)i * 64, q, 64);
+            }
+        return hipSuccess;
+    }
+
+    hipError_t commit(const zk_msm_item *it, int count, uint8_t *out, hipStream_t caller) {
+        if (count <= 0 || count > max_items) return hipErrorInvalidValue;
+        Ticket t{it, count, out, nullptr, false, hipSuccess};
+        SCHECK(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
+        hipError_t e = hipEventRecord(t.ready, caller);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(t.ready);
+            return e;
+        }
+        std::unique_lock<std::mutex> lk(mu);
+        try {
+            pending.push_back(&t);
+        } catch (...) {
+            lk.unlock();
+            (void)hipEventDestroy(t.ready);
+            return hipErrorOutOfMemory;
+        }
+        while (!t.done) {
+            if (running) {
+                cv.wait(lk);
+                continue;
+            }
+            // lead: every pending ticket whose items still fit, in order
+            running = true;
+            std::vector<Ticket *> batch;
+            size_t nitems = 0;
+            hipError_t r = hipSuccess;
+            try {
+                std::vector<zk_msm_item> probe;
+                for (auto p = pending.begin(); p != pending.end();) {
+                    std::vector<zk_msm_item> next(probe);
+                    next.insert(next.end(), (*p)->items, (*p)->items + (*p)->count);
+                    const size_t need = zk_msm_batch_scratch_bytes(next.data(), (int)next.size(), 0);
+                    if (!batch.empty() && ((int)next.size() > max_items || !need || need > scratch_bytes)) {
+                        ++p;
+                        continue;
+                    }
+                    batch.push_back(*p);
+                    probe.swap(next);
+                    p = pending.erase(p);
+                }
+                nitems = probe.size();
+            } catch (...) {
+                r = hipErrorOutOfMemory;
+            }
+            if (batch.empty()) {  // nothing taken: this ticket fails alone
+                for (auto p = pending.begin(); p != pending.end(); ++p)
+                    if (*p == &t) {
+                        pending.erase(p);
+                        break;
+                    }
+                t.err = r != hipSuccess ? r : hipErrorUnknown;
+                t.done = true;
+                running = false;
+                cv.notify_all();
+                break;
+            }
+            lk.unlock();
+            if (r == hipSuccess) r = zk::guard([&] { return run(batch); });
+            lk.lock();
+            launches++;
+            merged += batch.size() > 1;
+            items += nitems;
+            for (Ticket *b : batch) {
+                b->err = r;
+                b->done = true;
+            }
+            running = false;
+            cv.notify_all();
+        }
+        e = t.err;
+        lk.unlock();
+        (void)hipEventDestroy(t.ready);
+        return e;
+    }
+};
+
+hipError_t batcher_commit(void *ctx, const void *items, int count, uint8_t *out, hipStream_t st) {
+    return ((Batcher *)ctx)->commit((const zk_msm_item *)items, count, out, st);
+}
 
 struct Server {
     std::vector<ShapeState> shapes;
@@ -58,6 +227,7 @@ struct Server {
     std::deque<int> queue;
     size_t budget = 0, used = 0, peak_used = 0;
     int flags = ZKP_BATCHED;
+    bool eager = false;
     int running = 0, peak_running = 0;
     bool stop = false;
     std::mutex mu;
@@ -65,11 +235,18 @@ struct Server {
     std::vector<std::thread> workers;
     std::vector<hipStream_t> streams;
     int device = 0;
+    std::unique_ptr<Batcher> batcher;
 
     // under mu: the first queued request that can start now, with a job
     // reserved for it (popped from the pool, or bytes reserved for a new one)
     bool pick(int &rid, void *&job, bool &fresh, std::vector<void *> &to_free) {
+        if (queue.empty()) return false;
+        // overlap only on backlog: a proof joins those in flight when more
+        // requests wait than are running
+        if (!eager && running > 0 && (int)queue.size() <= running) return false;
+        const bool head_only = reqs[queue.front()].bypassed >= kMaxBypass;
         for (auto it = queue.begin(); it != queue.end(); ++it) {
+            if (head_only && it != queue.begin()) break;
             ShapeState &sh = shapes[reqs[*it].shape];
             if (!sh.idle.empty()) {
                 job = sh.idle.back();
@@ -95,11 +272,28 @@ struct Server {
                 job = nullptr;
                 fresh = true;
             }
+            if (it != queue.begin()) reqs[queue.front()].bypassed++;
             rid = *it;
             queue.erase(it);
             return true;
         }
         return false;
+    }
+
+    // under mu: a finished or failed proof gives back its slot and its job
+    void finish(Request &rq, ShapeState &sh, void *job, hipError_t e) {
+        running--;
+        rq.err = e;
+        rq.t_done = now_ms();
+        rq.done = true;
+        if (job && e == hipSuccess && sh.idle.size() < sh.idle.capacity()) {
+            sh.idle.push_back(job);  // capacity reserved at creation: cannot throw
+        } else {
+            if (job) (void)zkp_job_destroy(job);
+            used -= sh.job_bytes;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
     }
 
     void worker(int w) {
@@ -113,8 +307,17 @@ struct Server {
             Request *req = nullptr;  // element addresses of the deque are stable
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return stop || pick(rid, job, fresh, to_free); });
-                if (rid < 0) return;  // stop with nothing admissible
+                bool got = false;
+                try {
+                    to_free.reserve(64);
+                    cv_work.wait(lk, [&] { return stop || (got = pick(rid, job, fresh, to_free)); });
+                } catch (...) {
+                    got = false;  // a host allocation failed: nothing was taken from the queue
+                }
+                if (!got) {
+                    if (stop) return;
+                    continue;
+                }
                 running++;
                 if (running > peak_running) peak_running = running;
                 req = &reqs[rid];
@@ -125,26 +328,19 @@ struct Server {
             ShapeState &sh = shapes[rq.shape];
             hipError_t e = hipSuccess;
             if (fresh) e = zkp_job_create(sh.pk, &job);
-            if (e == hipSuccess)
-                e = zkp_prove(job, rq.seed, flags, rq.commitments.data(), nullptr, &rq.info, st);
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                running--;
-                rq.err = e;
-                rq.t_done = now_ms();
-                rq.done = true;
-                if (job && e == hipSuccess) {
-                    sh.idle.push_back(job);
-                } else {
-                    if (job) (void)zkp_job_destroy(job);
-                    used -= sh.job_bytes;
-                }
-            }
-            cv_done.notify_all();
-            cv_work.notify_all();
+            if (e == hipSuccess && batcher) e = zk::job_set_commit_fn(job, batcher_commit, batcher.get());
+            if (e == hipSuccess) e = zkp_prove(job, rq.seed, flags, rq.commitments.data(), nullptr, &rq.info, st);
+            if (job && batcher) (void)zk::job_set_commit_fn(job, nullptr, nullptr);
+            std::lock_guard<std::mutex> lk(mu);
+            finish(rq, sh, job, e);
         }
     }
 };
+
+size_t group_scratch(const zkp_circuit &c) {
+    std::vector<zk_msm_item> it(kGroupItems, zk_msm_item{nullptr, nullptr, 1 << c.k, 0});
+    return zk_msm_batch_scratch_bytes(it.data(), kGroupItems, 0);
+}
 
 }  // namespace
 
@@ -152,22 +348,34 @@ extern "C" {
 
 // Builds the keys of every shape (resident) and starts `slots` workers.
 // hbm_budget: bytes the per-proof states may take together (0: what the
-// device reports free after the keys, less 4 GiB).
+// device reports free after the keys and the batcher, less 4 GiB).
 hipError_t zks_create(const zkp_circuit *shapes, int nshapes, int slots, size_t hbm_budget, int flags,
                       unsigned long long key_seed, void **srv_out) {
     return zk::guard([&]() -> hipError_t {
         if (!shapes || nshapes <= 0 || slots <= 0 || slots > 32 || !srv_out) return hipErrorInvalidValue;
-        Server *s = new Server();
+        std::unique_ptr<Server> s(new Server());
         s->flags = flags & (ZKP_BATCHED | ZKP_SHPLONK | ZKP_NO_OPEN);
+        s->eager = (flags & ZKS_EAGER) != 0;
         hipError_t e = hipGetDevice(&s->device);
         s->shapes.resize(nshapes);
+        size_t max_group = 0;
         for (int i = 0; i < nshapes && e == hipSuccess; i++) {
             ShapeState &sh = s->shapes[i];
             sh.circuit = shapes[i];
             sh.job_bytes = zkp_job_bytes(&shapes[i]);
             sh.ncommit = zkp_commitment_count(&shapes[i]);
             if (!sh.job_bytes) e = hipErrorInvalidValue;
-            if (e == hipSuccess) e = zkp_pk_create(&shapes[i], key_seed + (unsigned long long)i, nullptr, &sh.pk);
+            if (e == hipSuccess) {
+                sh.idle.reserve((size_t)slots + 1);
+                const size_t g = group_scratch(shapes[i]);
+                if (g > max_group) max_group = g;
+                e = zkp_pk_create(&shapes[i], key_seed + (unsigned long long)i, nullptr, &sh.pk);
+            }
+        }
+        if (e == hipSuccess && (flags & ZKS_CROSS_BATCH) && (s->flags & ZKP_BATCHED)) {
+            // room for two commitment groups of the largest shape in one pass
+            s->batcher.reset(new Batcher());
+            e = s->batcher->init(2 * max_group + 256, 2 * kGroupItems);
         }
         if (e == hipSuccess) {
             size_t free_b = 0, total_b = 0;
@@ -180,23 +388,24 @@ hipError_t zks_create(const zkp_circuit *shapes, int nshapes, int slots, size_t 
         if (e != hipSuccess) {
             for (auto &sh : s->shapes)
                 if (sh.pk) (void)zkp_pk_destroy(sh.pk);
-            delete s;
+            if (s->batcher) s->batcher->release();
             return e;
         }
         s->streams.assign(slots, nullptr);
         for (int w = 0; w < slots && e == hipSuccess; w++) e = hipStreamCreate(&s->streams[w]);
+        Server *raw = s.release();  // zks_destroy owns it from here
         if (e == hipSuccess) {
             try {
-                for (int w = 0; w < slots; w++) s->workers.emplace_back([s, w] { s->worker(w); });
+                for (int w = 0; w < slots; w++) raw->workers.emplace_back([raw, w] { raw->worker(w); });
             } catch (...) {
                 e = hipErrorUnknown;  // a worker thread could not be started
             }
         }
         if (e != hipSuccess) {
-            (void)zks_destroy(s);  // stops and joins the workers started, frees streams and keys
+            (void)zks_destroy(raw);  // stops and joins the workers started, frees streams and keys
             return e;
         }
-        *srv_out = s;
+        *srv_out = raw;
         return hipSuccess;
     });
 }
@@ -268,6 +477,26 @@ hipError_t zks_stats(void *srv, int *peak_running, size_t *peak_bytes, size_t *b
     return hipSuccess;
 }
 
+// Cross-proof batcher counters: merged passes run, passes that held groups
+// of more than one proof, MSM items committed through it (zeros without
+// ZKS_CROSS_BATCH).
+hipError_t zks_batch_stats(void *srv, unsigned long long *launches, unsigned long long *merged,
+                           unsigned long long *items) {
+    Server *s = (Server *)srv;
+    if (!s) return hipErrorInvalidValue;
+    unsigned long long l = 0, m = 0, i = 0;
+    if (s->batcher) {
+        std::lock_guard<std::mutex> lk(s->batcher->mu);
+        l = s->batcher->launches;
+        m = s->batcher->merged;
+        i = s->batcher->items;
+    }
+    if (launches) *launches = l;
+    if (merged) *merged = m;
+    if (items) *items = i;
+    return hipSuccess;
+}
+
 // Finishes every queued request, stops the workers, frees all device state.
 hipError_t zks_destroy(void *srv) {
     return zk::guard([&]() -> hipError_t {
@@ -290,6 +519,7 @@ hipError_t zks_destroy(void *srv) {
             for (void *j : sh.idle) (void)zkp_job_destroy(j);
             if (sh.pk) (void)zkp_pk_destroy(sh.pk);
         }
+        if (s->batcher) s->batcher->release();
         delete s;
         return hipSuccess;
     });

@@ -30,4 +30,13 @@ hipError_t wit_any_nonzero(const void *col, uint64_t lo, uint64_t hi, uint32_t *
 hipError_t ntt_batch_from(const void *const *src, void *const *buf, void *const *tmp, const void *omegas,
                           const void *n_inv, int log_n, int count, unsigned char *swapped, hipStream_t s);
 
+// Commitment hook of a prover-core job: when set, the job's batched
+// commitments go through fn instead of its own zk_msm_batch call.  fn gets
+// the group's MSM items (inputs ready once `stream` reaches its current
+// position) and returns their affine results in host memory (64 B each,
+// zeros for the identity) after the MSMs have read their inputs.  The proof
+// server sets it to merge the MSMs of concurrent proofs. (prover_core.cpp)
+typedef hipError_t (*CommitFn)(void *ctx, const void *items, int count, uint8_t *out_affine, hipStream_t stream);
+hipError_t job_set_commit_fn(void *job, CommitFn fn, void *ctx);
+
 }  // namespace zk

@@ -44,6 +44,7 @@ TAMPER_GATE = 2
 SHPLONK = 4
 NO_OPEN = 8
 TAMPER_EVAL = 16
+HOST_WITNESS = 32
 
 
 def _decl():
@@ -69,6 +70,8 @@ def _decl():
     L.zkp_proof_bytes.restype = ctypes.c_longlong
     L.zkp_proof_instance.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     L.zkp_pk_vk_repr.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.zkp_set_host_witness.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+    L.zkp_proof_witness.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     return L
 
 
@@ -156,6 +159,21 @@ class Prover:
         check(self.L.zkp_proof_instance(self.handle, out.ctypes.data, n), "zkp_proof_instance")
         return out[:self.pk.circuit.instance]
 
+    def set_host_witness(self, advice_ptrs, instance_ptrs):
+        """host columns (2^k Fr each) the next HOST_WITNESS proof uploads: lists
+        of addresses, kept by the library until that proof returns"""
+        adv = (ctypes.c_void_p * max(len(advice_ptrs), 1))(*advice_ptrs)
+        ins = (ctypes.c_void_p * max(len(instance_ptrs), 1))(*instance_ptrs)
+        self._host_witness = (adv, ins)
+        check(self.L.zkp_set_host_witness(self.handle, adv, len(advice_ptrs), ins, len(instance_ptrs)),
+              "zkp_set_host_witness")
+
+    def witness_to_host(self, advice_ptrs, instance_ptrs):
+        """copies the last proof's advice and instance columns to host addresses"""
+        adv = (ctypes.c_void_p * max(len(advice_ptrs), 1))(*advice_ptrs)
+        ins = (ctypes.c_void_p * max(len(instance_ptrs), 1))(*instance_ptrs)
+        check(self.L.zkp_proof_witness(self.handle, adv, ins), "zkp_proof_witness")
+
     def openings(self):
         """the last proof's evaluations ((count, 4) Montgomery; h(x) first, then
         the written ones in the reference's order) and its opening challenges
@@ -174,4 +192,4 @@ class Prover:
 
 
 __all__ = ["Circuit", "ProofInfo", "ProvingKey", "Prover", "footprint", "BATCHED", "TAMPER_GATE", "SHPLONK",
-           "NO_OPEN", "TAMPER_EVAL", "HipError"]
+           "NO_OPEN", "TAMPER_EVAL", "HOST_WITNESS", "HipError"]

@@ -30,10 +30,15 @@ def _decl():
     L.zks_submit.restype = ctypes.c_int
     L.zks_wait.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     L.zks_stats.argtypes = [ctypes.c_void_p] * 4
+    L.zks_batch_stats.argtypes = [ctypes.c_void_p] * 4
     L.zks_destroy.argtypes = [ctypes.c_void_p]
     L.zkp_commitment_count.restype = ctypes.c_int
     L.zkp_commitment_count.argtypes = [ctypes.c_void_p]
     return L
+
+
+EAGER = 0x100        # ZKS_EAGER: overlap whenever a slot and memory allow
+CROSS_BATCH = 0x200  # ZKS_CROSS_BATCH: merge concurrent proofs' commitment MSMs
 
 
 class Server:
@@ -62,6 +67,12 @@ class Server:
         pr, pb, bud = ctypes.c_int(), ctypes.c_size_t(), ctypes.c_size_t()
         check(self.L.zks_stats(self.handle, ctypes.byref(pr), ctypes.byref(pb), ctypes.byref(bud)), "zks_stats")
         return {"peak_running": pr.value, "peak_bytes": pb.value, "budget_bytes": bud.value}
+
+    def batch_stats(self):
+        l, m, i = ctypes.c_ulonglong(), ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        check(self.L.zks_batch_stats(self.handle, ctypes.byref(l), ctypes.byref(m), ctypes.byref(i)),
+              "zks_batch_stats")
+        return {"merged_passes": l.value, "passes_with_several_proofs": m.value, "msm_items": i.value}
 
     def close(self):
         if self.handle:
