@@ -43,6 +43,11 @@ MULS_PER_BUCKET_ADD = 11
 OPS_PER_MONT_MUL = 136
 
 
+def progress(msg):
+    """one progress line on stderr (the JSON result is the only stdout line)"""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -289,15 +294,18 @@ def main():
         "roofline": roofline,
         "secondary_rooflines": secondary,
     }
+    progress("primitives done: MSM 2^22 %.3f ms" % prim_ms["msm_2^22"])
     if not args.no_batch:
         out["batched"] = {"msm": bench_batched_msm(L, points, max(1, args.steps // 2)),
                           "ntt": bench_batched_ntt(L, args.steps)}
+        progress("batched done")
     if not args.no_proof:
         out["proof_core"] = bench_proof_core([int(k) for k in args.proof_ks.split(",") if k])
     if not args.no_serve:
         out["serving"] = bench_serving(args.serve_count)
     out["host_copy"] = bench_host_copy(prim_ms["msm_2^22"])
     if not args.no_cpu:
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)
 
@@ -427,6 +435,7 @@ def bench_proof_core(ks, reps=5):
     from zkprover.prover import BATCHED, HOST_WITNESS, SHPLONK, Circuit, Prover, ProvingKey, footprint
     res = {}
     for k in ks:
+        progress("proof core k=%d" % k)
         circ = Circuit.benchmark(k)
         pk_b, job_b = footprint(circ)
         pk = ProvingKey(circ, seed=7)
@@ -508,6 +517,7 @@ def bench_serving(count):
                 out["capacity_per_s"] = round(1000.0 / mean_ms, 3)
             res[mode] = {}
             for ld in loads:
+                progress("serving %s load %.1f" % (mode, ld))
                 recs, wall = run_poisson(srv, ld * 1000.0 / mean_ms, count, weights, seed=10)
                 res[mode]["load_%.1f" % ld] = summarize(recs, wall)
             res[mode]["server"] = srv.stats()
