@@ -424,15 +424,18 @@ def bench_proof_core(ks, reps=5):
     2^16 table, 12 permutation columns, 2 shuffles of up to 2^18 rows) at
     each k, after one warm-up proof: the median of `reps` proofs through the
     batched MSM, through one MSM call per polynomial (both opened with GWC),
-    batched and opened with Shplonk, and batched with the advice and instance
-    columns uploaded from pinned host memory during the commitment phase (as
-    create_proof_from_advices receives them).  The device-witness modes
+    batched and opened with Shplonk, batched with the h(X) round replayed
+    from a HIP graph, batched with the advice and instance columns uploaded
+    from pinned host memory during the commitment phase (as
+    create_proof_from_advices receives them), and batched after the MSM
+    window and NTT radix autotuners ran for the proof's sizes.  The device-witness modes
     exclude the witness synthesis (the reference takes the advice as input)."""
     import statistics
 
     import torch
     from zkprover import hostmem
-    from zkprover.prover import BATCHED, HOST_WITNESS, SHPLONK, Circuit, Prover, ProvingKey, footprint
+    from zkprover import autotune
+    from zkprover.prover import BATCHED, GRAPH, HOST_WITNESS, SHPLONK, Circuit, Prover, ProvingKey, footprint
     res = {}
     for k in ks:
         progress("proof core k=%d" % k)
@@ -444,7 +447,9 @@ def bench_proof_core(ks, reps=5):
         try:
             pr.prove(1, flags=BATCHED, want_pieces=False)
             out, ok = {}, True
-            for mode, flags in (("batched", BATCHED), ("plain", 0), ("shplonk_batched", BATCHED | SHPLONK)):
+            pr.prove(1, flags=BATCHED | GRAPH, want_pieces=False)  # captures the h(X) round
+            for mode, flags in (("batched", BATCHED), ("plain", 0), ("shplonk_batched", BATCHED | SHPLONK),
+                                ("graph_batched", BATCHED | GRAPH)):
                 t, info = [], None
                 for r in range(reps):
                     _, _, info = pr.prove(2 + r, flags=flags, want_pieces=False)
@@ -464,6 +469,23 @@ def bench_proof_core(ks, reps=5):
                 t.append(info.total_ms)
                 ok = ok and bool(info.z_ok and info.quotient_ok and info.open_ok)
             out["host_witness"] = (statistics.median(t), info)
+            # the autotuners for this proof's MSM and NTT sizes, then the
+            # batched proof again; the default table is restored afterwards
+            progress("autotune k=%d" % k)
+            tuned_msm = autotune.tune_msm([k], reps=2)
+            tuned_ntt = autotune.tune_ntt([k, k + EXT], reps=3)
+            t = []
+            for r in range(reps):
+                _, _, info = pr.prove(2 + r, flags=BATCHED, want_pieces=False)
+                t.append(info.total_ms)
+                ok = ok and bool(info.z_ok and info.quotient_ok and info.open_ok)
+            out["tuned_batched"] = (statistics.median(t), info)
+            out["tuned_choice"] = {"msm_window": {kk: v[0] for kk, v in tuned_msm.items()},
+                                   "ntt_max_deg": {kk: v[0] for kk, v in tuned_ntt.items()}}
+            from zkprover._lib import call as _call
+            _call("zk_msm_set_window", k, 0)
+            for kk in (k, k + EXT):
+                _call("zk_ntt_set_max_deg", kk, 0)
         finally:
             for b in bufs:
                 b.close()
@@ -475,6 +497,11 @@ def bench_proof_core(ks, reps=5):
                           "speedup": round(out["plain"][0] / out["batched"][0], 3),
                           "shplonk_batched_s": round(out["shplonk_batched"][0] / 1e3, 4),
                           "host_witness_batched_s": round(out["host_witness"][0] / 1e3, 4),
+                          "graph_batched_s": round(out["graph_batched"][0] / 1e3, 4),
+                          "tuned_batched_s": round(out["tuned_batched"][0] / 1e3, 4),
+                          "tuned_choice": out["tuned_choice"],
+                          "h_eval_ms": {"eager": round(ib.phase_ms[4], 3),
+                                        "graph": round(out["graph_batched"][1].phase_ms[4], 3)},
                           "proofs_per_mode": reps, "statistic": "median",
                           "phases_ms": ib.as_dict()["phase_ms"],
                           "host_witness_phases_ms": out["host_witness"][1].as_dict()["phase_ms"],

@@ -544,6 +544,10 @@ hipError_t zkp_job_destroy(void *job);
  * (zkp_set_host_witness) and are uploaded during the commitment phase,
  * overlapped with the MSMs, instead of being synthesised on the device. */
 #define ZKP_HOST_WITNESS 32
+/* ZKP_GRAPH: the h(X) round (coset extensions and constraint kernels) is
+ * captured into a HIP graph on the job's first proof with this flag and
+ * replayed on later ones (same results as the eager launches). */
+#define ZKP_GRAPH 64
 /* The last proof's opening: its evaluations (host, Montgomery Fr) in the
  * reference's order (src/lib.rs:1025-1078): h(x) -- computed, not written --
  * then instance, advice, fixed (q, t, f_j) and the random polynomial at x,

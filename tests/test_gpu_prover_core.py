@@ -278,3 +278,25 @@ def test_prover_core_host_witness_same_proof(core):
     finally:
         for b in bufs:
             b.close()
+
+
+def test_prover_core_graph_replays_equal_eager(gpu):
+    """the h(X) round captured into a HIP graph (ZKP_GRAPH) on a job's first
+    proof and replayed on the next ones, with eager proofs of other seeds and
+    host synchronisations in between: every replayed proof equals the eager
+    proof of the same seed (commitments, quotient, transcript)"""
+    from zkprover.prover import BATCHED, GRAPH, SHPLONK, Prover, ProvingKey
+    pk = ProvingKey(_small(11), seed=5)
+    eager, graph = Prover(pk), Prover(pk)
+    try:
+        for seed, extra in ((61, 0), (62, SHPLONK), (63, 0), (64, 0)):
+            ce, pe, ie = eager.prove(seed, flags=BATCHED | extra)
+            cg, pg, ig = graph.prove(seed, flags=BATCHED | GRAPH | extra)
+            assert ig.z_ok and ig.quotient_ok and ig.open_ok, seed
+            assert (ce == cg).all() and (pe == pg).all(), seed
+            assert bytes(ie.transcript) == bytes(ig.transcript), seed
+            eager.prove(seed + 100, flags=BATCHED)  # other work between the replays
+    finally:
+        eager.close()
+        graph.close()
+        pk.close()

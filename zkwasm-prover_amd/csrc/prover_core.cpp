@@ -522,6 +522,9 @@ struct Job {
     std::vector<const void *> host_adv, host_inst;
     hipStream_t upload = nullptr;
     std::vector<hipEvent_t> upload_ev;
+    bool p4_staged = false;              // the h(X) round's pointer arrays are on the device
+    hipGraphExec_t h_graph = nullptr;    // the captured h(X) round (ZKP_GRAPH)
+    hipStream_t capture = nullptr;       // the stream it is recorded on
     zk::CommitFn commit_fn = nullptr;  // batched commitments through a hook (the proof server's)
     void *commit_ctx = nullptr;
     void *cf(int c) const { return cst + (size_t)c * FR; }
@@ -663,37 +666,55 @@ hipError_t upload_witness(Job &j, size_t ncols, hipStream_t st) {
     return hipSuccess;
 }
 
-// iNTT of each lag column into coef (tmp spare); coef[i] is updated to
-// whichever buffer holds the result, tmp[i] to the other one
-hipError_t intt_all(Job &j, std::vector<void *> &lag, std::vector<void *> &coef, std::vector<void *> &tmp,
-                    hipStream_t st) {
+// Whether an NTT of 2^log_n ends in its spare buffer (an odd pass count).
+bool ntt_ends_in_spare(int log_n) {
+    uint32_t degs[32];
+    return (zk::ntt_plan((uint32_t)log_n, degs) & 1) != 0;
+}
+
+// iNTT of each lag column into coef[i] (tmp[i] a spare).  The buffers are
+// handed over so that the result lands in coef[i] (with an odd pass count
+// coef[i] is the spare), so the buffer of a column never changes between
+// proofs: device pointer arrays and captured graphs stay valid.
+hipError_t intt_all(Job &j, const std::vector<void *> &lag, const std::vector<void *> &coef,
+                    const std::vector<void *> &tmp, hipStream_t st) {
     const Shape &s = j.pk->sh;
     const Pk &pk = *j.pk;
     size_t c = lag.size();
     if (!c) return hipSuccess;
+    const bool odd = ntt_ends_in_spare(s.k);
+    const std::vector<void *> &buf = odd ? tmp : coef, &spare = odd ? coef : tmp;
     // the first pass reads the Lagrange values in place: no copy into coef
     std::vector<unsigned char> sw(c);
-    PCHECK(zk::ntt_batch_from(lag.data(), coef.data(), tmp.data(), pk.w_n_inv, pk.cf(C_NINV), s.k, (int)c, sw.data(),
+    PCHECK(zk::ntt_batch_from(lag.data(), buf.data(), spare.data(), pk.w_n_inv, pk.cf(C_NINV), s.k, (int)c, sw.data(),
                               st));
-    for (size_t i = 0; i < c; i++)
-        if (sw[i]) std::swap(coef[i], tmp[i]);
+    for (size_t i = 0; i < c; i++) {
+        const void *res = sw[i] ? spare[i] : buf[i];
+        if (res != coef[i]) PCHECK(hipMemcpyAsync(coef[i], res, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
+    }
     return hipSuccess;
 }
 
-// extended cosets of coefficient polys into j.ext[first..]; ext pointers are
-// updated to where each result landed (the spare takes the other buffer)
+// extended cosets of coefficient polys into j.ext[first..] (j.ext_tmp spares),
+// the result always in j.ext[i] as in intt_all
 hipError_t extend_all(Job &j, const std::vector<const void *> &coefs, int first, hipStream_t st) {
     const Shape &s = j.pk->sh;
     const Pk &pk = *j.pk;
+    const bool odd = ntt_ends_in_spare(s.ek);
     for (size_t g = 0; g < coefs.size(); g += kExtGroup) {
         int cnt = (int)(coefs.size() - g < (size_t)kExtGroup ? coefs.size() - g : kExtGroup);
         std::vector<zk_coset_item> it(cnt);
-        for (int m = 0; m < cnt; m++)
-            it[m] = zk_coset_item{coefs[g + m], j.ext[first + g + m], j.ext_tmp[m], pk.w_e, s.k, kExtBits};
+        for (int m = 0; m < cnt; m++) {
+            void *dst = j.ext[first + g + m], *spare = j.ext_tmp[m];
+            it[m] = zk_coset_item{coefs[g + m], odd ? spare : dst, odd ? dst : spare, pk.w_e, s.k, kExtBits};
+        }
         std::vector<unsigned char> sw(cnt);
         PCHECK(zk_coset_extend_batch(it.data(), cnt, pk.cf(C_FWD0), 3, sw.data(), st));
-        for (int m = 0; m < cnt; m++)
-            if (sw[m]) std::swap(j.ext[first + g + m], j.ext_tmp[m]);
+        for (int m = 0; m < cnt; m++) {
+            void *res = sw[m] ? it[m].tmp : it[m].buf;
+            if (res != j.ext[first + g + m])
+                PCHECK(hipMemcpyAsync(j.ext[first + g + m], res, (size_t)s.N * FR, hipMemcpyDeviceToDevice, st));
+        }
     }
     return hipSuccess;
 }
@@ -1287,6 +1308,8 @@ hipError_t zkp_job_create(void *pk, void **job_out) {
 hipError_t zkp_job_destroy(void *job) {
     if (!job) return hipSuccess;
     Job *j = (Job *)job;
+    if (j->h_graph) (void)hipGraphExecDestroy(j->h_graph);
+    if (j->capture) (void)hipStreamDestroy(j->capture);
     if (j->upload) (void)hipStreamSynchronize(j->upload);
     for (hipEvent_t e : j->upload_ev) (void)hipEventDestroy(e);
     if (j->upload) (void)hipStreamDestroy(j->upload);
@@ -1546,37 +1569,24 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
             PCHECK(intt_all(j, j.zl_lag, j.zl_coef, j.zl_tmp, st));
             PCHECK(intt_all(j, j.zp_lag, j.zp_coef, j.zp_tmp, st));
             PCHECK(intt_all(j, j.zs_lag, j.zs_coef, j.zs_tmp, st));
-            std::vector<void *> rl{j.rnd_lag}, rc{j.rnd_coef}, rt{j.rnd_tmp};
-            PCHECK(intt_all(j, rl, rc, rt, st));
-            j.rnd_coef = rc[0];
-            j.rnd_tmp = rt[0];
+            PCHECK(intt_all(j, {j.rnd_lag}, {j.rnd_coef}, {j.rnd_tmp}, st));
         }
         PCHECK(mark(3));
 
         // ---- 4. h(X) on the extended coset ----------------------------------------
         // ext slots: gate a,b,c,d | f_l | m_l | zl_l | perm columns (instance,
-        // advice) | zp | shuffle in, sh | zs | coefficient-gate outputs e_j
+        // advice) | zp | shuffle in, sh | zs | coefficient-gate outputs e_j.
+        // Every buffer of the round is fixed for the job, so its device
+        // pointer arrays are staged once, and with ZKP_GRAPH the whole round
+        // (coset extensions and constraint kernels, nothing but launches) is
+        // captured on the job's first proof and replayed on later ones.
         const int E_GATE = 0, E_F = 4, E_M = E_F + s.L, E_ZL = E_M + s.L, E_V = E_ZL + s.L, E_ZP = E_V + s.P,
                   E_IN = E_ZP + s.Sp, E_ZS = E_IN + 2 * s.S, E_EG = E_ZS + s.S;
-        {
-            std::vector<const void *> c;
-            for (int i = 0; i < 4; i++) c.push_back(j.adv_coef[s.col_gate + i]);
-            for (int l = 0; l < s.L; l++) c.push_back(j.adv_coef[s.col_lookup + l]);
-            for (auto p : j.m_coef) c.push_back(p);
-            for (auto p : j.zl_coef) c.push_back(p);
-            for (int i = 0; i < s.P; i++) c.push_back(perm_col(i, true));
-            for (auto p : j.zp_coef) c.push_back(p);
-            for (int t = 0; t < 2 * s.S; t++) c.push_back(j.adv_coef[s.col_shuffle + t]);
-            for (auto p : j.zs_coef) c.push_back(p);
-            for (int g = 0; g < s.FG; g++) c.push_back(j.adv_coef[s.col_fixed_gate + g]);
-            PCHECK(extend_all(j, c, 0, st));
-        }
         void *y = j.cf(J_Y);
         const int rot1 = 1 << kExtBits;  // w X on the coset
         const int N = (int)s.N;
-        PCHECK(zk::fill_zero(j.h, (size_t)s.N * FR, st));
-        // gates, folded in y: y^FG q (a b + c - d) + sum_j y^(FG-1-j) q (f_j a - e_j)
-        {
+        if (!j.p4_staged) {
+            // gates, folded in y: y^FG q (a b + c - d) + sum_j y^(FG-1-j) q (f_j a - e_j)
             std::vector<void *> hv;
             auto term = [&](void *coef, std::initializer_list<void *> cols) {
                 hv.push_back(coef);
@@ -1594,55 +1604,87 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
             std::vector<int> rots(gate_rot_len(s), 0);
             PCHECK(hipMemcpyAsync(pa_gate, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, st));
             PCHECK(hipMemcpyAsync(pa_rots, rots.data(), rots.size() * 4, hipMemcpyHostToDevice, st));
-            PCHECK(field_op_batch_mul_sum(j.h, pa_gate, pa_rots, (int)hv.size(), N, st));
-            PCHECK(hipStreamSynchronize(st));
-        }
-        // permutation
-        if (s.Sp > 0) {
-            PCHECK(zk_permutation_eval_h_p1_on(j.h, j.ext[E_ZP], j.ext[E_ZP + s.Sp - 1], pk.l0_ext, pk.llast_ext, y, N,
-                                               st));
-            std::vector<void *> sets(s.Sp);
+            std::vector<void *> sets(s.Sp + 2 * s.P + 2 * s.S);
             for (int t = 0; t < s.Sp; t++) sets[t] = j.ext[E_ZP + t];
-            PCHECK(hipMemcpyAsync(pa_sets, sets.data(), (size_t)s.Sp * 8, hipMemcpyHostToDevice, st));
-            PCHECK(zk_permutation_eval_h_p2_on(j.h, pa_sets, pk.l0_ext, pk.llast_ext, y, s.Sp, (int)s.u * rot1, N, st));
-            PCHECK(hipStreamSynchronize(st));
-            // per set: the fused product term (sigma and X resident as cosets)
-            std::vector<void *> cols(2 * s.P);
             for (int c = 0; c < s.P; c++) {
-                cols[c] = j.ext[E_V + c];
-                cols[s.P + c] = pk.sig_ext[c];
-                PCHECK(zk_fr_op(j.cf(J_BDELTA + c), beta, pk.cf(C_DELTAS + c), OP_MUL, 1, st));
+                sets[s.Sp + c] = j.ext[E_V + c];
+                sets[s.Sp + s.P + c] = pk.sig_ext[c];
             }
-            PCHECK(hipMemcpyAsync(pa_pv, cols.data(), cols.size() * 8, hipMemcpyHostToDevice, st));
-            for (int t = 0; t < s.Sp; t++) {
-                int c0 = t * kChunk, nc = s.P - c0 < kChunk ? s.P - c0 : kChunk;
-                PCHECK(zk_permutation_eval_h_set(j.h, j.ext[E_ZP + t], pa_pv + c0, pa_ps + c0, nc, beta, gamma,
-                                                 j.cf(J_BDELTA + c0), pk.x_ext, pk.lact_ext, y, rot1, N, st));
-            }
-            PCHECK(hipStreamSynchronize(st));  // the host pointer array
-        }
-        // logup: (t + beta) once (every lookup reads the one table), (f + beta),
-        // product-sum 1
-        if (s.L > 0) PCHECK(field_op(j.aux1, pk.t_ext, 0, nullptr, nullptr, 0, beta, N, 0, st));
-        for (int l = 0; l < s.L; l++) {
-            PCHECK(field_op(j.aux0, j.ext[E_F + l], 0, nullptr, nullptr, 0, beta, N, 0, st));
-            PCHECK(logup_eval_h(j.h, j.aux0, pk.ones_ext, j.aux1, j.ext[E_M + l], j.ext[E_ZL + l], j.ext[E_ZL + l],
-                                pk.l0_ext, pk.llast_ext, pk.lact_ext, y, rot1, N, st));
-        }
-        // shuffle: each group one plain input column and one plain table column,
-        // the case the reference hands to shuffle_eval_h_v2 with betas [beta, ...]
-        // (src/eval_h.rs:1095-1152); the kernel adds beta itself
-        if (s.S > 0) {
-            std::vector<void *> sh(2 * s.S);
             for (int t = 0; t < s.S; t++) {
-                sh[t] = j.ext[E_IN + 2 * t];
-                sh[s.S + t] = j.ext[E_IN + 2 * t + 1];
+                sets[s.Sp + 2 * s.P + t] = j.ext[E_IN + 2 * t];
+                sets[s.Sp + 2 * s.P + s.S + t] = j.ext[E_IN + 2 * t + 1];
             }
-            PCHECK(hipMemcpyAsync(pa_sh, sh.data(), sh.size() * 8, hipMemcpyHostToDevice, st));
+            // pa_sets, pa_pv, pa_ps and pa_sh are consecutive
+            if (!sets.empty())
+                PCHECK(hipMemcpyAsync(pa_sets, sets.data(), sets.size() * 8, hipMemcpyHostToDevice, st));
+            PCHECK(hipStreamSynchronize(st));  // the host arrays are temporaries
+            j.p4_staged = true;
+        }
+        auto h_round = [&](hipStream_t st) -> hipError_t {
+            std::vector<const void *> c;
+            for (int i = 0; i < 4; i++) c.push_back(j.adv_coef[s.col_gate + i]);
+            for (int l = 0; l < s.L; l++) c.push_back(j.adv_coef[s.col_lookup + l]);
+            for (auto p : j.m_coef) c.push_back(p);
+            for (auto p : j.zl_coef) c.push_back(p);
+            for (int i = 0; i < s.P; i++) c.push_back(perm_col(i, true));
+            for (auto p : j.zp_coef) c.push_back(p);
+            for (int t = 0; t < 2 * s.S; t++) c.push_back(j.adv_coef[s.col_shuffle + t]);
+            for (auto p : j.zs_coef) c.push_back(p);
+            for (int g = 0; g < s.FG; g++) c.push_back(j.adv_coef[s.col_fixed_gate + g]);
+            PCHECK(extend_all(j, c, 0, st));
+            PCHECK(zk::fill_zero(j.h, (size_t)s.N * FR, st));
+            PCHECK(field_op_batch_mul_sum(j.h, pa_gate, pa_rots, gate_list_len(s), N, st));
+            // permutation: p1, p2, then per set the fused product term (sigma
+            // and X resident as cosets)
+            if (s.Sp > 0) {
+                PCHECK(zk_permutation_eval_h_p1_on(j.h, j.ext[E_ZP], j.ext[E_ZP + s.Sp - 1], pk.l0_ext, pk.llast_ext,
+                                                   y, N, st));
+                PCHECK(zk_permutation_eval_h_p2_on(j.h, pa_sets, pk.l0_ext, pk.llast_ext, y, s.Sp, (int)s.u * rot1, N,
+                                                   st));
+                for (int c0 = 0; c0 < s.P; c0++)
+                    PCHECK(zk_fr_op(j.cf(J_BDELTA + c0), beta, pk.cf(C_DELTAS + c0), OP_MUL, 1, st));
+                for (int t = 0; t < s.Sp; t++) {
+                    int c0 = t * kChunk, nc = s.P - c0 < kChunk ? s.P - c0 : kChunk;
+                    PCHECK(zk_permutation_eval_h_set(j.h, j.ext[E_ZP + t], pa_pv + c0, pa_ps + c0, nc, beta, gamma,
+                                                     j.cf(J_BDELTA + c0), pk.x_ext, pk.lact_ext, y, rot1, N, st));
+                }
+            }
+            // logup: (t + beta) once (every lookup reads the one table), (f + beta),
+            // product-sum 1
+            if (s.L > 0) PCHECK(field_op(j.aux1, pk.t_ext, 0, nullptr, nullptr, 0, beta, N, 0, st));
+            for (int l = 0; l < s.L; l++) {
+                PCHECK(field_op(j.aux0, j.ext[E_F + l], 0, nullptr, nullptr, 0, beta, N, 0, st));
+                PCHECK(logup_eval_h(j.h, j.aux0, pk.ones_ext, j.aux1, j.ext[E_M + l], j.ext[E_ZL + l],
+                                    j.ext[E_ZL + l], pk.l0_ext, pk.llast_ext, pk.lact_ext, y, rot1, N, st));
+            }
+            // shuffle: each group one plain input column and one plain table
+            // column, the case the reference hands to shuffle_eval_h_v2 with
+            // betas [beta, ...] (src/eval_h.rs:1095-1152); the kernel adds beta
             for (int t = 0; t < s.S; t++)
                 PCHECK(zk_shuffle_eval_h_v2_on(j.h, pa_sh + t, pa_sh + s.S + t, beta, 1, j.ext[E_ZS + t], pk.l0_ext,
                                                pk.llast_ext, pk.lact_ext, y, rot1, N, st));
-            PCHECK(hipStreamSynchronize(st));  // the host pointer array
+            return hipSuccess;
+        };
+        if (flags & ZKP_GRAPH) {
+            if (!j.h_graph) {
+                // recorded on a stream of the job's own (the caller's may be the
+                // legacy stream, which cannot be captured), launched on st
+                if (!j.capture) PCHECK(hipStreamCreateWithFlags(&j.capture, hipStreamNonBlocking));
+                hipGraph_t g = nullptr;
+                PCHECK(hipStreamBeginCapture(j.capture, hipStreamCaptureModeThreadLocal));
+                hipError_t ce = h_round(j.capture);
+                hipError_t ee = hipStreamEndCapture(j.capture, &g);
+                if (ce == hipSuccess) ce = ee;
+                if (ce == hipSuccess) ce = hipGraphInstantiate(&j.h_graph, g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+                if (ce != hipSuccess) {
+                    j.h_graph = nullptr;
+                    return ce;
+                }
+            }
+            PCHECK(hipGraphLaunch(j.h_graph, st));
+        } else {
+            PCHECK(h_round(st));
         }
         PCHECK(mark(4));
 

@@ -8,6 +8,8 @@ namespace zk {
 
 // zero-fill by kernel (no memset node in a captured graph) (ntt.hip)
 hipError_t fill_zero(void *p, size_t bytes, hipStream_t stream);
+// the NTT's radix plan for 2^log_n: pass count (degs: room for 32) (ntt.hip)
+int ntt_plan(uint32_t log_n, uint32_t *degs);
 
 // witness synthesis and helpers for the prover core (witness.hip)
 hipError_t wit_u32_rows_to_fr(void *col, const uint32_t *v, uint32_t rows, hipStream_t s);

@@ -45,6 +45,7 @@ SHPLONK = 4
 NO_OPEN = 8
 TAMPER_EVAL = 16
 HOST_WITNESS = 32
+GRAPH = 64
 
 
 def _decl():
@@ -192,4 +193,4 @@ class Prover:
 
 
 __all__ = ["Circuit", "ProofInfo", "ProvingKey", "Prover", "footprint", "BATCHED", "TAMPER_GATE", "SHPLONK",
-           "NO_OPEN", "TAMPER_EVAL", "HOST_WITNESS", "HipError"]
+           "NO_OPEN", "TAMPER_EVAL", "HOST_WITNESS", "GRAPH", "HipError"]
