@@ -44,7 +44,7 @@ def test_c4_64_uneven_msms_one_batch(gpu, zo):
     assert len(sizes) == 64 and min(sizes) >= 1 << 12 and max(sizes) <= 1 << 20
     nb = 1 << 20
     bases = zo.gen_bases_ap(nb, 808)
-    assert zo.g1a_is_on_curve(bases[:4]).all() and zo.g1a_is_on_curve(bases[-4:]).all()
+    assert all(zo.g1a_is_on_curve(bases[i:i + 1]) for i in (0, 1, 2, 3, nb - 4, nb - 3, nb - 2, nb - 1))
     rng = np.random.default_rng(88)
     offs = [int(rng.integers(0, nb - k + 1)) for k in sizes]
     scal = [zo.rand_fr(k, 8000 + m) for m, k in enumerate(sizes)]
