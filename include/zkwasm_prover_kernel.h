@@ -345,6 +345,22 @@ size_t zk_msm_scratch_bytes(int n, int c);
 hipError_t zk_msm(void *out, const void *points, const void *scalars, int n, int c, void *scratch,
                   size_t scratch_bytes, hipStream_t stream);
 
+/* zk_msm without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The pipeline's scratch is a
+ * device array of ZK_MSM_NOALLOC_SCRATCH_BYTES inside the library (one per
+ * GPU, held until the process ends); n up to ZK_MSM_NOALLOC_MAX_N fits at the
+ * default window (c = 0), and sizes or windows whose scratch exceeds the
+ * array return hipErrorInvalidValue.  Asynchronous: the call only enqueues
+ * kernels on stream (window combination on the device); the result
+ * (Jacobian, z = 1) lands in out[0..96) when the stream reaches it, 96 bytes
+ * of 0xff if the MSM could not finish; out holds ZK_MSM_OUT_BYTES.  Every call
+ * on one GPU shares its array: calls that overlap on one GPU (other streams
+ * or threads) race, so serialise them, e.g. on one stream. */
+#define ZK_MSM_NOALLOC_SCRATCH_BYTES (40u << 20)
+#define ZK_MSM_NOALLOC_MAX_N (1 << 17)
+hipError_t zk_msm_noalloc(void *out, const void *points, const void *scalars, int n, int c, hipStream_t stream);
+
 /* Batched MSM: count independent MSMs (device point / scalar arrays, any
  * sizes; items may share points) through one pass of the pipeline, one
  * launch per phase for the whole batch.  Replaces the per-polynomial loop of

@@ -997,7 +997,15 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
                    const size_t *region_bytes, int nreg, hipStream_t stream, uint8_t ***slots_dev) {
     if (count == 0 || !jobs || nreg < 1 || nreg > 8) return hipErrorInvalidValue;
     if (c != 0 && (c < 2 || c > 17)) return hipErrorInvalidValue;  // sort kernels: <= 8 + 8 bucket bits
-    std::vector<uint32_t> ns(count);
+    // item sizes: on the stack for small batches (no host allocation on the
+    // path of a single MSM)
+    uint32_t ns_small[64];
+    std::vector<uint32_t> ns_big;
+    uint32_t *ns = ns_small;
+    if (count > 64) {
+        ns_big.resize(count);
+        ns = ns_big.data();
+    }
     uint64_t ntot = 0;
     for (uint32_t m = 0; m < count; m++) {
         const MsmJob &j = jobs[m];
@@ -1006,7 +1014,7 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
         ntot += j.n;
     }
     MsmPlan p;
-    if (!choose_plan(ns.data(), count, slice_bytes, c, regions, region_bytes, nreg, p)) return hipErrorOutOfMemory;
+    if (!choose_plan(ns, count, slice_bytes, c, regions, region_bytes, nreg, p)) return hipErrorOutOfMemory;
     c = p.c;
     // phase timing waits on events, which a stream under graph capture cannot
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -1215,6 +1223,19 @@ hipError_t device_collect(uint8_t *const *slices_dev, uint32_t batch, hipStream_
     ZK_CHECK(hipGetLastError());
     k_msm_collect<<<(batch + 63) / 64, 64, 0, stream>>>(slices_dev, batch);
     return hipGetLastError();
+}
+
+// One MSM enqueued on stream with the device tail: no host wait and no host
+// allocation; the result lands in out[0..96) once the stream reaches it
+// (96 bytes of 0xff if the MSM did not finish).  out holds ZK_MSM_OUT_BYTES.
+hipError_t msm_one_async(void *out, const void *points, const void *scalars, uint32_t n, uint32_t c, void *scratch,
+                         size_t scratch_bytes, hipStream_t stream) {
+    uint8_t *regions[1] = {(uint8_t *)scratch};
+    size_t bytes[1] = {scratch_bytes};
+    MsmJob job{(const G1Aff *)points, (const Fr *)scalars, n, (uint8_t *)out};
+    uint8_t **slots = nullptr;
+    ZK_CHECK(msm_run(&job, 1, ZK_MSM_OUT_BYTES, c, regions, bytes, 1, stream, &slots));
+    return device_collect(slots, 1, stream);
 }
 
 hipError_t msm_collect(uint8_t *const *slices_dev, uint32_t batch, hipStream_t stream) {

@@ -38,6 +38,11 @@ hipError_t ntt_batch_from(const void *const *src, void *const *buf, void *const 
 // position) and returns their affine results in host memory (64 B each,
 // zeros for the identity) after the MSMs have read their inputs.  The proof
 // server sets it to merge the MSMs of concurrent proofs. (prover_core.cpp)
+// One MSM through the pipeline with the device tail, enqueued on stream
+// without host waits or host allocation (msm.hip)
+hipError_t msm_one_async(void *out, const void *points, const void *scalars, uint32_t n, uint32_t c, void *scratch,
+                         size_t scratch_bytes, hipStream_t stream);
+
 typedef hipError_t (*CommitFn)(void *ctx, const void *items, int count, uint8_t *out_affine, hipStream_t stream);
 hipError_t job_set_commit_fn(void *job, CommitFn fn, void *ctx);
 
