@@ -15,6 +15,7 @@
 // `expand_omega_buffer` builds (one gather per element, no extra multiply);
 // the 2^(d-1) in-pass twiddles are staged once per workgroup in LDS.  The
 // n^-1 scaling of the iNTT is fused into the last pass.
+#include <stdlib.h>
 #include <string.h>
 
 #include <atomic>
@@ -50,9 +51,184 @@ struct NttJob {
 };
 constexpr uint32_t kJobsPerLaunch = 48;
 struct NttPassArgs {
-    uint32_t count, pad;
+    uint32_t count, reg_rounds;
     NttJob job[kJobsPerLaunch];
 };
+
+// ---- the in-tile DFT with each wave's 256 elements in registers ----------
+// A wave holds 256 consecutive tile elements e = (group << deg) | i, four per
+// lane.  Eight "slots" address them: lane bits 0..5 and register bits
+// rho0, rho1 (register r = rho0 + 2 rho1).  Radix-2 DIF rounds run from the
+// top bit of i down; a round whose bit sits in a register slot pairs
+// registers of one lane, a round whose bit sits in lane slot L first swaps
+// that lane bit with a register slot whose bit is done: lane distance 32 and
+// 16 by v_permlane32_swap / v_permlane16_swap, 8 / 4 / 2 / 1 by DPP
+// exchanges.  No LDS traffic and no barrier between rounds.
+//
+// Initial slots: lane bit b holds e-bit b for b <= deg - 3 (the low bits of
+// i, so consecutive lanes read consecutive LDS elements) and e-bit b + 2
+// above (the group bits); rho0 = e-bit deg - 2, rho1 = e-bit deg - 1.  Round
+// r >= 2 handles i-bit deg - 1 - r, found in lane slot deg - 1 - r, and
+// swaps it with rho0 (r even) or rho1 (r odd).
+struct SlotMap {
+    int e[8];  // e-bit held by slot: 0..5 lane bits, 6 rho0, 7 rho1
+};
+__host__ __device__ constexpr SlotMap slots_initial(int deg) {
+    SlotMap s{};
+    for (int b = 0; b < 6; b++) s.e[b] = b <= deg - 3 ? b : b + 2;
+    s.e[6] = deg - 2;
+    s.e[7] = deg - 1;
+    return s;
+}
+__host__ __device__ constexpr SlotMap slots_final(int deg) {
+    SlotMap s = slots_initial(deg);
+    for (int r = 2; r < deg; r++) {
+        const int b = deg - 1 - r, k = 6 + (r & 1);
+        const int t = s.e[b];
+        s.e[b] = s.e[k];
+        s.e[k] = t;
+    }
+    return s;
+}
+template <int DEG, bool FINAL>
+__device__ __forceinline__ uint32_t slot_elem(uint32_t lane, uint32_t reg) {
+    constexpr SlotMap m = FINAL ? slots_final(DEG) : slots_initial(DEG);
+    uint32_t e = 0;
+#pragma unroll
+    for (int b = 0; b < 6; b++) e |= ((lane >> b) & 1u) << m.e[b];
+    e |= (reg & 1u) << m.e[6];
+    e |= (reg >> 1) << m.e[7];
+    return e;
+}
+
+// v from lane (lane ^ 2^L), L <= 3, by DPP within rows of 16 lanes:
+// xor 1 / xor 2 as quad permutations, xor 4 = half-row mirror of the
+// reversed quads, xor 8 = half-row mirror of the row mirror
+template <int L>
+__device__ __forceinline__ uint32_t lanes_xor(uint32_t v) {
+    const int x = (int)v;
+    if constexpr (L == 0) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (L == 1) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (L == 2) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x1B, 0xF, 0xF, false), 0x141, 0xF,
+                                                  0xF, false);
+    } else {
+        return (uint32_t)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, false), 0x141, 0xF,
+                                                  0xF, false);
+    }
+}
+
+// Swaps lane bit L with the register bit that tells ra (bit 0) from rb
+// (bit 1): afterwards ra / rb of a lane with lane bit L = 0 hold the old
+// ra of this lane / of its partner, and with lane bit L = 1 the old rb of
+// the partner / of this lane.
+template <int L>
+__device__ __forceinline__ void swap_lane_reg(Fr &ra, Fr &rb, uint32_t lane) {
+    if constexpr (L == 5) {
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const auto r = __builtin_amdgcn_permlane32_swap(ra.v[d], rb.v[d], false, false);
+            ra.v[d] = r[0];
+            rb.v[d] = r[1];
+        }
+    } else if constexpr (L == 4) {
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const auto r = __builtin_amdgcn_permlane16_swap(ra.v[d], rb.v[d], false, false);
+            ra.v[d] = r[0];
+            rb.v[d] = r[1];
+        }
+    } else {
+        const bool up = (lane >> L) & 1u;
+#pragma unroll
+        for (int d = 0; d < 8; d++) {
+            const uint32_t recv = lanes_xor<L>(up ? ra.v[d] : rb.v[d]);
+            // two selects, not a store through a selected address (which
+            // would move ra / rb to scratch memory)
+            ra.v[d] = up ? recv : ra.v[d];
+            rb.v[d] = up ? rb.v[d] : recv;
+        }
+    }
+}
+
+template <bool MUL>
+__device__ __forceinline__ void butterfly(Fr &a, Fr &c, const Fr &w) {
+    const Fr s = a + c;
+    Fr d = a - c;
+    if (MUL) d = d * w;
+    a = s;
+    c = d;
+}
+
+// Rounds RND..DEG-1 (RND >= 2), one template instance per round so that
+// every register index and lane distance is a compile-time constant.
+template <int DEG, int RND>
+__device__ __forceinline__ void dft_rounds(Fr &r0, Fr &r1, Fr &r2, Fr &r3, const Fr *tw, uint32_t lane) {
+    if constexpr (RND < DEG) {
+        constexpr int b = DEG - 1 - RND;  // i-bit of this round, in lane slot b
+        if constexpr ((RND & 1) == 0) {   // swap with rho0: pairs (0,1), (2,3)
+            swap_lane_reg<b>(r0, r1, lane);
+            swap_lane_reg<b>(r2, r3, lane);
+            if constexpr (b > 0) {
+                const Fr w = tw[(lane & ((1u << b) - 1)) << RND];
+                butterfly<true>(r0, r1, w);
+                butterfly<true>(r2, r3, w);
+            } else {  // last round: twiddle 1
+                butterfly<false>(r0, r1, r0);
+                butterfly<false>(r2, r3, r0);
+            }
+        } else {  // swap with rho1: pairs (0,2), (1,3)
+            swap_lane_reg<b>(r0, r2, lane);
+            swap_lane_reg<b>(r1, r3, lane);
+            if constexpr (b > 0) {
+                const Fr w = tw[(lane & ((1u << b) - 1)) << RND];
+                butterfly<true>(r0, r2, w);
+                butterfly<true>(r1, r3, w);
+            } else {
+                butterfly<false>(r0, r2, r0);
+                butterfly<false>(r1, r3, r0);
+            }
+        }
+        dft_rounds<DEG, RND + 1>(r0, r1, r2, r3, tw, lane);
+    }
+}
+
+// The DIF of every group of 2^DEG in the wave's 256 elements (s: the wave's
+// LDS region, e-indexed; tw: w_D^j for j < D/2).  Results go back to s in
+// the layout the radix-2 LDS rounds leave (U[bitrev(i)] at i).
+template <int DEG>
+__device__ __forceinline__ void wave_dft_regs(Fr *s, const Fr *tw, uint32_t lane) {
+    static_assert(DEG >= 2 && DEG <= 8, "register DFT covers 2 <= deg <= 8");
+    Fr r0 = s[slot_elem<DEG, false>(lane, 0)], r1 = s[slot_elem<DEG, false>(lane, 1)];
+    Fr r2 = s[slot_elem<DEG, false>(lane, 2)], r3 = s[slot_elem<DEG, false>(lane, 3)];
+    const uint32_t lo = lane & ((1u << (DEG - 2)) - 1);
+    // round 0: i-bit DEG-1 in rho1; pairs (0,2) and (1,3)
+    butterfly<true>(r0, r2, tw[lo]);
+    butterfly<true>(r1, r3, tw[lo + (1u << (DEG - 2))]);
+    // round 1: i-bit DEG-2 in rho0; pairs (0,1) and (2,3)
+    if constexpr (DEG == 2) {
+        butterfly<false>(r0, r1, r0);
+        butterfly<false>(r2, r3, r0);
+    } else {
+        const Fr w = tw[lo << 1];
+        butterfly<true>(r0, r1, w);
+        butterfly<true>(r2, r3, w);
+    }
+    dft_rounds<DEG, 2>(r0, r1, r2, r3, tw, lane);
+    s[slot_elem<DEG, true>(lane, 0)] = r0;
+    s[slot_elem<DEG, true>(lane, 1)] = r1;
+    s[slot_elem<DEG, true>(lane, 2)] = r2;
+    s[slot_elem<DEG, true>(lane, 3)] = r3;
+}
+
+// Register rounds for 1024-element tiles of degree 2..8 (the radix-2 LDS
+// rounds below stay for smaller tiles and degree 1).  ZK_NTT_ROUNDS=lds
+// selects the LDS rounds everywhere (A/B measurement).
+__host__ __device__ inline bool use_register_rounds(uint32_t tile, uint32_t deg) {
+    return tile == kTile && deg >= 2 && deg <= 8;
+}
 
 __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
     extern __shared__ Fr lds[];
@@ -106,21 +282,37 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
     }
     __syncthreads();
 
-    for (uint32_t rnd = 0; rnd < deg; rnd++) {
-        const uint32_t bit = (D >> 1) >> rnd;
-        for (uint32_t b = tid; b < tile / 2; b += bs) {
-            uint32_t g = b >> (deg - 1);
-            uint32_t ii = b & (D / 2 - 1);
-            uint32_t di = ii & (bit - 1);
-            uint32_t i0 = (ii << 1) - di;
-            uint32_t i1 = i0 + bit;
-            Fr a = s[(g << deg) + i0], c = s[(g << deg) + i1];
-            s[(g << deg) + i0] = a + c;
-            Fr d = a - c;
-            if (di) d = d * tw[di << rnd];
-            s[(g << deg) + i1] = d;
+    if (a.reg_rounds && use_register_rounds(tile, deg)) {
+        // bs == kThreads: four waves, each owning 256 elements of the tile
+        Fr *ws = s + ((tid >> 6) << 8);
+        const uint32_t lane = tid & 63;
+        switch (deg) {
+            case 2: wave_dft_regs<2>(ws, tw, lane); break;
+            case 3: wave_dft_regs<3>(ws, tw, lane); break;
+            case 4: wave_dft_regs<4>(ws, tw, lane); break;
+            case 5: wave_dft_regs<5>(ws, tw, lane); break;
+            case 6: wave_dft_regs<6>(ws, tw, lane); break;
+            case 7: wave_dft_regs<7>(ws, tw, lane); break;
+            default: wave_dft_regs<8>(ws, tw, lane); break;
         }
         __syncthreads();
+    } else {
+        for (uint32_t rnd = 0; rnd < deg; rnd++) {
+            const uint32_t bit = (D >> 1) >> rnd;
+            for (uint32_t b = tid; b < tile / 2; b += bs) {
+                uint32_t g = b >> (deg - 1);
+                uint32_t ii = b & (D / 2 - 1);
+                uint32_t di = ii & (bit - 1);
+                uint32_t i0 = (ii << 1) - di;
+                uint32_t i1 = i0 + bit;
+                Fr a = s[(g << deg) + i0], c = s[(g << deg) + i1];
+                s[(g << deg) + i0] = a + c;
+                Fr d = a - c;
+                if (di) d = d * tw[di << rnd];
+                s[(g << deg) + i1] = d;
+            }
+            __syncthreads();
+        }
     }
 
     const uint32_t lp = log_p < log_g ? log_p : log_g;
@@ -213,6 +405,15 @@ struct NttSpec {
 // Runs a batch of NTTs: pass r of every transform that has one goes into the
 // same launch(es), kJobsPerLaunch transforms per launch.  passes[m] receives
 // transform m's pass count (its result is in b when odd).
+// ZK_NTT_ROUNDS=lds: radix-2 rounds through LDS for every tile (A/B runs)
+static uint32_t register_rounds_enabled() {
+    static const uint32_t on = [] {
+        const char *e = getenv("ZK_NTT_ROUNDS");
+        return (e && strcmp(e, "lds") == 0) ? 0u : 1u;
+    }();
+    return on;
+}
+
 hipError_t ntt_run_batch(const NttSpec *specs, uint32_t count, hipStream_t stream, int *passes) {
     std::vector<uint32_t> degs((size_t)count * 32);
     std::vector<int> np(count);
@@ -226,6 +427,7 @@ hipError_t ntt_run_batch(const NttSpec *specs, uint32_t count, hipStream_t strea
     for (int r = 0; r < rounds; r++) {
         NttPassArgs args;
         memset(&args, 0, sizeof(args));
+        args.reg_rounds = register_rounds_enabled();
         uint32_t blocks = 0;
         size_t lds = 0;
         auto flush = [&]() -> hipError_t {
@@ -233,6 +435,7 @@ hipError_t ntt_run_batch(const NttSpec *specs, uint32_t count, hipStream_t strea
             k_ntt_pass<<<blocks, kThreads, lds, stream>>>(args);
             ZK_CHECK(hipGetLastError());
             memset(&args, 0, sizeof(args));
+            args.reg_rounds = register_rounds_enabled();
             blocks = 0;
             lds = 0;
             return hipSuccess;
