@@ -59,15 +59,21 @@ using Fr = Fe<FrParams>;
 using Fp = Fe<FpParams>;
 
 // ---- limb helpers --------------------------------------------------------
+// One 32-bit add / subtract with carry (borrow) in and out, 0 or 1: the
+// builtins become v_add_co / v_addc_co (v_sub_co / v_subb_co) chains through
+// the carry mask; the same written with 64-bit sums compiled to 64-bit adds
+// and moves, about three times the instructions per limb.
 __device__ __forceinline__ uint32_t add_cc(uint32_t a, uint32_t b, uint32_t &carry) {
-    uint64_t s = (uint64_t)a + b + carry;
-    carry = (uint32_t)(s >> 32);
-    return (uint32_t)s;
+    uint32_t c;
+    const uint32_t r = __builtin_addc(a, b, carry, &c);
+    carry = c;
+    return r;
 }
 __device__ __forceinline__ uint32_t sub_bb(uint32_t a, uint32_t b, uint32_t &borrow) {
-    uint64_t s = (uint64_t)a - b - borrow;
-    borrow = (uint32_t)(s >> 63);
-    return (uint32_t)s;
+    uint32_t c;
+    const uint32_t r = __builtin_subc(a, b, borrow, &c);
+    borrow = c;
+    return r;
 }
 
 template <class P>
