@@ -5,8 +5,8 @@
 // src/lib.rs:590-745), plus zk_batch_invert.
 //
 // Batch inversion is Montgomery's trick in three launches over tiles of 4096
-// elements (prefix products into the caller's temp, one workgroup inverting
-// the tile totals with a single field inversion, then two products per
+// elements (prefix products into the caller's temp, every tile's total
+// inverted by its own thread of a small launch, then two products per
 // element on the way back; details at k_binv_up below).  Above 2^24 elements
 // a multi-level hierarchy takes over: level 0 thread t owns the m elements
 // t, t + T, t + 2T, ... (T = n/m, coalesced) and writes only their product;
@@ -134,9 +134,8 @@ __global__ __launch_bounds__(256) void k_binv_top(const Fr *__restrict__ x, cons
 //         first element is not needed: that prefix is 1), the 256 thread
 //         products into an LDS product tree; the tile's total goes into the
 //         free slot pre[b * 4096].
-//   top:  one workgroup inverts the T tile totals (per-thread prefixes in
-//         registers, a product tree, one field inversion) and writes each
-//         inverse back into its tile's slot.
+//   tiles: one thread per tile inverts its total in place (T independent
+//         binary-GCD inversions side by side, no product tree over tiles).
 //   down: each tile rebuilds its thread products (pre of the last element
 //         times that element), the tree, walks it down from the tile's
 //         inverse, and each thread emits its elements' inverses backwards
@@ -146,7 +145,7 @@ __global__ __launch_bounds__(256) void k_binv_top(const Fr *__restrict__ x, cons
 constexpr uint32_t kBT = 256;
 constexpr int kBM = 16;
 constexpr uint32_t kTile = kBT * kBM;
-constexpr uint32_t kTopMax = kBT * kBM;  // tile totals one workgroup inverts
+constexpr uint32_t kTopMax = kBT * kBM;  // tiles up to which the tiled path runs (n <= 2^24)
 
 // LDS product tree over kBT leaves at [kBT, 2 kBT): tree[1] = their product
 __device__ __forceinline__ void tree_up(Fr *tree) {
@@ -192,37 +191,14 @@ __global__ __launch_bounds__(kBT) void k_binv_up(const Fr *__restrict__ x, const
     if (t == 0) fe_store<FrParams>(&pre[base], tree[1]);
 }
 
-// T tile totals at pre[b * kTile] -> their inverses, in place; c = ceil(T / kBT)
-__global__ __launch_bounds__(kBT) void k_binv_top2(Fr *pre, uint32_t T, uint32_t c) {
-    __shared__ Fr tree[2 * kBT];
-    const uint32_t t = threadIdx.x;
-    Fr v[kBM], pr[kBM];
-    Fr acc = fe_one<FrParams>();
-#pragma unroll
-    for (int k = 0; k < kBM; k++) {
-        const uint32_t b = t * c + (uint32_t)k;
-        v[k] = fe_one<FrParams>();
-        pr[k] = acc;
-        if ((uint32_t)k < c && b < T) {
-            v[k] = fe_load<FrParams>(&pre[(size_t)b * kTile]);
-            acc = acc * v[k];
-        }
-    }
-    tree[kBT + t] = acc;
-    __syncthreads();
-    tree_up(tree);
-    if (t == 0) tree[1] = fe_inv<FrParams>(tree[1]);
-    __syncthreads();
-    tree_down(tree);
-    acc = tree[kBT + t];
-#pragma unroll
-    for (int k = kBM - 1; k >= 0; k--) {
-        const uint32_t b = t * c + (uint32_t)k;
-        if ((uint32_t)k < c && b < T) {
-            fe_store<FrParams>(&pre[(size_t)b * kTile], acc * pr[k]);
-            acc = acc * v[k];
-        }
-    }
+// the T tile totals at pre[b * kTile] -> their inverses, in place: one
+// independent inversion per thread (a separate launch keeps the inversion's
+// registers out of the streaming kernels)
+__global__ __launch_bounds__(64) void k_binv_tiles(Fr *pre, uint32_t T) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= T) return;
+    Fr *p = &pre[(size_t)b * kTile];
+    fe_store<FrParams>(p, fe_inv<FrParams>(fe_load<FrParams>(p)));
 }
 
 template <int MODE>
@@ -398,7 +374,7 @@ hipError_t batch_invert_ex(const Fr *x, const Fr *beta, Fr *out, int mode, uint3
         const uint32_t T = (n + kTile - 1) / kTile;
         k_binv_up<<<T, kBT, 0, stream>>>(x, beta, temp, n);
         ZK_CHECK(hipGetLastError());
-        k_binv_top2<<<1, kBT, 0, stream>>>(temp, T, (T + kBT - 1) / kBT);
+        k_binv_tiles<<<(T + 63) / 64, 64, 0, stream>>>(temp, T);
         ZK_CHECK(hipGetLastError());
         if (mode == OUT_ADD)
             k_binv_down<OUT_ADD><<<T, kBT, 0, stream>>>(x, beta, temp, out, n);
