@@ -208,6 +208,14 @@ hipError_t shuffle_eval_h_v2(void *res, const void *inputs, const void *tables, 
  * scratch; any n >= 1 (the reference needs a power of two and leaves res
  * unwritten for n = 1). */
 hipError_t poly_eval(const void *p, void *res, const void *x, int n, hipStream_t stream);
+/* Many polynomials at one point (replaces the per-polynomial poly_eval loop of
+ * batch_poly_eval, src/eval_poly.rs:18-140): polys is a HOST array of count
+ * device pointers (n coefficients each), out a HOST array of count device
+ * pointers, each receiving one Fr, p_m(x); x is a device Fr.  scratch holds
+ * zk_poly_eval_batch_scratch_bytes(n, count) bytes.  Asynchronous. */
+size_t zk_poly_eval_batch_scratch_bytes(int n, int count);
+hipError_t zk_poly_eval_batch(const void *const *polys, void *const *out, int count, const void *x, int n,
+                              void *scratch, hipStream_t stream);
 
 /* bn254_c.rs:208 `shplonk_h_x_merge`: res[i] = res[i]*v
  * + values[i] * prod_{j < diff_points_n} (omegas[i] - diff_points[j]).

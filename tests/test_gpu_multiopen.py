@@ -62,6 +62,45 @@ def test_poly_eval_any_length(gpu, zo, n):
     assert zo.from_mont(_gpu_poly_eval(p, xt))[0] == _horner_ints(zo, p, x)
 
 
+def _gpu_poly_eval_batch(polys, xt):
+    """zk_poly_eval_batch over equal-length polynomials; returns (count, 4)"""
+    dv, call, lib = _dv()
+    n, count = len(polys[0]), len(polys)
+    dps = [dv.to_dev(p) for p in polys]
+    dx = dv.to_dev(xt)
+    out = dv.zeros(count)
+    L = lib()
+    L.zk_poly_eval_batch_scratch_bytes.restype = ctypes.c_size_t
+    L.zk_poly_eval_batch_scratch_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.zk_poly_eval_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p]
+    nbytes = L.zk_poly_eval_batch_scratch_bytes(n, count)
+    import torch
+    scratch = torch.empty(nbytes, dtype=torch.uint8, device=dv.device())
+    pa = (ctypes.c_void_p * count)(*[d.data_ptr() for d in dps])
+    oa = (ctypes.c_void_p * count)(*[out.data_ptr() + 32 * m for m in range(count)])
+    call("zk_poly_eval_batch", ctypes.addressof(pa), ctypes.addressof(oa), count, dv.ptr(dx), n, scratch.data_ptr(),
+         dv.stream_handle())
+    return dv.from_dev(out)
+
+
+@pytest.mark.parametrize("n,count", [(1, 3), (3, 2), (2047, 3), (2049, 3), (5000, 2), (4096, 70), (1 << 16, 4),
+                                     ((1 << 23) + (1 << 12) + 5, 2)])
+def test_poly_eval_batch_matches_oracle(gpu, zo, n, count):
+    """many polynomials at one point in one call (more than the 64 of one
+    launch at count 70; two chunks of block values at 2^23 + 2^12 + 5), each
+    equal to the oracle's poly_eval"""
+    polys = [zo.rand_fr(n, 7000 + 31 * m + n % 1000) for m in range(count)]
+    lg = _log2_ceil(n)
+    xt = zo.x_table(zo.rand_fr(1, 300 + count), lg)
+    got = _gpu_poly_eval_batch(polys, xt)
+    for m in range(count):
+        # the oracle takes power-of-two lengths: zero coefficients up to 2^lg
+        padded = zo.fr_zeros(1 << lg)
+        padded[:n] = polys[m]
+        assert (got[m:m + 1] == zo.poly_eval(padded, xt)).all(), (n, m)
+
+
 def test_poly_eval_special_points(gpu, zo):
     n = 1 << 13
     p = zo.rand_fr(n, 1)

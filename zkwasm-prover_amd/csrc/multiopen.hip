@@ -25,6 +25,8 @@
 // a product tree over the 256 thread products in LDS, one binary-Euclid
 // inverse per workgroup, then prefix products in registers to split each
 // thread's inverse over its rows.
+#include <string.h>
+
 #include "zkcore.h"
 
 using namespace zk;
@@ -263,6 +265,89 @@ __global__ __launch_bounds__(kT) void k_h_x_div_points(Fr *values, const Fr *__r
     }
 }
 
+// ---- many polynomials at one point ---------------------------------------
+// p_m(x) for up to kEvalItems polynomials of n coefficients per launch pair.
+// k_peval_pows: pw[k] = x^(2^k), k < 32, by one thread (31 squarings, once
+// per point).  k_peval_blocks, block (b, m): thread t folds its 8 coefficients
+// b * 2048 + 8t .. + 7 by Horner with x, then a tree over the 256 thread
+// values with factors x^(8 * 2^s) leaves the block's chunk value
+// sum_j a_(b*2048+j) x^j.  k_peval_combine, one workgroup per polynomial, folds
+// the nb chunk values with W = x^2048 the same way (8 per thread, then the
+// tree), and writes p_m(x) to the item's output.  One read of every
+// coefficient; no carries or suffix values are written.
+constexpr int kEvalItems = 64;
+struct EvalArgs {
+    const Fr *poly[kEvalItems];
+    Fr *out[kEvalItems];
+};
+
+__global__ void k_peval_pows(const Fr *__restrict__ xp, Fr *__restrict__ pw) {
+    if (threadIdx.x != 0) return;
+    Fr p = fe_load<FrParams>(xp);
+    for (int k = 0; k < 32; k++) {
+        fe_store<FrParams>(&pw[k], p);
+        p = p * p;
+    }
+}
+
+// sum_t v_t x^(8t) over the block's 256 thread values (tree in LDS; the
+// factor of level s is pw[3 + s] = x^(8 * 2^s) relative to the step e^lvl)
+__device__ __forceinline__ Fr block_fold(Fr v, Fr *S, const Fr *pw, int first_pow) {
+    const uint32_t t = threadIdx.x;
+    S[t] = v;
+    __syncthreads();
+#pragma unroll 1
+    for (int s = 0; s < kLogT; s++) {
+        const uint32_t d = 1u << s;
+        if ((t & (2 * d - 1)) == 0) S[t] = S[t] + fe_load<FrParams>(&pw[first_pow + s]) * S[t + d];
+        __syncthreads();
+    }
+    return S[0];
+}
+
+__global__ __launch_bounds__(kT) void k_peval_blocks(const EvalArgs a, uint32_t n, uint32_t nb,
+                                                     const Fr *__restrict__ pw, Fr *__restrict__ tot) {
+    __shared__ Fr S[kT];
+    const uint32_t b = blockIdx.x, m = blockIdx.y;
+    const Fr *__restrict__ src = a.poly[m];
+    const Fr x = fe_load<FrParams>(&pw[0]);
+    const uint64_t t0 = (uint64_t)b * kBlk + (uint64_t)threadIdx.x * kE;
+    Fr L = fe_zero<FrParams>();
+#pragma unroll
+    for (int e = kE - 1; e >= 0; e--) {
+        const Fr c = t0 + e < n ? fe_load<FrParams>(&src[t0 + e]) : fe_zero<FrParams>();
+        L = c + x * L;
+    }
+    const Fr v = block_fold(L, S, pw, 3);
+    if (threadIdx.x == 0) fe_store<FrParams>(&tot[(size_t)m * nb + b], v);
+}
+
+__global__ __launch_bounds__(kT) void k_peval_combine(const EvalArgs a, uint32_t nb, const Fr *__restrict__ pw,
+                                                      const Fr *__restrict__ tot) {
+    __shared__ Fr S[kT];
+    const uint32_t m = blockIdx.x;
+    const Fr W = fe_load<FrParams>(&pw[11]);  // x^2048
+    Fr acc = fe_zero<FrParams>();
+    // chunks of kBlk block values: thread t folds values 8t .. 8t + 7 of the
+    // chunk, the tree folds the threads (factors W^(8 * 2^s) = x^(2^(14 + s)))
+    const uint32_t nchunks = ceil_div(nb, kBlk);
+    Fr Wc = fe_one<FrParams>();  // W^(kBlk * c) for chunk c
+    for (uint32_t c = 0; c < nchunks; c++) {
+        const uint32_t t0 = c * kBlk + threadIdx.x * kE;
+        Fr L = fe_zero<FrParams>();
+#pragma unroll
+        for (int e = kE - 1; e >= 0; e--) {
+            const Fr v = t0 + e < nb ? fe_load<FrParams>(&tot[(size_t)m * nb + t0 + e]) : fe_zero<FrParams>();
+            L = v + W * L;
+        }
+        const Fr part = block_fold(L, S, pw, 14);
+        acc = acc + Wc * part;
+        Wc = Wc * fe_load<FrParams>(&pw[22]);  // W^2048 = x^(2^22)
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) fe_store<FrParams>(a.out[m], acc);
+}
+
 uint32_t grid_for(uint32_t n) {
     uint32_t b = ceil_div(n, kT);
     return b < 16384 ? (b ? b : 1) : 16384;
@@ -305,6 +390,39 @@ hipError_t poly_eval(const void *p, void *res, const void *x, int n, hipStream_t
     ZK_CHECK(hipGetLastError());
     k_sfx_top<<<1, kT, 0, stream>>>(tot, nb, (const Fr *)x, nullptr, (Fr *)res);
     return hipGetLastError();
+}
+
+// count polynomials (device pointers in the host array polys, n
+// coefficients each) at the point x (device Fr); p_m(x) goes to out[m] (device
+// pointers, host array).  scratch: zk_poly_eval_batch_scratch_bytes(n, count).
+size_t zk_poly_eval_batch_scratch_bytes(int n, int count) {
+    if (n < 1 || count < 1) return 0;
+    return (32 + (size_t)count * ceil_div((uint32_t)n, kBlk)) * sizeof(Fr);
+}
+
+hipError_t zk_poly_eval_batch(const void *const *polys, void *const *out, int count, const void *x, int n,
+                              void *scratch, hipStream_t stream) {
+    if (!polys || !out || count < 1 || !x || n < 1 || !scratch) return hipErrorInvalidValue;
+    for (int m = 0; m < count; m++)
+        if (!polys[m] || !out[m]) return hipErrorInvalidValue;
+    const uint32_t nb = ceil_div((uint32_t)n, kBlk);
+    Fr *pw = (Fr *)scratch, *tot = pw + 32;
+    k_peval_pows<<<1, 64, 0, stream>>>((const Fr *)x, pw);
+    ZK_CHECK(hipGetLastError());
+    for (int first = 0; first < count; first += kEvalItems) {
+        const int k = count - first < kEvalItems ? count - first : kEvalItems;
+        EvalArgs a;
+        memset(&a, 0, sizeof(a));
+        for (int m = 0; m < k; m++) {
+            a.poly[m] = (const Fr *)polys[first + m];
+            a.out[m] = (Fr *)out[first + m];
+        }
+        k_peval_blocks<<<dim3(nb, (uint32_t)k), kT, 0, stream>>>(a, (uint32_t)n, nb, pw, tot);
+        ZK_CHECK(hipGetLastError());
+        k_peval_combine<<<(uint32_t)k, kT, 0, stream>>>(a, nb, pw, tot);
+        ZK_CHECK(hipGetLastError());
+    }
+    return hipSuccess;
 }
 
 hipError_t zk_shplonk_h_x_merge_on(void *res, const void *v, const void *values, const void *omegas,

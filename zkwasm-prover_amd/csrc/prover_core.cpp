@@ -952,12 +952,20 @@ hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, b
     // evaluations in the reference's order
     std::vector<int> eidx((size_t)npoly * ROT_COUNT, -1);
     const int e = (int)lay.evals.size();
-    for (int i = 0; i < e; i++) {
-        const int p = lay.evals[i].first, r = lay.evals[i].second;
-        eidx[(size_t)p * ROT_COUNT + r] = i;
-        PCHECK(poly_eval(lay.poly[p], ev_scratch, pts_dev[r], n, st));
-        PCHECK(hipMemcpyAsync((uint8_t *)j.evals + (size_t)i * FR, ev_scratch, FR, hipMemcpyDeviceToDevice, st));
+    // one batched evaluation per point, each result written to its slot
+    for (int r = 0; r < ROT_COUNT; r++) {
+        std::vector<const void *> polys;
+        std::vector<void *> outs;
+        for (int i = 0; i < e; i++) {
+            if (lay.evals[i].second != r) continue;
+            polys.push_back(lay.poly[lay.evals[i].first]);
+            outs.push_back((uint8_t *)j.evals + (size_t)i * FR);
+        }
+        if (polys.empty()) continue;
+        if (zk_poly_eval_batch_scratch_bytes(n, (int)polys.size()) > nb) return hipErrorInvalidValue;
+        PCHECK(zk_poly_eval_batch(polys.data(), outs.data(), (int)polys.size(), pts_dev[r], n, ev_scratch, st));
     }
+    for (int i = 0; i < e; i++) eidx[(size_t)lay.evals[i].first * ROT_COUNT + lay.evals[i].second] = i;
     nevals = e;
     if (tamper)  // self-test of the opening check: one written evaluation off by one
         PCHECK(zk_fr_op((uint8_t *)j.evals + (size_t)(e - 1) * FR, (uint8_t *)j.evals + (size_t)(e - 1) * FR,
