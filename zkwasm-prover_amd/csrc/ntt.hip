@@ -197,24 +197,42 @@ __device__ __forceinline__ void dft_rounds(Fr &r0, Fr &r1, Fr &r2, Fr &r3, const
 
 // The DIF of every group of 2^DEG in the wave's 256 elements (s: the wave's
 // LDS region, e-indexed; tw: w_D^j for j < D/2).  Results go back to s in
-// the layout the radix-2 LDS rounds leave (U[bitrev(i)] at i).
+// the layout the radix-2 LDS rounds leave (U[bitrev(i)] at i).  quarter:
+// (wave-uniform) every element with i >= D/4 is zero, as in the first pass
+// of a coset extension by 4 -- those are r1..r3 (rho != 0), so rounds 0 and 1
+// reduce to three products and no additions.
 template <int DEG>
-__device__ __forceinline__ void wave_dft_regs(Fr *s, const Fr *tw, uint32_t lane) {
+__device__ __forceinline__ void wave_dft_regs(Fr *s, const Fr *tw, uint32_t lane, bool quarter) {
     static_assert(DEG >= 2 && DEG <= 8, "register DFT covers 2 <= deg <= 8");
-    Fr r0 = s[slot_elem<DEG, false>(lane, 0)], r1 = s[slot_elem<DEG, false>(lane, 1)];
-    Fr r2 = s[slot_elem<DEG, false>(lane, 2)], r3 = s[slot_elem<DEG, false>(lane, 3)];
     const uint32_t lo = lane & ((1u << (DEG - 2)) - 1);
-    // round 0: i-bit DEG-1 in rho1; pairs (0,2) and (1,3)
-    butterfly<true>(r0, r2, tw[lo]);
-    butterfly<true>(r1, r3, tw[lo + (1u << (DEG - 2))]);
-    // round 1: i-bit DEG-2 in rho0; pairs (0,1) and (2,3)
-    if constexpr (DEG == 2) {
-        butterfly<false>(r0, r1, r0);
-        butterfly<false>(r2, r3, r0);
+    Fr r0 = s[slot_elem<DEG, false>(lane, 0)], r1, r2, r3;
+    if (quarter) {
+        // round 0: (r0, 0) -> (r0, r0 w); round 1: (x, 0) -> (x, x w')
+        r2 = r0 * tw[lo];
+        if constexpr (DEG == 2) {
+            r1 = r0;
+            r3 = r2;
+        } else {
+            const Fr w = tw[lo << 1];
+            r1 = r0 * w;
+            r3 = r2 * w;
+        }
     } else {
-        const Fr w = tw[lo << 1];
-        butterfly<true>(r0, r1, w);
-        butterfly<true>(r2, r3, w);
+        r1 = s[slot_elem<DEG, false>(lane, 1)];
+        r2 = s[slot_elem<DEG, false>(lane, 2)];
+        r3 = s[slot_elem<DEG, false>(lane, 3)];
+        // round 0: i-bit DEG-1 in rho1; pairs (0,2) and (1,3)
+        butterfly<true>(r0, r2, tw[lo]);
+        butterfly<true>(r1, r3, tw[lo + (1u << (DEG - 2))]);
+        // round 1: i-bit DEG-2 in rho0; pairs (0,1) and (2,3)
+        if constexpr (DEG == 2) {
+            butterfly<false>(r0, r1, r0);
+            butterfly<false>(r2, r3, r0);
+        } else {
+            const Fr w = tw[lo << 1];
+            butterfly<true>(r0, r1, w);
+            butterfly<true>(r2, r3, w);
+        }
     }
     dft_rounds<DEG, 2>(r0, r1, r2, r3, tw, lane);
     s[slot_elem<DEG, true>(lane, 0)] = r0;
@@ -286,14 +304,17 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
         // bs == kThreads: four waves, each owning 256 elements of the tile
         Fr *ws = s + ((tid >> 6) << 8);
         const uint32_t lane = tid & 63;
+        // first pass of a zero-padded input with nz <= n / 4: i >= D/4 reads
+        // positions index + i * t >= (D/4) * t = n / 4 >= nz, all zero
+        const bool quarter = nz && log_p == 0 && ((size_t)nz << 2) <= ((size_t)1 << log_n);
         switch (deg) {
-            case 2: wave_dft_regs<2>(ws, tw, lane); break;
-            case 3: wave_dft_regs<3>(ws, tw, lane); break;
-            case 4: wave_dft_regs<4>(ws, tw, lane); break;
-            case 5: wave_dft_regs<5>(ws, tw, lane); break;
-            case 6: wave_dft_regs<6>(ws, tw, lane); break;
-            case 7: wave_dft_regs<7>(ws, tw, lane); break;
-            default: wave_dft_regs<8>(ws, tw, lane); break;
+            case 2: wave_dft_regs<2>(ws, tw, lane, quarter); break;
+            case 3: wave_dft_regs<3>(ws, tw, lane, quarter); break;
+            case 4: wave_dft_regs<4>(ws, tw, lane, quarter); break;
+            case 5: wave_dft_regs<5>(ws, tw, lane, quarter); break;
+            case 6: wave_dft_regs<6>(ws, tw, lane, quarter); break;
+            case 7: wave_dft_regs<7>(ws, tw, lane, quarter); break;
+            default: wave_dft_regs<8>(ws, tw, lane, quarter); break;
         }
         __syncthreads();
     } else {
