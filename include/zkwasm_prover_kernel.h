@@ -603,12 +603,16 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
 
 /* ---- proof server: continuous admission by HBM footprint -------------- */
 /* Keys of every shape stay resident; `slots` workers (one stream each) start
- * queued requests FIFO as slots free up and the proof's device state fits
- * hbm_budget (0: free memory less 4 GiB), with backfill: a request that does
- * not fit lets smaller ones pass until it has been passed 8 times.  A proof
- * joins others in flight only when more requests wait than are running
- * (ZKS_EAGER: whenever a slot and memory allow); per-proof states are pooled
- * per shape.  flags: ZKP_BATCHED / ZKP_SHPLONK / ZKP_NO_OPEN for every proof;
+ * queued requests FIFO as slots free up.  The per-proof device states are
+ * created with the server (one per shape, then up to `slots` per shape,
+ * smallest shapes first, within hbm_budget; 0: free memory less 4 GiB) and
+ * reused, so serving allocates nothing; a request whose shape has no idle
+ * state lets others pass until it has been passed 8 times.  (If one state of
+ * every shape does not fit the budget, states are created on demand and idle
+ * ones of other shapes freed.)  A proof joins others in flight only when more
+ * than two requests wait per proof running (the environment variable
+ * ZKS_OVERLAP_RATIO, 1..64, sets that number; ZKS_EAGER: whenever a slot and
+ * a state are free).  flags: ZKP_BATCHED / ZKP_SHPLONK / ZKP_NO_OPEN for every proof;
  * ZKS_CROSS_BATCH (with ZKP_BATCHED) merges the commitment MSMs of proofs in
  * flight into shared zk_msm_batch passes.  zks_batch_stats: merged passes,
  * passes holding more than one proof's groups, MSM items.  No reference

@@ -2,7 +2,8 @@
 // slots with a budget for about two proof states, submitting while workers
 // run (eager admission); the same through the cross-proof batcher; a lone
 // request never gets company at light load; a large request that does not
-// fit is passed at most 8 times before it starts.  Exits non-zero on any
+// fit is passed at most 8 times before it starts; states pooled at creation
+// are reused with none created or freed while serving.  Exits non-zero on any
 // error; the commitments of each seed must match.
 #include <hip/hip_runtime.h>
 #include <atomic>
@@ -16,6 +17,7 @@
 extern std::mutex g_log_mu;
 extern std::vector<unsigned long long> g_start_log;
 extern std::atomic<int> g_hook_errors;
+extern std::atomic<int> g_job_creates, g_job_destroys;
 
 static int mix(int flags) {
     zkp_circuit a{10, 30, 2, 1, 3, 7, 6, 2, 7, 5}, b{11, 30, 2, 1, 3, 8, 6, 2, 8, 5};
@@ -96,11 +98,37 @@ static int no_starvation() {
     return pos <= 1 + 8 ? 0 : 23;
 }
 
+// states are created with the server (one per shape, then up to `slots` per
+// shape, smallest first, within the budget): a burst of mixed requests then
+// creates and frees none, and each shape runs at most its state count at once
+static int pooled_states() {
+    zkp_circuit a{10, 30, 2, 1, 3, 7, 6, 2, 7, 5}, b{12, 30, 2, 1, 3, 8, 6, 2, 8, 5};
+    zkp_circuit shapes[2] = {a, b};
+    const size_t ja = zkp_job_bytes(&a), jb = zkp_job_bytes(&b);
+    void *s;
+    const int c0 = g_job_creates.load();
+    if (zks_create(shapes, 2, 3, jb + 3 * ja, ZKP_BATCHED | ZKS_EAGER, 1, &s)) return 31;
+    const int c1 = g_job_creates.load(), d1 = g_job_destroys.load();
+    if (c1 - c0 != 4) return 32;  // b once, a three times
+    std::vector<int> ids;
+    for (int i = 0; i < 12; i++) ids.push_back(zks_submit(s, i % 3 == 0 ? 1 : 0, 700 + i));
+    for (int id : ids)
+        if (zks_wait(s, id, nullptr, nullptr)) return 33;
+    int pr; size_t pb, bud;
+    (void)zks_stats(s, &pr, &pb, &bud);
+    const int c2 = g_job_creates.load(), d2 = g_job_destroys.load();
+    (void)zks_destroy(s);
+    printf("pooled states: created %d while serving, freed %d, peak running %d, bytes %zu of %zu\n", c2 - c1,
+           d2 - d1, pr, pb, bud);
+    return (c2 == c1 && d2 == d1 && pr >= 2 && pb <= bud) ? 0 : 34;
+}
+
 int main() {
     int r = mix(ZKP_BATCHED | ZKS_EAGER);
     if (!r) r = mix(ZKP_BATCHED | ZKS_EAGER | ZKS_CROSS_BATCH);
     if (!r) r = alone_at_light_load();
     if (!r) r = no_starvation();
+    if (!r) r = pooled_states();
     if (r) printf("failed: %d\n", r);
     return r;
 }

@@ -71,8 +71,19 @@ int zkp_commitment_count(const zkp_circuit *c) { return 5 + c->k; }
 hipError_t zkp_pk_create(const zkp_circuit *, unsigned long long, hipStream_t, void **pk) { *pk = new int(1); return hipSuccess; }
 hipError_t zkp_pk_destroy(void *pk) { delete (int *)pk; return hipSuccess; }
 std::atomic<int> live{0};
-hipError_t zkp_job_create(void *, void **job) { *job = new MockJob(); live++; return hipSuccess; }
-hipError_t zkp_job_destroy(void *job) { delete (MockJob *)job; live--; return hipSuccess; }
+std::atomic<int> g_job_creates{0}, g_job_destroys{0};
+hipError_t zkp_job_create(void *, void **job) {
+    *job = new MockJob();
+    live++;
+    g_job_creates++;
+    return hipSuccess;
+}
+hipError_t zkp_job_destroy(void *job) {
+    delete (MockJob *)job;
+    live--;
+    g_job_destroys++;
+    return hipSuccess;
+}
 hipError_t zkp_prove(void *job, unsigned long long seed, int, void *com, void *, zkp_proof_info *info,
                      hipStream_t st) {
     {

@@ -7,11 +7,16 @@
 // and HBM free up: a request that does not fit yet does not block smaller
 // ones behind it (backfill), until it has been passed over kMaxBypass times;
 // then nothing starts ahead of it.  A second proof joins one in flight only
-// when the backlog is longer than the number of proofs in flight (with
-// ZKS_EAGER: whenever a slot and memory are free), so at light load a proof
-// runs alone, as on the one-at-a-time path, and under load proofs overlap.
-// Per-proof device states are kept in per-shape pools and reused; idle states
-// of other shapes are freed when a request needs the room.
+// when more than two requests wait per proof in flight (ZKS_OVERLAP_RATIO;
+// with ZKS_EAGER: whenever a slot and a state are free), so at light load a
+// proof runs alone, as on the one-at-a-time path, and under load proofs
+// overlap.
+// Per-proof device states are created when the server starts (one per shape,
+// then up to `slots` per shape, smallest shapes first, within the HBM budget)
+// and reused from per-shape pools: a request whose shape has no idle state
+// waits for one, so no state is created or freed while proofs run.  Only if
+// one state of every shape does not fit the budget are states created on
+// demand and idle states of other shapes freed to make room.
 //
 // With ZKS_CROSS_BATCH the commitment MSMs of the proofs in flight go
 // through one batcher: a group submitted while the batcher's stream is busy
@@ -26,6 +31,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -58,6 +64,7 @@ struct ShapeState {
     void *pk = nullptr;
     size_t job_bytes = 0;
     int ncommit = 0;
+    int states = 0;            // per-proof states of this shape, idle or in use
     std::vector<void *> idle;  // pooled per-proof states
 };
 
@@ -228,6 +235,7 @@ struct Server {
     size_t budget = 0, used = 0, peak_used = 0;
     int flags = ZKP_BATCHED;
     bool eager = false;
+    int overlap_ratio = 2;  // ZKS_OVERLAP_RATIO: waiting requests per running proof before another joins
     int running = 0, peak_running = 0;
     bool stop = false;
     std::mutex mu;
@@ -242,8 +250,8 @@ struct Server {
     bool pick(int &rid, void *&job, bool &fresh, std::vector<void *> &to_free) {
         if (queue.empty()) return false;
         // overlap only on backlog: a proof joins those in flight when more
-        // requests wait than are running
-        if (!eager && running > 0 && (int)queue.size() <= running) return false;
+        // than overlap_ratio requests wait per proof running
+        if (!eager && running > 0 && (int)queue.size() <= running * overlap_ratio) return false;
         const bool head_only = reqs[queue.front()].bypassed >= kMaxBypass;
         for (auto it = queue.begin(); it != queue.end(); ++it) {
             if (head_only && it != queue.begin()) break;
@@ -253,6 +261,10 @@ struct Server {
                 sh.idle.pop_back();
                 fresh = false;
             } else {
+                // a shape that already has a state waits for it rather than
+                // evicting the pooled states of other shapes (creating a
+                // state of tens of GB next to running proofs takes seconds)
+                if (sh.states > 0 && used + sh.job_bytes > budget) continue;
                 if (used + sh.job_bytes > budget) {
                     // free idle states of other shapes until it fits
                     size_t idle_other = 0;
@@ -263,12 +275,14 @@ struct Server {
                         while (&o != &sh && !o.idle.empty() && used + sh.job_bytes > budget) {
                             to_free.push_back(o.idle.back());
                             o.idle.pop_back();
+                            o.states--;
                             used -= o.job_bytes;
                         }
                     }
                 }
                 used += sh.job_bytes;
                 if (used > peak_used) peak_used = used;
+                sh.states++;
                 job = nullptr;
                 fresh = true;
             }
@@ -278,6 +292,36 @@ struct Server {
             return true;
         }
         return false;
+    }
+
+    // The per-proof states, created before any request runs: one of every
+    // shape (largest first), then more, smallest shapes first, up to `slots`
+    // of a shape while the budget holds them.  Serving then allocates nothing
+    // while the budget is full; if one state of every shape does not fit,
+    // states are created and evicted on demand instead.
+    hipError_t prefill(int slots) {
+        std::vector<int> order(shapes.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+        std::sort(order.begin(), order.end(),
+                  [&](int a, int b) { return shapes[a].job_bytes > shapes[b].job_bytes; });
+        size_t one_each = 0;
+        for (auto &sh : shapes) one_each += sh.job_bytes;
+        if (one_each > budget) return hipSuccess;
+        auto add = [&](ShapeState &sh) -> hipError_t {
+            void *job = nullptr;
+            SCHECK(zkp_job_create(sh.pk, &job));
+            sh.idle.push_back(job);  // capacity slots + 1 reserved
+            sh.states++;
+            used += sh.job_bytes;
+            return hipSuccess;
+        };
+        for (int i : order) SCHECK(add(shapes[i]));
+        for (auto it = order.rbegin(); it != order.rend(); ++it) {
+            ShapeState &sh = shapes[*it];
+            while (sh.states < slots && used + sh.job_bytes <= budget) SCHECK(add(sh));
+        }
+        if (used > peak_used) peak_used = used;
+        return hipSuccess;
     }
 
     // under mu: a finished or failed proof gives back its slot and its job
@@ -291,6 +335,7 @@ struct Server {
         } else {
             if (job) (void)zkp_job_destroy(job);
             used -= sh.job_bytes;
+            sh.states--;
         }
         cv_done.notify_all();
         cv_work.notify_all();
@@ -356,6 +401,10 @@ hipError_t zks_create(const zkp_circuit *shapes, int nshapes, int slots, size_t 
         std::unique_ptr<Server> s(new Server());
         s->flags = flags & (ZKP_BATCHED | ZKP_SHPLONK | ZKP_NO_OPEN);
         s->eager = (flags & ZKS_EAGER) != 0;
+        if (const char *r = getenv("ZKS_OVERLAP_RATIO")) {
+            const int v = atoi(r);
+            if (v >= 1 && v <= 64) s->overlap_ratio = v;
+        }
         hipError_t e = hipGetDevice(&s->device);
         s->shapes.resize(nshapes);
         size_t max_group = 0;
@@ -385,9 +434,12 @@ hipError_t zks_create(const zkp_circuit *shapes, int nshapes, int slots, size_t 
             for (auto &sh : s->shapes)
                 if (sh.job_bytes > s->budget) e = hipErrorOutOfMemory;  // a shape that could never run
         }
+        if (e == hipSuccess) e = s->prefill(slots);
         if (e != hipSuccess) {
-            for (auto &sh : s->shapes)
+            for (auto &sh : s->shapes) {
+                for (void *j : sh.idle) (void)zkp_job_destroy(j);
                 if (sh.pk) (void)zkp_pk_destroy(sh.pk);
+            }
             if (s->batcher) s->batcher->release();
             return e;
         }
