@@ -79,21 +79,20 @@ enum MsmArr {
 constexpr int kFirstPhase2 = A_BUCKETS;
 
 constexpr uint32_t kSortTile = 4096;  // scalars per histogram / scatter tile
-constexpr uint32_t kDigitBlock = 256;  // scalars per digit-kernel block
 constexpr uint32_t kPackedMaxN = 1u << 23;  // items up to this size sort with one-word entries
 
 // One MSM of a batch as the kernels see it (built on the device).
 struct MsmItemDev {
     const G1Aff *pts;
     const Fr *scalars;
-    uint32_t n, blk0, tile0, ntiles;  // blk0 / tile0: first digit block / sort tile of the item
+    uint32_t n, tile0, ntiles;  // tile0: the item's first sort tile
     uint64_t dig0;                    // offset of the item's digits (W * scalars before it)
 };
 
 struct MsmPlan {
     uint32_t count, c, W, B, T1, nbits;
     uint32_t hi_bits, lo_bits, nbins;
-    uint64_t ntot, nblk, ntiles, nhist, NB, max_entries, chunks;
+    uint64_t ntot, ntiles, nhist, NB, max_entries, chunks;
     bool packed;  // every item has at most kPackedMaxN scalars: one-word sort entries
     size_t size[A_COUNT];
     uint8_t *ptr[A_COUNT];
@@ -117,7 +116,6 @@ MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c) {
     for (uint32_t m = 0; m < count; m++) {
         if (ns[m] > kPackedMaxN) p.packed = false;
         p.ntot += ns[m];
-        p.nblk += (ns[m] + kDigitBlock - 1) / kDigitBlock;
         p.ntiles += (ns[m] + kSortTile - 1) / kSortTile;
     }
     p.NB = (uint64_t)count * p.W * p.B;
@@ -239,13 +237,12 @@ __device__ __forceinline__ int32_t signed_digit(const uint32_t s[8], uint32_t w,
     return (int32_t)v;
 }
 
-// item of a global block / tile index: the last item whose first index is <= x
-__device__ __forceinline__ uint32_t item_of(const MsmItemDev *__restrict__ items, uint32_t count, uint32_t x,
-                                            bool by_tile) {
+// item of a global sort-tile index: the last item whose first tile is <= x
+__device__ __forceinline__ uint32_t item_of(const MsmItemDev *__restrict__ items, uint32_t count, uint32_t x) {
     uint32_t lo = 0, hi = count;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
-        if ((by_tile ? items[mid].tile0 : items[mid].blk0) <= x) lo = mid; else hi = mid;
+        if (items[mid].tile0 <= x) lo = mid; else hi = mid;
     }
     return lo;
 }
@@ -267,23 +264,6 @@ __global__ void k_msm_put_items(ItemPack pk, MsmItemDev *__restrict__ items, uin
     slots[pk.first + j] = pk.slot[j];
 }
 
-// digits[dig0 + w * n + i] = 0 for a zero digit, else |d| | (d < 0) << 31 with
-// |d| in [1, 2^(c-1)] (bucket |d| - 1); one block = 256 scalars of one item
-__global__ __launch_bounds__(256) void k_msm_digits(const MsmItemDev *__restrict__ items, uint32_t count, uint32_t c,
-                                                    uint32_t W, uint32_t *__restrict__ digits) {
-    const uint32_t m = item_of(items, count, blockIdx.x, false);
-    const MsmItemDev it = items[m];
-    const uint32_t i = (blockIdx.x - it.blk0) * kDigitBlock + threadIdx.x;
-    if (i >= it.n) return;
-    Fr s = fe_from_mont<FrParams>(fe_load<FrParams>(&it.scalars[i]));
-    uint32_t *dst = digits + it.dig0;
-    uint32_t carry = 0;
-    for (uint32_t w = 0; w < W; w++) {
-        int32_t d = signed_digit(s.v, w, c, carry);
-        dst[(size_t)w * it.n + i] = d ? ((uint32_t)(d < 0 ? -d : d) | (d < 0 ? 0x80000000u : 0u)) : 0u;
-    }
-}
-
 // hist index of (item m, window w, bin, tile-in-item): groups (m, w, bin) are
 // contiguous and ordered, so their exclusive scan gives every group's start
 __device__ __forceinline__ uint64_t hist_index(const MsmItemDev &it, uint32_t W, uint32_t nbins, uint32_t w,
@@ -291,41 +271,58 @@ __device__ __forceinline__ uint64_t hist_index(const MsmItemDev &it, uint32_t W,
     return (uint64_t)W * nbins * it.tile0 + ((uint64_t)w * nbins + bin) * it.ntiles + tl;
 }
 
-// per (tile, window): histogram of the high bucket bits
-__global__ __launch_bounds__(256) void k_msm_hist(const MsmItemDev *__restrict__ items, uint32_t count, uint32_t W,
-                                                  const uint32_t *__restrict__ digits, uint32_t lo_bits,
-                                                  uint32_t nbins, uint32_t *__restrict__ hist) {
-    __shared__ uint32_t h[256];
-    const uint32_t w = blockIdx.y;
-    const uint32_t m = item_of(items, count, blockIdx.x, true);
+// Digits and their histogram in one pass, one block per sort tile (4096
+// scalars of one item, 4 per thread, coalesced).  digits[dig0 + w * n + i] =
+// 0 for a zero digit, else |d| | (d < 0) << 31 with |d| in [1, 2^(c-1)]
+// (bucket |d| - 1); per window the counts of the high bucket bits are kept in
+// LDS (W * nbins <= 3840 counters for every window 2 <= c <= 17) and written
+// once per tile, so the digits are not read back for the histogram.
+constexpr uint32_t kHistMax = 3840;
+constexpr uint32_t kDigitThreads = 1024;
+__global__ __launch_bounds__(kDigitThreads) void k_msm_digits_hist(const MsmItemDev *__restrict__ items, uint32_t count,
+                                                         uint32_t c, uint32_t W, uint32_t lo_bits, uint32_t nbins,
+                                                         uint32_t *__restrict__ digits, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[kHistMax];
+    const uint32_t m = item_of(items, count, blockIdx.x);
     const MsmItemDev it = items[m];
     const uint32_t tl = blockIdx.x - it.tile0;
-    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
+    const uint32_t nh = W * nbins;
+    for (uint32_t k = threadIdx.x; k < nh; k += blockDim.x) h[k] = 0;
     __syncthreads();
-    const uint32_t *dw = digits + it.dig0 + (size_t)w * it.n;
-    uint32_t end = (tl + 1) * kSortTile < it.n ? (tl + 1) * kSortTile : it.n;
+    uint32_t *dst = digits + it.dig0;
+    const uint32_t end = (tl + 1) * kSortTile < it.n ? (tl + 1) * kSortTile : it.n;
     for (uint32_t i = tl * kSortTile + threadIdx.x; i < end; i += blockDim.x) {
-        uint32_t d = dw[i];
-        if (d) atomicAdd(&h[((d & 0x7fffffffu) - 1) >> lo_bits], 1u);
+        const Fr s = fe_from_mont<FrParams>(fe_load<FrParams>(&it.scalars[i]));
+        uint32_t carry = 0;
+        for (uint32_t w = 0; w < W; w++) {
+            const int32_t d = signed_digit(s.v, w, c, carry);
+            const uint32_t a = (uint32_t)(d < 0 ? -d : d);
+            dst[(size_t)w * it.n + i] = d ? (a | (d < 0 ? 0x80000000u : 0u)) : 0u;
+            if (d) atomicAdd(&h[w * nbins + ((a - 1) >> lo_bits)], 1u);
+        }
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < nbins; k += blockDim.x) hist[hist_index(it, W, nbins, w, k, tl)] = h[k];
+    for (uint32_t k = threadIdx.x; k < nh; k += blockDim.x)
+        hist[hist_index(it, W, nbins, k / nbins, k % nbins, tl)] = h[k];
 }
 
-// exclusive scan of x[0..256) in LDS by 256 threads (x[k] = 0 past the used
-// range); t is a 256-entry LDS scratch
+// exclusive scan of x[0..256) in LDS by 256 threads (4 waves): a
+// shuffle scan inside each wave, the 4 wave totals through t[0..4), two
+// barriers (x[k] = 0 past the used range)
 __device__ __forceinline__ void lds_exclusive_scan256(uint32_t *x, uint32_t *t) {
-    const uint32_t i = threadIdx.x;
-    uint32_t v = x[i];
-    t[i] = v;
-    __syncthreads();
-    for (uint32_t d = 1; d < 256; d <<= 1) {
-        uint32_t add = i >= d ? t[i - d] : 0u;
-        __syncthreads();
-        t[i] += add;
-        __syncthreads();
+    const uint32_t i = threadIdx.x, lane = i & 63, wv = i >> 6;
+    const uint32_t v = x[i];
+    uint32_t s = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)s, d, 64);
+        if (lane >= d) s += o;
     }
-    x[i] = t[i] - v;
+    if (lane == 63) t[wv] = s;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < wv; k++) off += t[k];
+    x[i] = s - v + off;
     __syncthreads();
 }
 
@@ -365,7 +362,7 @@ __global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__rest
     __shared__ typename E::T stage[kSortTile];
     __shared__ uint8_t sbin[kSortTile];
     const uint32_t w = blockIdx.y;
-    const uint32_t m = item_of(items, count, blockIdx.x, true);
+    const uint32_t m = item_of(items, count, blockIdx.x);
     const MsmItemDev it = items[m];
     const uint32_t tl = blockIdx.x - it.tile0;
     const uint32_t tid = threadIdx.x;
@@ -1041,7 +1038,7 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     ZK_CHECK(prof_mark(prof, PH_COUNT, stream));
     // item table with prefixes, written from kernel arguments
     {
-        uint32_t blk = 0, tile = 0;
+        uint32_t tile = 0;
         uint64_t dig = 0;
         for (uint32_t first = 0; first < count; first += kItemsPerLaunch) {
             ItemPack pk;
@@ -1054,12 +1051,10 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
                 it.pts = jb.pts;
                 it.scalars = jb.scalars;
                 it.n = jb.n;
-                it.blk0 = blk;
                 it.tile0 = tile;
                 it.ntiles = (jb.n + kSortTile - 1) / kSortTile;
                 it.dig0 = dig;
                 pk.slot[j] = jb.slice;
-                blk += (jb.n + kDigitBlock - 1) / kDigitBlock;
                 tile += it.ntiles;
                 dig += (uint64_t)jb.n * p.W;
             }
@@ -1069,10 +1064,9 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     }
     k_slice_clear<<<(count + 63) / 64, 64, 0, stream>>>(slots, count);
     ZK_CHECK(hipGetLastError());
-    k_msm_digits<<<(uint32_t)p.nblk, kDigitBlock, 0, stream>>>(items, count, c, p.W, digits);
-    ZK_CHECK(hipGetLastError());
-    k_msm_hist<<<dim3((uint32_t)p.ntiles, p.W), 256, 0, stream>>>(items, count, p.W, digits, p.lo_bits, p.nbins,
-                                                                  hist);
+    if ((uint64_t)p.W * p.nbins > kHistMax) return hipErrorInvalidValue;  // not for 2 <= c <= 17
+    k_msm_digits_hist<<<(uint32_t)p.ntiles, kDigitThreads, 0, stream>>>(items, count, c, p.W, p.lo_bits, p.nbins,
+                                                                         digits, hist);
     ZK_CHECK(hipGetLastError());
     ZK_CHECK(prof_mark(prof, PH_SCAN, stream));
     ZK_CHECK(launch_scan(hist, (uint32_t)p.nhist, starts, nullptr, scan_sums, stream));
