@@ -192,10 +192,14 @@ hipError_t zk_copyq_push(void *q, void *host_dst, const void *dev_src, size_t by
         // host_dst is in flight, nothing below can throw
         cq.q.emplace_back();
         Entry &e = cq.q.back();
+        // once a copy is enqueued on cq.cs, a later failure waits for the
+        // stream before the stage and the caller's buffer are given back
+        bool enqueued = false;
     #define QCHECK(x)                                                                    \
         do {                                                                             \
             err = (x);                                                                   \
             if (err != hipSuccess) {                                                     \
+                if (enqueued) (void)hipStreamSynchronize(cq.cs);                         \
                 if (e.stage >= 0) cq.stages[e.stage].busy = false;                       \
                 for (hipEvent_t ev : {e.ready, e.snap, e.done})                          \
                     if (ev) (void)hipEventDestroy(ev);                                   \
@@ -213,11 +217,13 @@ hipError_t zk_copyq_push(void *q, void *host_dst, const void *dev_src, size_t by
             if (e.stage < 0) QCHECK(err);
             QCHECK(hipEventCreateWithFlags(&e.snap, hipEventDisableTiming));
             QCHECK(hipMemcpyAsync(cq.stages[e.stage].p, dev_src, bytes, hipMemcpyDeviceToDevice, cq.cs));
+            enqueued = true;
             QCHECK(hipEventRecord(e.snap, cq.cs));
             QCHECK(hipStreamWaitEvent(producer, e.snap, 0));
             src = cq.stages[e.stage].p;
         }
         QCHECK(hipMemcpyAsync(host_dst, src, bytes, hipMemcpyDeviceToHost, cq.cs));
+        enqueued = true;
         QCHECK(hipEventRecord(e.done, cq.cs));
     #undef QCHECK
         if (flags & ZK_COPYQ_RELEASE) e.release = (void *)dev_src;
