@@ -1206,6 +1206,45 @@ hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stre
     return hipStreamSynchronize(stream);  // the host buffers must outlive the copies
 }
 
+// host_collect for slices at base + m * stride (stride >= kSliceReserve, the
+// layout zk_msm_batch uses): one device-to-host copy of the whole region, the
+// tails on the host, one host-to-device copy back, instead of three small
+// copies per slice.  jac_out (optional, host): the count results, 96 B each.
+hipError_t host_collect_strided(uint8_t *base, size_t stride, uint32_t batch, hipStream_t stream,
+                                uint8_t *jac_out) {
+    if (stride < kSliceReserve) return hipErrorInvalidValue;
+    const size_t bytes = (size_t)(batch - 1) * stride + kSliceReserve;
+    std::vector<uint8_t> img(bytes);
+    ZK_CHECK(hipMemcpyAsync(img.data(), base, bytes, hipMemcpyDeviceToHost, stream));
+    ZK_CHECK(hipStreamSynchronize(stream));
+    // a slice msm() never finished (cleared or stale header) has no result
+    for (uint32_t m = 0; m < batch; m++)
+        if (!header_ok(reinterpret_cast<const SliceHeader *>(&img[(size_t)m * stride + 64]))) return hipErrorInvalidValue;
+    std::vector<uint8_t> out((size_t)batch * 96);
+    unsigned hw = std::thread::hardware_concurrency();
+    unsigned nt = batch < 2 ? 1 : (hw ? (hw < batch ? hw : batch) : 1);
+    if (nt > 16) nt = 16;
+    auto run = [&](unsigned t) {
+        for (uint32_t m = t; m < batch; m += nt) host_tail_one(&img[(size_t)m * stride], &out[(size_t)m * 96]);
+    };
+    if (nt <= 1) {
+        run(0);
+    } else {
+        std::vector<std::thread> th;
+        unsigned started = 0;
+        try {
+            for (unsigned t = 0; t < nt; t++, started++) th.emplace_back(run, t);
+        } catch (...) {
+        }
+        for (auto &x : th) x.join();
+        for (unsigned t = started; t < nt; t++) run(t);  // shares whose thread could not be started
+    }
+    for (uint32_t m = 0; m < batch; m++) memcpy(&img[(size_t)m * stride], &out[(size_t)m * 96], 96);
+    ZK_CHECK(hipMemcpyAsync(base, img.data(), bytes, hipMemcpyHostToDevice, stream));
+    if (jac_out) memcpy(jac_out, out.data(), out.size());
+    return hipStreamSynchronize(stream);  // the host image must outlive the copy
+}
+
 bool device_tail_requested() {
     const char *e = getenv("ZK_MSM_TAIL");
     return e && strcmp(e, "device") == 0;
@@ -1230,6 +1269,37 @@ hipError_t msm_one_async(void *out, const void *points, const void *scalars, uin
     uint8_t **slots = nullptr;
     ZK_CHECK(msm_run(&job, 1, ZK_MSM_OUT_BYTES, c, regions, bytes, 1, stream, &slots));
     return device_collect(slots, 1, stream);
+}
+
+// zk_msm_batch whose results also land in host memory (jac_host: count x 96
+// B) without a copy per item (the prover core's commitments)
+hipError_t msm_batch_host(void *out, const zk_msm_item *items, int count, void *scratch, size_t scratch_bytes,
+                          hipStream_t stream, uint8_t *jac_host) {
+    if (count <= 0 || !items || !out || !scratch || !jac_host) return hipErrorInvalidValue;
+    std::vector<MsmJob> jobs(count);
+    std::vector<uint8_t *> slices(count);
+    for (int m = 0; m < count; m++) {
+        if (items[m].n <= 0) return hipErrorInvalidValue;
+        slices[m] = (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES;
+        jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n, slices[m]};
+    }
+    uint8_t *regions[1] = {(uint8_t *)scratch};
+    size_t bytes[1] = {scratch_bytes};
+    uint8_t **slots = nullptr;
+    ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, 0, regions, bytes, 1, stream, &slots));
+    if (!device_tail_requested())
+        return host_collect_strided((uint8_t *)out, ZK_MSM_OUT_BYTES, (uint32_t)count, stream, jac_host);
+    ZK_CHECK(device_collect(slots, (uint32_t)count, stream));
+    for (int m = 0; m < count; m++)
+        ZK_CHECK(hipMemcpyAsync(jac_host + (size_t)m * 96, slices[m], 96, hipMemcpyDeviceToHost, stream));
+    ZK_CHECK(hipStreamSynchronize(stream));
+    for (int m = 0; m < count; m++) {  // the device tail's no-result marker: 96 bytes of 0xff
+        const uint8_t *q = jac_host + (size_t)m * 96;
+        bool all = true;
+        for (int b = 0; b < 96; b++) all &= q[b] == 0xff;
+        if (all) return hipErrorInvalidValue;
+    }
+    return hipSuccess;
 }
 
 hipError_t msm_collect(uint8_t *const *slices_dev, uint32_t batch, hipStream_t stream) {
@@ -1392,7 +1462,8 @@ hipError_t zk_msm_batch(void *out, const zk_msm_item *items, int count, int c, v
         uint8_t **slots = nullptr;
         ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1,
                          stream, &slots));
-        return finish_batch(slices.data(), slots, (uint32_t)count, stream);
+        if (device_tail_requested()) return finish_batch(slices.data(), slots, (uint32_t)count, stream);
+        return host_collect_strided((uint8_t *)out, ZK_MSM_OUT_BYTES, (uint32_t)count, stream, nullptr);
     });
 }
 

@@ -613,20 +613,20 @@ hipError_t commit(Job &j, const void *bases, const std::vector<const void *> &po
             PCHECK(j.commit_fn(j.commit_ctx, it.data(), cnt, out + first * 64, st));
             continue;
         }
+        std::vector<uint8_t> jac((size_t)cnt * 96);
         if (batched) {
             std::vector<zk_msm_item> it(cnt);
             for (int m = 0; m < cnt; m++) it[m] = zk_msm_item{bases, polys[first + m], (int)s.n, 0};
-            PCHECK(zk_msm_batch(j.msm_out, it.data(), cnt, 0, j.msm_scratch, j.msm_bytes, st));
+            PCHECK(zk::msm_batch_host(j.msm_out, it.data(), cnt, j.msm_scratch, j.msm_bytes, st, jac.data()));
         } else {
             for (int m = 0; m < cnt; m++)
                 PCHECK(zk_msm((uint8_t *)j.msm_out + (size_t)m * ZK_MSM_OUT_BYTES, bases, polys[first + m],
                               (int)s.n, 0, j.msm_scratch, j.msm_bytes, st));
+            for (int m = 0; m < cnt; m++)
+                PCHECK(hipMemcpyAsync(&jac[(size_t)m * 96], (uint8_t *)j.msm_out + (size_t)m * ZK_MSM_OUT_BYTES, 96,
+                                      hipMemcpyDeviceToHost, st));
+            PCHECK(hipStreamSynchronize(st));
         }
-        std::vector<uint8_t> jac((size_t)cnt * 96);
-        for (int m = 0; m < cnt; m++)
-            PCHECK(hipMemcpyAsync(&jac[(size_t)m * 96], (uint8_t *)j.msm_out + (size_t)m * ZK_MSM_OUT_BYTES, 96,
-                                  hipMemcpyDeviceToHost, st));
-        PCHECK(hipStreamSynchronize(st));
         for (int m = 0; m < cnt; m++) {
             const uint8_t *q = &jac[(size_t)m * 96];
             bool id = true;

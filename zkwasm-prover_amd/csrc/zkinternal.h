@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "zkwasm_prover_kernel.h"
+
 namespace zk {
 
 // zero-fill by kernel (no memset node in a captured graph) (ntt.hip)
@@ -42,6 +44,11 @@ hipError_t ntt_batch_from(const void *const *src, void *const *buf, void *const 
 // without host waits or host allocation (msm.hip)
 hipError_t msm_one_async(void *out, const void *points, const void *scalars, uint32_t n, uint32_t c, void *scratch,
                          size_t scratch_bytes, hipStream_t stream);
+
+// zk_msm_batch at the default window whose count results (Jacobian, 96 B
+// each) are also returned in host memory, without a copy per item (msm.hip)
+hipError_t msm_batch_host(void *out, const zk_msm_item *items, int count, void *scratch, size_t scratch_bytes,
+                          hipStream_t stream, uint8_t *jac_host);
 
 typedef hipError_t (*CommitFn)(void *ctx, const void *items, int count, uint8_t *out_affine, hipStream_t stream);
 hipError_t job_set_commit_fn(void *job, CommitFn fn, void *ctx);
