@@ -348,12 +348,51 @@ __global__ __launch_bounds__(kT) void k_peval_combine(const EvalArgs a, uint32_t
     if (threadIdx.x == 0) fe_store<FrParams>(a.out[m], acc);
 }
 
+// out[j] = out_in[j] v^k + sum_i v^(k-1-i) p_i[j] over the launch's k
+// polynomials (Horner per element; out_in = 0 on the first launch): one read
+// of every polynomial and one write of out, instead of a read-modify-write
+// pass over out per polynomial
+__global__ __launch_bounds__(kT) void k_horner_combine(const EvalArgs a, uint32_t k, bool first,
+                                                       const Fr *__restrict__ vp, Fr *__restrict__ out, uint32_t n) {
+    const Fr v = fe_load<FrParams>(vp);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        Fr acc = first ? fe_load<FrParams>(&a.poly[0][j]) : fe_load<FrParams>(&out[j]) * v + fe_load<FrParams>(&a.poly[0][j]);
+        for (uint32_t i = 1; i < k; i++) acc = acc * v + fe_load<FrParams>(&a.poly[i][j]);
+        fe_store<FrParams>(&out[j], acc);
+    }
+}
+
 uint32_t grid_for(uint32_t n) {
     uint32_t b = ceil_div(n, kT);
     return b < 16384 ? (b ? b : 1) : 16384;
 }
 
 }  // namespace
+
+namespace zk {
+
+// out = sum_i v^(count-1-i) polys[i] (n elements each, device pointers in a
+// host array; out may alias none of them), Horner per element
+hipError_t poly_horner_combine(void *out, const void *const *polys, int count, const void *v, int n,
+                               hipStream_t stream) {
+    if (!out || !polys || count < 1 || !v || n < 1) return hipErrorInvalidValue;
+    for (int first = 0; first < count; first += kEvalItems) {
+        const int k = count - first < kEvalItems ? count - first : kEvalItems;
+        EvalArgs a;
+        memset(&a, 0, sizeof(a));
+        for (int m = 0; m < k; m++) {
+            if (!polys[first + m] || polys[first + m] == out) return hipErrorInvalidValue;
+            a.poly[m] = (const Fr *)polys[first + m];
+        }
+        k_horner_combine<<<grid_for((uint32_t)n), kT, 0, stream>>>(a, (uint32_t)k, first == 0, (const Fr *)v,
+                                                                   (Fr *)out, (uint32_t)n);
+        ZK_CHECK(hipGetLastError());
+    }
+    return hipSuccess;
+}
+
+}  // namespace zk
 
 extern "C" {
 

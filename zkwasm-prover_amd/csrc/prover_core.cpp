@@ -1008,13 +1008,12 @@ hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, b
             if (grp.empty()) continue;
             void *B = slot[W.size()];
             Fr acc = fr_zero();
+            std::vector<const void *> gp;
             for (size_t i = 0; i < grp.size(); i++) {
-                if (i == 0)
-                    PCHECK(hipMemcpyAsync(B, lay.poly[grp[i]], nb, hipMemcpyDeviceToDevice, st));
-                else
-                    PCHECK(field_op(B, B, 0, j.cf(J_V), lay.poly[grp[i]], 0, nullptr, n, FO_ADD, st));
+                gp.push_back(lay.poly[grp[i]]);
                 acc = fr_add(fr_mul(acc, v), eval_of(grp[i], r));
             }
+            PCHECK(zk::poly_horner_combine(B, gp.data(), (int)gp.size(), j.cf(J_V), n, st));
             PCHECK(zk_kate_divide(B, B, pts_dev[r], ocs((int)W.size()), n, kate_tmp, st));
             expect.push_back(acc);
             W.push_back(B);
@@ -1075,19 +1074,16 @@ hipError_t open_phase(Job &j, Transcript &tr, const void *hpoly, bool shplonk, b
         }
         for (size_t d = 0; d < diffs[si].size(); d++) stage[12 + 3 * si + d] = P[diffs[si][d]];
         V[si] = slot[si];
-        bool first = true;
         const int npts = (int)set_rots[si].size();
         std::vector<Fr> ec(npts, fr_zero()), pts(npts);
         for (int q = 0; q < npts; q++) pts[q] = P[set_rots[si][q]];
+        std::vector<const void *> sp;
         for (int p : order) {
             if (mask[p] != keys[si].second) continue;
-            if (first)
-                PCHECK(hipMemcpyAsync(V[si], lay.poly[p], nb, hipMemcpyDeviceToDevice, st));
-            else
-                PCHECK(field_op(V[si], V[si], 0, j.cf(J_SY), lay.poly[p], 0, nullptr, n, FO_ADD, st));
-            first = false;
+            sp.push_back(lay.poly[p]);
             for (int q = 0; q < npts; q++) ec[q] = fr_add(fr_mul(ec[q], y), eval_of(p, set_rots[si][q]));
         }
+        if (!sp.empty()) PCHECK(zk::poly_horner_combine(V[si], sp.data(), (int)sp.size(), j.cf(J_SY), n, st));
         r[si] = lagrange(pts.data(), ec.data(), npts);
         for (int c = 0; c < npts; c++) stage[24 + 3 * si + c] = r[si][c];
     }

@@ -50,6 +50,11 @@ hipError_t msm_one_async(void *out, const void *points, const void *scalars, uin
 hipError_t msm_batch_host(void *out, const zk_msm_item *items, int count, void *scratch, size_t scratch_bytes,
                           hipStream_t stream, uint8_t *jac_host);
 
+// out = sum_i v^(count-1-i) polys[i] (Horner per element; polys: host array
+// of device pointers, none equal to out) (multiopen.hip)
+hipError_t poly_horner_combine(void *out, const void *const *polys, int count, const void *v, int n,
+                               hipStream_t stream);
+
 typedef hipError_t (*CommitFn)(void *ctx, const void *items, int count, uint8_t *out_affine, hipStream_t stream);
 hipError_t job_set_commit_fn(void *job, CommitFn fn, void *ctx);
 
