@@ -642,6 +642,13 @@ hipError_t zks_destroy(void *srv);
  * stay with the pool until the process ends; reserve pre-grows it
  * (reserve_pinned_buffer, hugetlb.rs:149). */
 hipError_t zk_pinned_reserve(size_t slabs);
+/* prepare_advice_buffer (src/lib.rs:107-112, src/buffer.rs:19-34): `count`
+ * columns of `rows` Fr each (the circuit's advice columns, 2^k rows), zeroed,
+ * from the pinned pool (so rows * 32 <= 2^28); cols (host array) receives the
+ * pointers, which zkp_set_host_witness takes as the advice.  On failure no
+ * column stays allocated.  zk_release_advice_buffer gives them back. */
+hipError_t zk_prepare_advice_buffer(int rows, int count, void **cols);
+hipError_t zk_release_advice_buffer(int count, void **cols);
 hipError_t zk_pinned_alloc(size_t bytes, void **out);
 hipError_t zk_pinned_free(void *p);
 hipError_t zk_pinned_stats(size_t *slabs, size_t *chunks_in_use, size_t *slab_bytes, size_t *chunk_bytes);

@@ -257,10 +257,15 @@ def test_prover_core_host_witness_same_proof(core):
     pk, pr = core
     n = 1 << pk.circuit.k
     c1, p1, i1 = pr.prove(23, flags=BATCHED)
-    bufs = [hostmem.PinnedBuffer(n * 32) for _ in range(pk.circuit.advice + pk.circuit.instance)]
+    # the advice columns as prepare_advice_buffer hands them out: zeroed,
+    # pinned; the instance column from the pool directly
+    advb = hostmem.AdviceBuffer(n, pk.circuit.advice)
+    assert all((advb.column(c) == 0).all() for c in range(pk.circuit.advice))
+    assert len(set(advb.ptrs)) == pk.circuit.advice
+    bufs = [hostmem.PinnedBuffer(n * 32) for _ in range(pk.circuit.instance)]
     try:
-        adv = [b.ptr for b in bufs[:pk.circuit.advice]]
-        ins = [b.ptr for b in bufs[pk.circuit.advice:]]
+        adv = list(advb.ptrs)
+        ins = [b.ptr for b in bufs]
         pr.witness_to_host(adv, ins)
         pr.set_host_witness(adv, ins)
         c2, p2, i2 = pr.prove(23, flags=BATCHED | HOST_WITNESS)
@@ -268,14 +273,14 @@ def test_prover_core_host_witness_same_proof(core):
         assert (c1 == c2).all() and (p1 == p2).all()
         assert bytes(i1.transcript) == bytes(i2.transcript)
         # pageable host memory works too (synchronous uploads)
-        host = [np.zeros((n, 4), dtype=np.uint64) for _ in bufs]
-        for h, b in zip(host, bufs):
-            h[:] = b.array(np.uint64).reshape(n, 4)
+        host = [advb.column(c).copy() for c in range(pk.circuit.advice)]
+        host += [b.array(np.uint64).reshape(n, 4).copy() for b in bufs]
         pr.set_host_witness([h.ctypes.data for h in host[:pk.circuit.advice]],
                             [h.ctypes.data for h in host[pk.circuit.advice:]])
         c3, _, i3 = pr.prove(23, flags=BATCHED | HOST_WITNESS)
         assert (c1 == c3).all() and bytes(i1.transcript) == bytes(i3.transcript)
     finally:
+        advb.close()
         for b in bufs:
             b.close()
 

@@ -21,10 +21,12 @@
 //             copy has landed (push / push_with_new_buffer, :57-82).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include <chrono>
 #include <deque>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "guard.h"
@@ -145,6 +147,55 @@ hipError_t zk_pinned_alloc(size_t bytes, void **out) {
 }
 
 hipError_t zk_pinned_free(void *p) { return pool().release(p) ? hipSuccess : hipErrorInvalidValue; }
+
+// src/lib.rs:107-112 / src/buffer.rs:19-34: `count` zeroed columns of `rows`
+// Fr (32 B) from the pinned pool, zeroed by up to 16 threads as the
+// reference's rayon loop does; on failure nothing stays allocated.
+hipError_t zk_prepare_advice_buffer(int rows, int count, void **cols) {
+    return zk::guard([&]() -> hipError_t {
+        if (rows <= 0 || count <= 0 || !cols) return hipErrorInvalidValue;
+        const size_t bytes = (size_t)rows * 32;
+        if (bytes > zkh::kSlabBytes) return hipErrorInvalidValue;
+        for (int c = 0; c < count; c++) cols[c] = nullptr;
+        for (int c = 0; c < count; c++) {
+            cols[c] = pool().alloc(bytes);
+            if (!cols[c]) {
+                for (int d = 0; d < c; d++) {
+                    (void)pool().release(cols[d]);
+                    cols[d] = nullptr;
+                }
+                return hipErrorOutOfMemory;
+            }
+        }
+        unsigned hw = std::thread::hardware_concurrency();
+        unsigned nt = hw ? (hw < 16 ? hw : 16) : 1;
+        if (nt > (unsigned)count) nt = (unsigned)count;
+        auto zero = [&](unsigned t) {
+            for (int c = (int)t; c < count; c += (int)nt) memset(cols[c], 0, bytes);
+        };
+        std::vector<std::thread> th;
+        unsigned started = 0;
+        try {
+            for (unsigned t = 1; t < nt; t++, started++) th.emplace_back(zero, t);
+        } catch (...) {
+        }
+        zero(0);
+        for (auto &x : th) x.join();
+        for (unsigned t = started + 1; t < nt; t++) zero(t);  // shares whose thread could not be started
+        return hipSuccess;
+    });
+}
+
+hipError_t zk_release_advice_buffer(int count, void **cols) {
+    if (count < 0 || (count && !cols)) return hipErrorInvalidValue;
+    hipError_t err = hipSuccess;
+    for (int c = 0; c < count; c++) {
+        if (!cols[c]) continue;
+        if (!pool().release(cols[c])) err = hipErrorInvalidValue;
+        cols[c] = nullptr;
+    }
+    return err;
+}
 
 hipError_t zk_pinned_stats(size_t *slabs, size_t *chunks_in_use, size_t *slab_bytes, size_t *chunk_bytes) {
     pool().stats(slabs, chunks_in_use);

@@ -24,7 +24,9 @@ def _decl():
     for name, args in (("zk_pinned_reserve", [SZ]), ("zk_pinned_alloc", [SZ, P]), ("zk_pinned_free", [P]),
                        ("zk_pinned_stats", [P, P, P, P]), ("zk_copyq_create", [ctypes.c_int, P]),
                        ("zk_copyq_push", [P, P, P, SZ, P, ctypes.c_int]), ("zk_copyq_sync_all", [P]),
-                       ("zk_copyq_stats", [P, P, P, P]), ("zk_copyq_destroy", [P])):
+                       ("zk_copyq_stats", [P, P, P, P]), ("zk_copyq_destroy", [P]),
+                       ("zk_prepare_advice_buffer", [ctypes.c_int, ctypes.c_int, P]),
+                       ("zk_release_advice_buffer", [ctypes.c_int, P])):
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
@@ -62,6 +64,31 @@ class PinnedBuffer:
         if self.ptr:
             check(self.L.zk_pinned_free(self.ptr), "zk_pinned_free")
             self.ptr = None
+
+
+class AdviceBuffer:
+    """prepare_advice_buffer (src/lib.rs:107-112): `count` zeroed pinned
+    columns of `rows` Fr each; `ptrs` are the addresses the prover's host
+    witness takes, `column(c)` a numpy (rows, 4) uint64 view of column c"""
+
+    def __init__(self, rows, count):
+        self.L = _decl()
+        self.rows, self.count = rows, count
+        self._cols = (ctypes.c_void_p * count)()
+        check(self.L.zk_prepare_advice_buffer(rows, count, self._cols), "zk_prepare_advice_buffer")
+        self.ptrs = [self._cols[c] for c in range(count)]
+
+    def column(self, c):
+        if not self.ptrs:
+            raise HipError("AdviceBuffer used after close")
+        nbytes = self.rows * 32
+        raw = np.ctypeslib.as_array((ctypes.c_uint8 * nbytes).from_address(self.ptrs[c]))
+        return raw.view(np.uint64).reshape(self.rows, 4)
+
+    def close(self):
+        if self.ptrs:
+            check(self.L.zk_release_advice_buffer(self.count, self._cols), "zk_release_advice_buffer")
+            self.ptrs = []
 
 
 class CopyQueue:
