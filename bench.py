@@ -297,7 +297,8 @@ def main():
     progress("primitives done: MSM 2^22 %.3f ms" % prim_ms["msm_2^22"])
     if not args.no_batch:
         out["batched"] = {"msm": bench_batched_msm(L, points, max(1, args.steps // 2)),
-                          "ntt": bench_batched_ntt(L, args.steps)}
+                          "ntt": bench_batched_ntt(L, args.steps),
+                          "msm_fixed_base_tables": bench_msm_table(L)}
         progress("batched done")
     if not args.no_proof:
         out["proof_core"] = bench_proof_core([int(k) for k in args.proof_ks.split(",") if k])
@@ -357,7 +358,73 @@ def bench_host_copy(msm_ms, reps=5):
 
 class _MsmItem(ctypes.Structure):
     _fields_ = [("points", ctypes.c_void_p), ("scalars", ctypes.c_void_p), ("n", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("table_c", ctypes.c_int)]
+
+
+def bench_msm_table(L, reps=3, count=16):
+    """Fixed-base tables against plain points for the sizes the prover core
+    builds them for (its keys hold tables up to 2^20 points): per size, one
+    batch of `count` MSMs (the prover core's commitment group) over shared
+    bases through zk_msm_batch, plain at the default window and against the
+    table at zk_msm_table_window; host clock, synchronous calls, identical
+    results required.  The table is built once per size (set-up time
+    reported beside it)."""
+    import numpy as np
+    import torch
+    from zkprover import device as dv
+    from zkprover._lib import call
+    dev = dv.device()
+    st = dv.stream_handle()
+    L.zk_msm_table_window.restype = ctypes.c_int
+    L.zk_msm_table_window.argtypes = [ctypes.c_int]
+    L.zk_msm_table_bytes.restype = ctypes.c_size_t
+    L.zk_msm_table_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.zk_msm_table_build.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    L.zk_msm_batch_scratch_bytes.restype = ctypes.c_size_t
+    L.zk_msm_batch_scratch_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.zk_msm_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                               ctypes.c_size_t, ctypes.c_void_p]
+    res = {}
+    for lg in (16, 18, 20):
+        n = 1 << lg
+        pts = torch.empty(n * 64, dtype=torch.uint8, device=dev)
+        sc = torch.empty(count * n * 32, dtype=torch.uint8, device=dev)
+        call("zk_gen_points", pts.data_ptr(), n, 40 + lg, st)
+        call("zk_rand_fr", sc.data_ptr(), count * n, 50 + lg, st)
+        tc = L.zk_msm_table_window(n)
+        table = torch.empty(L.zk_msm_table_bytes(n, tc), dtype=torch.uint8, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        call("zk_msm_table_build", table.data_ptr(), pts.data_ptr(), n, tc, st)
+        torch.cuda.synchronize()
+        build_ms = (time.perf_counter() - t0) * 1e3
+        outs, per = {}, {}
+        for name, src, c in (("plain", pts, 0), ("table", table, tc)):
+            items = (_MsmItem * count)(*[_MsmItem(src.data_ptr(), sc.data_ptr() + m * n * 32, n, c)
+                                         for m in range(count)])
+            nb = L.zk_msm_batch_scratch_bytes(ctypes.addressof(items), count, 0)
+            scratch = torch.empty(nb, dtype=torch.uint8, device=dev)
+            out = torch.zeros(count * 65536, dtype=torch.uint8, device=dev)
+
+            def run():
+                call("zk_msm_batch", out.data_ptr(), ctypes.addressof(items), count, 0, scratch.data_ptr(), nb, st)
+            run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run()
+            torch.cuda.synchronize()
+            per[name] = (time.perf_counter() - t0) * 1e3 / reps / count
+            outs[name] = out.view(count, 65536)[:, :96].cpu().numpy().copy()
+            del scratch, out
+        res[f"2^{lg}"] = {"plain_ms_per_msm": round(per["plain"], 4), "table_ms_per_msm": round(per["table"], 4),
+                          "speedup": round(per["plain"] / per["table"], 3), "plain_window": L.zk_msm_default_window(n),
+                          "table_window": tc, "table_bytes": table.numel(), "build_ms": round(build_ms, 2),
+                          "identical_results": bool((outs["plain"] == outs["table"]).all())}
+        del pts, sc, table
+        torch.cuda.empty_cache()
+    res["workload"] = f"{count} MSMs per size over shared bases, uniform scalars (one commitment group)"
+    return res
 
 
 def bench_batched_msm(L, points, reps):

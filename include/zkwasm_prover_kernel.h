@@ -374,12 +374,16 @@ hipError_t zk_msm_noalloc(void *out, const void *points, const void *scalars, in
  * launch per phase for the whole batch.  Replaces the per-polynomial loop of
  * the reference's batch_msm driver (src/cuda/msm.rs:369-455), whose MSMs go
  * one by one over two streams.  items is a HOST array; result m (Jacobian,
- * z = 1) lands in out + m * ZK_MSM_OUT_BYTES.  Synchronous. */
+ * z = 1) lands in out + m * ZK_MSM_OUT_BYTES.  Synchronous.
+ * table_c = 0: points holds n affine points.  table_c = c in [8, 20]: points
+ * is a fixed-base table that zk_msm_table_build made from n points for window
+ * c (the argument c of the batch call then does not apply to the item).  The
+ * plain items, and the table items of each window, go through one pass each. */
 typedef struct {
     const void *points;
     const void *scalars;
     int n;
-    int reserved;
+    int table_c;
 } zk_msm_item;
 size_t zk_msm_batch_scratch_bytes(const zk_msm_item *items, int count, int c);
 hipError_t zk_msm_batch(void *out, const zk_msm_item *items, int count, int c, void *scratch,
@@ -389,6 +393,22 @@ hipError_t zk_msm_batch(void *out, const zk_msm_item *items, int count, int c, v
  * whose MSM did not finish reads as 96 bytes of 0xff (no point). */
 hipError_t zk_msm_batch_async(void *out, const zk_msm_item *items, int count, int c, void *scratch,
                               size_t scratch_bytes, hipStream_t stream);
+
+/* Fixed-base MSM tables.  No counterpart in the reference, whose msm()
+ * takes the bases afresh on every call: KZG commitments always use the same
+ * SRS points, so a table of T[w * n + i] = 2^(c w) P_i for the ceil(255 / c)
+ * windows w (affine, 64 B each, the identity as (0, 0)) lets every window's
+ * digits share ONE set of 2^(c-1) buckets: the buckets are summed once
+ * instead of once per window, so a wider window pays (c = 20 at 2^22 points:
+ * 13 bucket additions per scalar instead of 15).  Results are the same points
+ * as plain MSMs over P.  zk_msm_table_bytes: the table's size for n points at
+ * window c (8 <= c <= 20; 0 if invalid); zk_msm_table_window: the window with
+ * the fewest additions for n points; zk_msm_table_build: fills table from the
+ * n device points (asynchronous; set-up work, about 0.1 s at 2^22 points).
+ * Use a table as a zk_msm_item with table_c = c. */
+size_t zk_msm_table_bytes(int n, int c);
+int zk_msm_table_window(int n);
+hipError_t zk_msm_table_build(void *table, const void *points, int n, int c, hipStream_t stream);
 
 /* HIP graph capture / replay of a launch sequence (no reference counterpart:
  * the reference re-issues every launch per proof, src/eval_h.rs, src/lib.rs).

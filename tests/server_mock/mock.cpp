@@ -54,6 +54,7 @@ size_t zk_msm_batch_scratch_bytes(const zk_msm_item *items, int count, int) {
     for (int i = 0; i < count; i++) b += (size_t)items[i].n;
     return b;
 }
+int zk_msm_table_window(int) { return 0; }  // no fixed-base tables in the mock
 // result m: x = the item's scalar pointer repeated, z = 1 (not the identity)
 hipError_t zk_msm_batch(void *out, const zk_msm_item *items, int count, int, void *, size_t scratch_bytes,
                         hipStream_t) {

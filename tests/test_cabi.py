@@ -90,7 +90,7 @@ def test_msm_scratch_and_window_queries(lib):
 
 class _Item(ctypes.Structure):
     _fields_ = [("points", ctypes.c_void_p), ("scalars", ctypes.c_void_p), ("n", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("table_c", ctypes.c_int)]
 
 
 def test_msm_batch_scratch_query(lib):
@@ -117,6 +117,37 @@ def test_msm_batch_scratch_query(lib):
     assert lib.zk_msm_batch(None, ctypes.addressof(one), 1, 0, None, 0, None) != 0
     assert lib.zk_msm_batch(ctypes.addressof(one), ctypes.addressof(bad), 2, 0, ctypes.addressof(one), 16,
                             None) != 0
+
+
+def test_msm_table_sizing_host_only(lib):
+    """fixed-base tables: ceil(255 / c) rows of 64 B per point for 8 <= c <= 20;
+    the window model grows with n up to c = 17 (wider windows sort slower than
+    the additions they save); a batch
+    mixing plain and table items needs the larger pass's scratch, and a table
+    item with an invalid window has none"""
+    lib.zk_msm_table_bytes.restype = ctypes.c_size_t
+    lib.zk_msm_table_bytes.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.zk_msm_table_window.restype = ctypes.c_int
+    lib.zk_msm_table_window.argtypes = [ctypes.c_int]
+    for c in range(8, 21):
+        assert lib.zk_msm_table_bytes(1000, c) == -(-255 // c) * 1000 * 64
+    for c in (0, 7, 21, 32):
+        assert lib.zk_msm_table_bytes(1000, c) == 0
+    assert lib.zk_msm_table_bytes(0, 12) == 0 and lib.zk_msm_table_bytes((1 << 31) - 1, 8) == 0
+    assert lib.zk_msm_table_window(1 << 16) == 16 and lib.zk_msm_table_window(1 << 22) == 17
+    ws = [lib.zk_msm_table_window(1 << k) for k in range(4, 27)]
+    assert all(8 <= w <= 17 for w in ws) and ws == sorted(ws)
+    f = lib.zk_msm_batch_scratch_bytes
+    f.restype = ctypes.c_size_t
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    n = 1 << 16
+    plain = (_Item * 1)(_Item(None, None, n, 0))
+    tab = (_Item * 1)(_Item(None, None, n, 16))
+    mixed = (_Item * 2)(_Item(None, None, n, 0), _Item(None, None, n, 16))
+    p, t, m = (f(ctypes.addressof(x), len(x), 0) for x in (plain, tab, mixed))
+    assert p > 0 and t > 0 and m == max(p, t)
+    bad = (_Item * 1)(_Item(None, None, n, 21))
+    assert f(ctypes.addressof(bad), 1, 0) == 0
 
 
 def test_invalid_arguments_rejected_without_gpu(lib):
@@ -198,7 +229,9 @@ def test_tuning_tables_host_only(lib, tmp_path):
 def test_msm_fits_reference_buffers(lib, count):
     """for every size the reference driver may pass, msm() finds a window
     whose scratch fits the index / sort buffers and the curve slice it gets
-    (src/cuda/msm.rs:274-311 sizes); at 2^22 it keeps the default window"""
+    (src/cuda/msm.rs:274-311 sizes); at 2^22 it keeps the default window,
+    elsewhere at most one below it (2^18 points in a batch of 2: the
+    reference's index buffers hold the buckets of c = 14, not of c = 15)"""
     wb, w, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     lib.zkp_msm_policy.restype = None
     lib.zk_msm_plan_window.restype = ctypes.c_int
@@ -209,7 +242,9 @@ def test_msm_fits_reference_buffers(lib, count):
         c = lib.zk_msm_plan_window(n, w.value, wb.value, k.value)
         assert 2 <= c <= 17, (n, count)
         if n >= 1 << 12:
-            assert c >= lib.zk_msm_default_window(n), (n, count, c)
+            assert c >= lib.zk_msm_default_window(n) - 1, (n, count, c)
+        if n == 1 << 22:
+            assert c == lib.zk_msm_default_window(n), (n, count, c)
 
 
 def test_prover_core_sizing_host_only(lib):

@@ -33,9 +33,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "guard.h"
@@ -89,8 +91,24 @@ struct MsmItemDev {
     uint64_t dig0;                    // offset of the item's digits (W * scalars before it)
 };
 
+// Fixed-base mode: the points of an item are a table T[w * n + i] = 2^(c w) P_i
+// (w < Wd = ceil(255 / c) digit windows, built by zk_msm_table_build), so the
+// digit of window w of scalar i adds T[w * n + i] to bucket |d| of ONE bucket
+// set: sum_i s_i P_i = sum_b b * (sum of the entries with digit b).  The
+// additions per scalar stay Wd, but the buckets are reduced once instead of
+// per window, which affords a wider window (c = 20: 13 additions per scalar
+// against 15 at c = 17).  The sort then runs over n * Wd virtual scalars j =
+// w * n + i with up to 2^9 low and 2^10 high bucket bits.
+constexpr uint32_t kHistMax = 3840;         // per-tile LDS counters, plain mode (W * nbins, 2 <= c <= 17)
+constexpr uint32_t kHistMaxFixed = 13312;   // the same in fixed-base mode (Wd * nbins)
+constexpr uint32_t kFixedMinC = 8, kFixedMaxC = 20;
+constexpr uint32_t kFixedModelMaxC = 17;  // the widest window zk_msm_table_window picks
+constexpr uint32_t kFixedMaxBins = 1024, kFixedMaxKeys = 512;
+
 struct MsmPlan {
     uint32_t count, c, W, B, T1, nbits;
+    uint32_t Wd;     // digit windows per scalar (= W, or ceil(255 / c) with W = 1 in fixed-base mode)
+    bool fixed;      // fixed-base mode: points are window tables, one bucket set per item
     uint32_t hi_bits, lo_bits, nbins;
     uint64_t ntot, ntiles, nhist, NB, max_entries, chunks;
     bool packed;  // every item has at most kPackedMaxN scalars: one-word sort entries
@@ -100,27 +118,51 @@ struct MsmPlan {
 
 __host__ __device__ inline uint64_t align256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
 
-MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c) {
+__host__ __device__ inline uint32_t fixed_windows(uint32_t c) { return (255 + c - 1) / c; }
+
+// high bucket bits of fixed-base mode: as many as the per-tile LDS counters
+// (Wd * 2^hi <= kHistMaxFixed) and the scatter's bins (<= 2^10) allow, so the
+// low-level sort has many groups; the rest (<= 9 bits) are low bits
+inline uint32_t fixed_hi_bits(uint32_t c) {
+    const uint32_t Wd = fixed_windows(c);
+    uint32_t hi = 0;
+    while (hi < c - 1 && (2u << hi) <= kFixedMaxBins && (uint64_t)Wd * (2u << hi) <= kHistMaxFixed) hi++;
+    return hi;
+}
+
+MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c, bool fixed = false) {
     MsmPlan p;
     memset(&p, 0, sizeof(p));
     p.count = count;
     p.c = c;
-    p.W = (255 + c - 1) / c;
+    p.fixed = fixed;
+    p.Wd = (255 + c - 1) / c;
+    p.W = fixed ? 1 : p.Wd;
     p.B = 1u << (c - 1);
     p.T1 = (p.B + kL1 - 1) / kL1;
     p.nbits = log2_exact(p.T1);
-    p.hi_bits = c - 1 > 8 ? c - 1 - 8 : 0;
+    if (fixed) {
+        p.hi_bits = fixed_hi_bits(c);
+    } else {
+        // as many high bits (<= 8) as the per-tile LDS counters allow (W *
+        // 2^hi <= kHistMax): the low-level sort gets W * 2^hi groups, so a
+        // narrow window still spreads over the device (c = 13 with 16 bins
+        // had 320 groups of 16K entries: 2^18 points took 0.62 ms to sort)
+        uint32_t hi = 0;
+        while (hi < c - 1 && hi < 8 && (uint64_t)p.W * (2u << hi) <= kHistMax) hi++;
+        p.hi_bits = hi;
+    }
     p.lo_bits = (c - 1) - p.hi_bits;
     p.nbins = 1u << p.hi_bits;
-    p.packed = true;
+    p.packed = !fixed;  // the virtual indices of fixed-base mode need the two-word entries
     for (uint32_t m = 0; m < count; m++) {
         if (ns[m] > kPackedMaxN) p.packed = false;
         p.ntot += ns[m];
         p.ntiles += (ns[m] + kSortTile - 1) / kSortTile;
     }
     p.NB = (uint64_t)count * p.W * p.B;
-    p.nhist = (uint64_t)p.W * p.nbins * p.ntiles;
-    p.max_entries = p.ntot * p.W;
+    p.nhist = (uint64_t)p.Wd * p.nbins * p.ntiles;
+    p.max_entries = p.ntot * p.Wd;
     p.chunks = (p.max_entries + kChunk - 1) / kChunk;
     p.size[A_ITEMS] = (size_t)(count + 1) * sizeof(MsmItemDev);
     p.size[A_SLOTS] = (size_t)count * sizeof(uint8_t *);
@@ -205,21 +247,31 @@ std::atomic<uint8_t> g_window[32];
 
 // Window for a single MSM of n points: the tuned entry for its size class,
 // else the model log2(n) - 5 clamped to [6, 17] (bucket reduction ~ 2^c per
-// window against n additions per window).
+// window against n additions per window), with c = 15 for 2^18..2^20 points:
+// there c = 13 and 14 leave a top window of 7 and 2 bits, whose few buckets
+// take the n / 2 or more entries of that window (measured at 2^18, per MSM:
+// c = 13 1.99 ms, c = 15 1.29 ms).
 uint32_t default_window(uint32_t n) {
     uint32_t lg = log2_exact(n ? n : 1);
     const uint8_t tuned = lg < 32 ? g_window[lg].load(std::memory_order_relaxed) : 0;
     if (tuned) return tuned;
-    int c = (int)lg - 5;
+    int c = lg >= 18 && lg <= 20 ? 15 : (int)lg - 5;
     if (c < 6) c = 6;
     if (c > 17) c = 17;
     return (uint32_t)c;
 }
 
 __device__ __forceinline__ uint32_t scalar_bits(const uint32_t s[8], uint32_t start, uint32_t c) {
-    uint32_t limb = start >> 5, off = start & 31;
-    uint32_t v = s[limb] >> off;
-    if (off && limb + 1 < 8) v |= s[limb + 1] << (32 - off);
+    const uint32_t limb = start >> 5, off = start & 31;
+    // the two limbs by selects: a dynamic index would put s in scratch memory
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) {
+        lo = k == limb ? s[k] : lo;
+        hi = k == limb + 1 ? s[k] : hi;
+    }
+    uint32_t v = lo >> off;
+    if (off) v |= hi << (32 - off);
     return v & ((1u << c) - 1);
 }
 
@@ -264,29 +316,39 @@ __global__ void k_msm_put_items(ItemPack pk, MsmItemDev *__restrict__ items, uin
     slots[pk.first + j] = pk.slot[j];
 }
 
-// hist index of (item m, window w, bin, tile-in-item): groups (m, w, bin) are
-// contiguous and ordered, so their exclusive scan gives every group's start
-__device__ __forceinline__ uint64_t hist_index(const MsmItemDev &it, uint32_t W, uint32_t nbins, uint32_t w,
-                                               uint32_t bin, uint32_t tl) {
-    return (uint64_t)W * nbins * it.tile0 + ((uint64_t)w * nbins + bin) * it.ntiles + tl;
+// hist index of (item m, digit window w, bin, tile-in-item).  Plain mode:
+// groups (m, w, bin) of ntiles counters each; fixed-base mode (one bucket
+// set per item): groups (m, bin) of Wd * ntiles counters, ordered (w, tile)
+// inside, so the virtual scalars w * n + i stay in order.  Groups are
+// contiguous and ordered, so their exclusive scan gives every group's start.
+__device__ __forceinline__ uint64_t hist_index(const MsmItemDev &it, uint32_t Wd, uint32_t nbins, bool fixed,
+                                               uint32_t w, uint32_t bin, uint32_t tl) {
+    const uint64_t h0 = (uint64_t)Wd * nbins * it.tile0;
+    return fixed ? h0 + ((uint64_t)bin * Wd + w) * it.ntiles + tl : h0 + ((uint64_t)w * nbins + bin) * it.ntiles + tl;
+}
+// counters per sort group
+__device__ __forceinline__ uint32_t group_tiles(const MsmItemDev &it, uint32_t Wd, bool fixed) {
+    return fixed ? Wd * it.ntiles : it.ntiles;
 }
 
 // Digits and their histogram in one pass, one block per sort tile (4096
 // scalars of one item, 4 per thread, coalesced).  digits[dig0 + w * n + i] =
 // 0 for a zero digit, else |d| | (d < 0) << 31 with |d| in [1, 2^(c-1)]
-// (bucket |d| - 1); per window the counts of the high bucket bits are kept in
-// LDS (W * nbins <= 3840 counters for every window 2 <= c <= 17) and written
-// once per tile, so the digits are not read back for the histogram.
-constexpr uint32_t kHistMax = 3840;
+// (bucket |d| - 1); per digit window the counts of the high bucket bits are
+// kept in LDS (Wd * nbins <= HMAX counters: 3840 for every plain window 2 <=
+// c <= 17, 13312 in fixed-base mode) and written once per tile, so the digits
+// are not read back for the histogram.
 constexpr uint32_t kDigitThreads = 1024;
+template <uint32_t HMAX>
 __global__ __launch_bounds__(kDigitThreads) void k_msm_digits_hist(const MsmItemDev *__restrict__ items, uint32_t count,
-                                                         uint32_t c, uint32_t W, uint32_t lo_bits, uint32_t nbins,
-                                                         uint32_t *__restrict__ digits, uint32_t *__restrict__ hist) {
-    __shared__ uint32_t h[kHistMax];
+                                                         uint32_t c, uint32_t Wd, uint32_t lo_bits, uint32_t nbins,
+                                                         uint32_t fixed, uint32_t *__restrict__ digits,
+                                                         uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[HMAX];
     const uint32_t m = item_of(items, count, blockIdx.x);
     const MsmItemDev it = items[m];
     const uint32_t tl = blockIdx.x - it.tile0;
-    const uint32_t nh = W * nbins;
+    const uint32_t nh = Wd * nbins;
     for (uint32_t k = threadIdx.x; k < nh; k += blockDim.x) h[k] = 0;
     __syncthreads();
     uint32_t *dst = digits + it.dig0;
@@ -294,7 +356,7 @@ __global__ __launch_bounds__(kDigitThreads) void k_msm_digits_hist(const MsmItem
     for (uint32_t i = tl * kSortTile + threadIdx.x; i < end; i += blockDim.x) {
         const Fr s = fe_from_mont<FrParams>(fe_load<FrParams>(&it.scalars[i]));
         uint32_t carry = 0;
-        for (uint32_t w = 0; w < W; w++) {
+        for (uint32_t w = 0; w < Wd; w++) {
             const int32_t d = signed_digit(s.v, w, c, carry);
             const uint32_t a = (uint32_t)(d < 0 ? -d : d);
             dst[(size_t)w * it.n + i] = d ? (a | (d < 0 ? 0x80000000u : 0u)) : 0u;
@@ -303,7 +365,7 @@ __global__ __launch_bounds__(kDigitThreads) void k_msm_digits_hist(const MsmItem
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < nh; k += blockDim.x)
-        hist[hist_index(it, W, nbins, k / nbins, k % nbins, tl)] = h[k];
+        hist[hist_index(it, Wd, nbins, fixed != 0, k / nbins, k % nbins, tl)] = h[k];
 }
 
 // exclusive scan of x[0..256) in LDS by 256 threads (4 waves): a
@@ -326,6 +388,43 @@ __device__ __forceinline__ void lds_exclusive_scan256(uint32_t *x, uint32_t *t) 
     __syncthreads();
 }
 
+// exclusive scan of x[0..K) (K = 256 R) by 256 threads: thread i owns the R
+// consecutive counters from i R, the 256 per-thread totals are scanned as
+// above
+template <uint32_t K>
+__device__ __forceinline__ void lds_exclusive_scan(uint32_t *x, uint32_t *t) {
+    constexpr uint32_t R = K / 256;
+    static_assert(R >= 1 && R * 256 == K, "K must be a multiple of 256");
+    if constexpr (R == 1) {
+        lds_exclusive_scan256(x, t);
+    } else {
+        const uint32_t i = threadIdx.x, lane = i & 63, wv = i >> 6;
+        uint32_t loc[R];
+        uint32_t v = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            loc[r] = x[i * R + r];
+            v += loc[r];
+        }
+        uint32_t s = v;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = (uint32_t)__shfl_up((int)s, d, 64);
+            if (lane >= d) s += o;
+        }
+        if (lane == 63) t[wv] = s;
+        __syncthreads();
+        uint32_t run = s - v;
+        for (uint32_t k = 0; k < wv; k++) run += t[k];
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            x[i * R + r] = run;
+            run += loc[r];
+        }
+        __syncthreads();
+    }
+}
+
 // Entries between the two sort levels: (low bucket bits, scalar index |
 // sign << 31), as a pair of words, or packed into one word when every item
 // has at most 2^23 scalars (sign | low bits << 23 | index), which halves the
@@ -345,39 +444,46 @@ template <> struct TmpEnt<true> {
     static __device__ __forceinline__ uint32_t key(const T &e) { return (e >> 23) & 0xffu; }
     static __device__ __forceinline__ uint32_t val(const T &e) { return e & 0x807fffffu; }
 };
-// per (tile, window): place each nonzero digit into its (item, window, bin)
-// group; tmp entry = (low bucket bits, index | sign).  The tile is first
-// counting-sorted by bin in LDS (the bin counts are the histogram's), then
-// written out bin run by bin run, so consecutive lanes store to consecutive
-// addresses instead of 256 scattered 8-byte writes per wave.
-template <bool kPacked>
+// per (tile, digit window): place each nonzero digit into its sort group
+// ((item, window, bin), or (item, bin) in fixed-base mode); tmp entry = (low
+// bucket bits, index | sign), the index being the scalar's (plain) or the
+// virtual scalar's w * n + i (fixed-base: the table row of the point).  The
+// tile is first counting-sorted by bin in LDS (the bin counts are the
+// histogram's), then written out bin run by bin run, so consecutive lanes
+// store to consecutive addresses instead of 256 scattered writes per wave.
+// NBMAX: bins the LDS arrays hold (256, or 1024 in fixed-base mode).
+template <bool kPacked, uint32_t NBMAX>
 __global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__restrict__ items, uint32_t count,
-                                                        uint32_t W, const uint32_t *__restrict__ digits,
-                                                        uint32_t lo_bits, uint32_t nbins,
+                                                        uint32_t Wd, const uint32_t *__restrict__ digits,
+                                                        uint32_t lo_bits, uint32_t nbins, uint32_t fixed,
                                                         const uint32_t *__restrict__ hist,
                                                         const uint32_t *__restrict__ starts,
                                                         typename TmpEnt<kPacked>::T *__restrict__ tmp) {
     using E = TmpEnt<kPacked>;
-    __shared__ uint32_t gstart[256], loff[256], lcur[256], sc[256];
+    using BinT = typename std::conditional<(NBMAX > 256), uint16_t, uint8_t>::type;
+    __shared__ uint32_t gstart[NBMAX], loff[NBMAX], lcur[NBMAX], sc[4];
     __shared__ typename E::T stage[kSortTile];
-    __shared__ uint8_t sbin[kSortTile];
+    __shared__ BinT sbin[kSortTile];
     const uint32_t w = blockIdx.y;
     const uint32_t m = item_of(items, count, blockIdx.x);
     const MsmItemDev it = items[m];
     const uint32_t tl = blockIdx.x - it.tile0;
     const uint32_t tid = threadIdx.x;
-    uint32_t cnt = 0;
-    if (tid < nbins) {
-        const uint64_t hi = hist_index(it, W, nbins, w, tid, tl);
-        gstart[tid] = starts[hi];
-        cnt = hist[hi];
+    for (uint32_t k = tid; k < NBMAX; k += 256) {
+        uint32_t cnt = 0;
+        if (k < nbins) {
+            const uint64_t hi = hist_index(it, Wd, nbins, fixed != 0, w, k, tl);
+            gstart[k] = starts[hi];
+            cnt = hist[hi];
+        }
+        loff[k] = cnt;  // bins past nbins count 0
     }
-    loff[tid] = cnt;  // bins past nbins count 0
     __syncthreads();
-    lds_exclusive_scan256(loff, sc);
-    lcur[tid] = loff[tid];
+    lds_exclusive_scan<NBMAX>(loff, sc);
+    for (uint32_t k = tid; k < NBMAX; k += 256) lcur[k] = loff[k];
     __syncthreads();
     const uint32_t *dw = digits + it.dig0 + (size_t)w * it.n;
+    const uint32_t vbase = fixed ? w * it.n : 0u;  // fixed-base: the table row of (w, i) is w * n + i
     const uint32_t lo_mask = (1u << lo_bits) - 1;
     const uint32_t begin = tl * kSortTile;
     const uint32_t end = begin + kSortTile < it.n ? begin + kSortTile : it.n;
@@ -386,8 +492,8 @@ __global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__rest
         if (d) {
             uint32_t b = (d & 0x7fffffffu) - 1;
             uint32_t p = atomicAdd(&lcur[b >> lo_bits], 1u);
-            stage[p] = E::make(b & lo_mask, i | (d & 0x80000000u));
-            sbin[p] = (uint8_t)(b >> lo_bits);
+            stage[p] = E::make(b & lo_mask, (vbase + i) | (d & 0x80000000u));
+            sbin[p] = (BinT)(b >> lo_bits);
         }
     }
     __syncthreads();
@@ -398,48 +504,52 @@ __global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__rest
     }
 }
 
-// per (item, window, bin) group: counting sort on the low bucket bits.  The
-// group histogram gives every bucket's offset; the entries are then placed
-// tile by tile (4096 at a time): each tile is counting-sorted in LDS and
-// written out bucket run by bucket run, so stores come in contiguous runs
-// rather than one 4-byte scatter per entry.  Also emits the bucket offsets.
+// per sort group ((item, window, bin), or (item, bin) in fixed-base mode):
+// counting sort on the low bucket bits.  The group histogram gives every
+// bucket's offset; the entries are then placed tile by tile (4096 at a time):
+// each tile is counting-sorted in LDS and written out bucket run by bucket
+// run, so stores come in contiguous runs rather than one 4-byte scatter per
+// entry.  Also emits the bucket offsets.  KEYS: low-bit values the LDS arrays
+// hold (256, or 512 in fixed-base mode).
 constexpr uint32_t kLoTile = 4096;
-template <bool kPacked>
-__global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restrict__ items, uint32_t W, uint32_t nbins,
+template <bool kPacked, uint32_t KEYS>
+__global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restrict__ items, uint32_t W, uint32_t Wd,
+                                                     uint32_t nbins, uint32_t fixed,
                                                      const typename TmpEnt<kPacked>::T *__restrict__ tmp,
                                                      const uint32_t *__restrict__ starts, uint32_t lo_bits,
                                                      uint32_t B, uint32_t ngroups, uint32_t *__restrict__ entries,
                                                      uint32_t *__restrict__ offsets) {
     using E = TmpEnt<kPacked>;
-    __shared__ uint32_t h[256], cur[256], toff[256], tcur[256], sc[256];
+    using KeyT = typename std::conditional<(KEYS > 256), uint16_t, uint8_t>::type;
+    __shared__ uint32_t h[KEYS], cur[KEYS], toff[KEYS], tcur[KEYS], sc[4];
     __shared__ uint32_t stage[kLoTile];
-    __shared__ uint8_t skey[kLoTile];
+    __shared__ KeyT skey[kLoTile];
     const uint32_t g = blockIdx.x;
     const uint32_t m = g / (W * nbins), r = g % (W * nbins);
     const uint32_t w = r / nbins, bin = r % nbins;
     const MsmItemDev it = items[m];
     const uint32_t nlo = 1u << lo_bits;
     const uint32_t tid = threadIdx.x;
-    const uint64_t idx = hist_index(it, W, nbins, w, bin, 0);
+    const uint64_t idx = hist_index(it, Wd, nbins, fixed != 0, w, bin, 0);
     const uint32_t gs = starts[idx];
-    const uint32_t ge = starts[idx + it.ntiles];  // the next group's start (or the total)
-    h[tid] = 0;
+    const uint32_t ge = starts[idx + group_tiles(it, Wd, fixed != 0)];  // the next group's start (or the total)
+    for (uint32_t k = tid; k < KEYS; k += 256) h[k] = 0;
     __syncthreads();
     for (uint32_t p = gs + tid; p < ge; p += blockDim.x) atomicAdd(&h[E::key(tmp[p])], 1u);
     __syncthreads();
-    cur[tid] = h[tid];
+    for (uint32_t k = tid; k < KEYS; k += 256) cur[k] = h[k];
     __syncthreads();
-    lds_exclusive_scan256(cur, sc);
-    cur[tid] += gs;
+    lds_exclusive_scan<KEYS>(cur, sc);
+    for (uint32_t k = tid; k < KEYS; k += 256) cur[k] += gs;
     __syncthreads();
     const uint64_t base = ((uint64_t)m * W + w) * B + ((uint64_t)bin << lo_bits);
-    if (tid < nlo) offsets[base + tid] = cur[tid];
+    for (uint32_t k = tid; k < nlo; k += 256) offsets[base + k] = cur[k];
     if (g == ngroups - 1 && tid == 0) offsets[base + nlo] = ge;  // = total: offsets[NB]
     for (uint32_t t0 = gs; t0 < ge; t0 += kLoTile) {
         const uint32_t cnt = ge - t0 < kLoTile ? ge - t0 : kLoTile;
         // the tile in registers: 16 entries per thread
         typename E::T v[kLoTile / 256];
-        h[tid] = 0;
+        for (uint32_t k = tid; k < KEYS; k += 256) h[k] = 0;
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < kLoTile / 256; k++) {
@@ -450,10 +560,10 @@ __global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restric
             }
         }
         __syncthreads();
-        toff[tid] = h[tid];
+        for (uint32_t k = tid; k < KEYS; k += 256) toff[k] = h[k];
         __syncthreads();
-        lds_exclusive_scan256(toff, sc);
-        tcur[tid] = toff[tid];
+        lds_exclusive_scan<KEYS>(toff, sc);
+        for (uint32_t k = tid; k < KEYS; k += 256) tcur[k] = toff[k];
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < kLoTile / 256; k++) {
@@ -461,7 +571,7 @@ __global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restric
             if (q < cnt) {
                 const uint32_t at = atomicAdd(&tcur[E::key(v[k])], 1u);
                 stage[at] = E::val(v[k]);
-                skey[at] = (uint8_t)E::key(v[k]);
+                skey[at] = (KeyT)E::key(v[k]);
             }
         }
         __syncthreads();
@@ -470,7 +580,7 @@ __global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restric
             entries[cur[key] + (q - toff[key])] = stage[q];
         }
         __syncthreads();
-        cur[tid] += h[tid];
+        for (uint32_t k = tid; k < KEYS; k += 256) cur[k] += h[k];
         __syncthreads();
     }
 }
@@ -836,7 +946,7 @@ __global__ void k_slice_header(uint8_t *const *slots, uint32_t count, uint32_t c
 }
 
 __host__ __device__ __forceinline__ bool header_ok(const SliceHeader *h) {
-    return h->magic == kMagic && h->W >= 1 && h->W <= 128 && h->c >= 2 && h->c <= 17 && h->nbits <= 16;
+    return h->magic == kMagic && h->W >= 1 && h->W <= 128 && h->c >= 2 && h->c <= kFixedMaxC && h->nbits <= 16;
 }
 
 // Device tail, step 1 (grid: W threads per slice, y = slice):
@@ -890,6 +1000,35 @@ __global__ __launch_bounds__(64) void k_msm_collect(uint8_t *const *__restrict__
 __global__ void k_slice_clear(uint8_t *const *slots, uint32_t count) {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < count) reinterpret_cast<SliceHeader *>(slots[m] + 64)->magic = 0;
+}
+
+// Fixed-base table T[w * n + i] = 2^(c w) P_i for w < Wd (affine, Montgomery,
+// the identity kept as (0, 0)): one thread per point, c Jacobian doublings
+// and one inversion per window.  A set-up pass (about 0.1 s at 2^22 points).
+__global__ __launch_bounds__(256) void k_msm_table(const G1Aff *__restrict__ pts, G1Aff *__restrict__ table,
+                                                   uint32_t n, uint32_t c, uint32_t Wd) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const G1Aff p = aff_load(&pts[i]);
+    fe_store<FpParams>(&table[i].x, p.x);
+    fe_store<FpParams>(&table[i].y, p.y);
+    G1Jac j;
+    j.x = p.x;
+    j.y = p.y;
+    j.z = aff_is_id(p) ? fe_zero<FpParams>() : fe_one<FpParams>();
+    for (uint32_t w = 1; w < Wd; w++) {
+        for (uint32_t k = 0; k < c; k++) j = jac_dbl(j);
+        Fp x = fe_zero<FpParams>(), y = fe_zero<FpParams>();
+        if (!fe_is_zero<FpParams>(j.z)) {
+            const Fp zi = fe_inv<FpParams>(j.z);
+            const Fp zi2 = fe_sqr<FpParams>(zi);
+            x = j.x * zi2;
+            y = j.y * zi2 * zi;
+        }
+        G1Aff *o = &table[(size_t)w * n + i];
+        fe_store<FpParams>(&o->x, x);
+        fe_store<FpParams>(&o->y, y);
+    }
 }
 
 // Opt-in phase timing: when enabled (and the stream is not being captured), msm_run records a HIP event on the
@@ -949,8 +1088,8 @@ struct MsmJob {
     uint8_t *slice;
 };
 
-size_t msm_scratch_bytes(const uint32_t *ns, uint32_t count, uint32_t c) {
-    return plan_total(make_plan(ns, count, c));
+size_t msm_scratch_bytes(const uint32_t *ns, uint32_t count, uint32_t c, bool fixed = false) {
+    return plan_total(make_plan(ns, count, c, fixed));
 }
 uint32_t msm_default_window(uint32_t n) { return default_window(n); }
 // window for a batch: the default for its mean size
@@ -962,19 +1101,20 @@ uint32_t batch_default_window(const uint32_t *ns, uint32_t count) {
 
 // Picks the window and places the arrays: the requested window, or (c == 0)
 // the default window first, then larger windows (fewer entries), then
-// smaller ones, until the arrays fit the regions.  Host-only.
+// smaller ones, until the arrays fit the regions.  Fixed-base mode: only the
+// table's window c.  Host-only.
 bool choose_plan(const uint32_t *ns, uint32_t count, size_t slice_bytes, uint32_t c, uint8_t *const *regions,
-                 const size_t *region_bytes, int nreg, MsmPlan &p) {
+                 const size_t *region_bytes, int nreg, MsmPlan &p, bool fixed = false) {
     const uint32_t c0 = c ? c : batch_default_window(ns, count);
     uint32_t order[32];
     int norder = 0;
     order[norder++] = c0;
-    if (!c) {
+    if (!c && !fixed) {
         for (uint32_t cc = c0 + 1; cc <= 17; cc++) order[norder++] = cc;
         for (uint32_t cc = c0 - 1; cc >= 2; cc--) order[norder++] = cc;
     }
     for (int k = 0; k < norder; k++) {
-        p = make_plan(ns, count, order[k]);
+        p = make_plan(ns, count, order[k], fixed);
         // entry positions are u32 and the masked-sum grid's y is < 2^16
         if (p.max_entries < 0x80000000ull && (uint64_t)count * p.W < 65536 &&
             slice_bytes_needed(p.W, p.nbits) <= slice_bytes && place(p, regions, region_bytes, nreg))
@@ -990,10 +1130,17 @@ bool choose_plan(const uint32_t *ns, uint32_t count, size_t slice_bytes, uint32_
 // are placed into the given regions; no allocation happens.  c == 0 picks the
 // largest window <= the default that fits the regions.  *slots_dev receives
 // the device array of slice pointers (inside the scratch) for a device tail.
+// fixed: every job's points are a table built for window c (kFixedMinC <= c
+// <= kFixedMaxC), see zk_msm_table_build.
 hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint32_t c, uint8_t *const *regions,
-                   const size_t *region_bytes, int nreg, hipStream_t stream, uint8_t ***slots_dev) {
+                   const size_t *region_bytes, int nreg, hipStream_t stream, uint8_t ***slots_dev,
+                   bool fixed = false) {
     if (count == 0 || !jobs || nreg < 1 || nreg > 8) return hipErrorInvalidValue;
-    if (c != 0 && (c < 2 || c > 17)) return hipErrorInvalidValue;  // sort kernels: <= 8 + 8 bucket bits
+    if (fixed) {
+        if (c < kFixedMinC || c > kFixedMaxC) return hipErrorInvalidValue;
+    } else if (c != 0 && (c < 2 || c > 17)) {
+        return hipErrorInvalidValue;  // sort kernels: <= 8 + 8 bucket bits
+    }
     // item sizes: on the stack for small batches (no host allocation on the
     // path of a single MSM)
     uint32_t ns_small[64];
@@ -1011,7 +1158,7 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
         ntot += j.n;
     }
     MsmPlan p;
-    if (!choose_plan(ns, count, slice_bytes, c, regions, region_bytes, nreg, p)) return hipErrorOutOfMemory;
+    if (!choose_plan(ns, count, slice_bytes, c, regions, region_bytes, nreg, p, fixed)) return hipErrorOutOfMemory;
     c = p.c;
     // phase timing waits on events, which a stream under graph capture cannot
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -1056,7 +1203,7 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
                 it.dig0 = dig;
                 pk.slot[j] = jb.slice;
                 tile += it.ntiles;
-                dig += (uint64_t)jb.n * p.W;
+                dig += (uint64_t)jb.n * p.Wd;
             }
             k_msm_put_items<<<1, kItemsPerLaunch, 0, stream>>>(pk, items, slots, longs);
             ZK_CHECK(hipGetLastError());
@@ -1064,26 +1211,43 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     }
     k_slice_clear<<<(count + 63) / 64, 64, 0, stream>>>(slots, count);
     ZK_CHECK(hipGetLastError());
-    if ((uint64_t)p.W * p.nbins > kHistMax) return hipErrorInvalidValue;  // not for 2 <= c <= 17
-    k_msm_digits_hist<<<(uint32_t)p.ntiles, kDigitThreads, 0, stream>>>(items, count, c, p.W, p.lo_bits, p.nbins,
-                                                                         digits, hist);
+    const uint32_t fx = p.fixed ? 1u : 0u;
+    if (p.fixed) {
+        if ((uint64_t)p.Wd * p.nbins > kHistMaxFixed || p.nbins > kFixedMaxBins || (1u << p.lo_bits) > kFixedMaxKeys)
+            return hipErrorInvalidValue;  // not for kFixedMinC <= c <= kFixedMaxC
+        k_msm_digits_hist<kHistMaxFixed><<<(uint32_t)p.ntiles, kDigitThreads, 0, stream>>>(
+            items, count, c, p.Wd, p.lo_bits, p.nbins, fx, digits, hist);
+    } else {
+        if ((uint64_t)p.W * p.nbins > kHistMax || p.nbins > 256 || p.lo_bits > 8)
+            return hipErrorInvalidValue;  // not for 2 <= c <= 17
+        k_msm_digits_hist<kHistMax><<<(uint32_t)p.ntiles, kDigitThreads, 0, stream>>>(items, count, c, p.Wd, p.lo_bits,
+                                                                                   p.nbins, fx, digits, hist);
+    }
     ZK_CHECK(hipGetLastError());
     ZK_CHECK(prof_mark(prof, PH_SCAN, stream));
     ZK_CHECK(launch_scan(hist, (uint32_t)p.nhist, starts, nullptr, scan_sums, stream));
     ZK_CHECK(prof_mark(prof, PH_SCATTER, stream));
     const uint32_t ngroups = count * p.W * p.nbins;
-    if (p.packed) {
-        k_msm_scatter_hi<true><<<dim3((uint32_t)p.ntiles, p.W), 256, 0, stream>>>(
-            items, count, p.W, digits, p.lo_bits, p.nbins, hist, starts, (uint32_t *)tmp);
+    const dim3 sgrid((uint32_t)p.ntiles, p.Wd);
+    if (p.fixed) {
+        k_msm_scatter_hi<false, kFixedMaxBins><<<sgrid, 256, 0, stream>>>(items, count, p.Wd, digits, p.lo_bits,
+                                                                         p.nbins, fx, hist, starts, (uint2 *)tmp);
         ZK_CHECK(hipGetLastError());
-        k_msm_sort_lo<true><<<ngroups, 256, 0, stream>>>(items, p.W, p.nbins, (const uint32_t *)tmp, starts,
-                                                          p.lo_bits, p.B, ngroups, entries, offsets);
+        k_msm_sort_lo<false, kFixedMaxKeys><<<ngroups, 256, 0, stream>>>(items, p.W, p.Wd, p.nbins, fx,
+                                                                        (const uint2 *)tmp, starts, p.lo_bits, p.B,
+                                                                        ngroups, entries, offsets);
+    } else if (p.packed) {
+        k_msm_scatter_hi<true, 256><<<sgrid, 256, 0, stream>>>(items, count, p.Wd, digits, p.lo_bits, p.nbins, fx,
+                                                               hist, starts, (uint32_t *)tmp);
+        ZK_CHECK(hipGetLastError());
+        k_msm_sort_lo<true, 256><<<ngroups, 256, 0, stream>>>(items, p.W, p.Wd, p.nbins, fx, (const uint32_t *)tmp,
+                                                              starts, p.lo_bits, p.B, ngroups, entries, offsets);
     } else {
-        k_msm_scatter_hi<false><<<dim3((uint32_t)p.ntiles, p.W), 256, 0, stream>>>(
-            items, count, p.W, digits, p.lo_bits, p.nbins, hist, starts, (uint2 *)tmp);
+        k_msm_scatter_hi<false, 256><<<sgrid, 256, 0, stream>>>(items, count, p.Wd, digits, p.lo_bits, p.nbins, fx,
+                                                                hist, starts, (uint2 *)tmp);
         ZK_CHECK(hipGetLastError());
-        k_msm_sort_lo<false><<<ngroups, 256, 0, stream>>>(items, p.W, p.nbins, (const uint2 *)tmp, starts,
-                                                           p.lo_bits, p.B, ngroups, entries, offsets);
+        k_msm_sort_lo<false, 256><<<ngroups, 256, 0, stream>>>(items, p.W, p.Wd, p.nbins, fx, (const uint2 *)tmp,
+                                                               starts, p.lo_bits, p.B, ngroups, entries, offsets);
     }
     ZK_CHECK(hipGetLastError());
     ZK_CHECK(prof_mark(prof, PH_ACCUMULATE, stream));
@@ -1271,27 +1435,83 @@ hipError_t msm_one_async(void *out, const void *points, const void *scalars, uin
     return device_collect(slots, 1, stream);
 }
 
+// Items of a batch by kind: plain points (table_c = 0) and, per table
+// window, fixed-base tables; each kind goes through the pipeline as one pass.
+// Plain first, then tables by window.  False if an item is invalid.
+struct ItemGroup {
+    uint32_t table_c;
+    std::vector<uint32_t> idx;
+};
+bool group_items(const zk_msm_item *items, int count, std::vector<ItemGroup> &groups) {
+    groups.clear();
+    for (int m = 0; m < count; m++) {
+        const int tc = items[m].table_c;
+        if (items[m].n <= 0) return false;
+        if (tc != 0 && (tc < (int)kFixedMinC || tc > (int)kFixedMaxC)) return false;
+        if (tc != 0 && (uint64_t)fixed_windows((uint32_t)tc) * (uint64_t)items[m].n >= 0x80000000ull) return false;
+        size_t g = 0;
+        while (g < groups.size() && groups[g].table_c != (uint32_t)tc) g++;
+        if (g == groups.size()) groups.push_back(ItemGroup{(uint32_t)tc, {}});
+        groups[g].idx.push_back((uint32_t)m);
+    }
+    std::sort(groups.begin(), groups.end(),
+              [](const ItemGroup &x, const ItemGroup &y) { return x.table_c < y.table_c; });
+    return true;
+}
+
+// scratch bytes for the batch: the largest pass (the passes run one after
+// another in the same scratch); 0 if an item is invalid
+size_t items_scratch_bytes(const zk_msm_item *items, int count, uint32_t c) {
+    std::vector<ItemGroup> groups;
+    if (count <= 0 || !items || !group_items(items, count, groups)) return 0;
+    size_t best = 0;
+    for (const ItemGroup &g : groups) {
+        std::vector<uint32_t> ns(g.idx.size());
+        for (size_t k = 0; k < g.idx.size(); k++) ns[k] = (uint32_t)items[g.idx[k]].n;
+        const uint32_t cnt = (uint32_t)ns.size();
+        const size_t b = g.table_c ? msm_scratch_bytes(ns.data(), cnt, g.table_c, true)
+                                   : msm_scratch_bytes(ns.data(), cnt, c ? c : batch_default_window(ns.data(), cnt));
+        best = b > best ? b : best;
+    }
+    return best + 256;
+}
+
+// Runs every pass of the batch up to the masked sums in item m's slice at out
+// + m * ZK_MSM_OUT_BYTES; device_tail: each pass's results are combined on the
+// device right after it (the pass's slice pointers live in the scratch, which
+// the next pass reuses).  c: the window of the plain items (0 = default).
+hipError_t run_items(void *out, const zk_msm_item *items, int count, uint32_t c, void *scratch, size_t scratch_bytes,
+                     hipStream_t stream, bool device_tail) {
+    std::vector<ItemGroup> groups;
+    if (count <= 0 || !items || !out || !scratch || !group_items(items, count, groups)) return hipErrorInvalidValue;
+    uint8_t *regions[1] = {(uint8_t *)scratch};
+    size_t bytes[1] = {scratch_bytes};
+    for (const ItemGroup &g : groups) {
+        std::vector<MsmJob> jobs(g.idx.size());
+        for (size_t k = 0; k < g.idx.size(); k++) {
+            const zk_msm_item &it = items[g.idx[k]];
+            jobs[k] = MsmJob{(const G1Aff *)it.points, (const Fr *)it.scalars, (uint32_t)it.n,
+                             (uint8_t *)out + (size_t)g.idx[k] * ZK_MSM_OUT_BYTES};
+        }
+        uint8_t **slots = nullptr;
+        ZK_CHECK(msm_run(jobs.data(), (uint32_t)jobs.size(), ZK_MSM_OUT_BYTES, g.table_c ? g.table_c : c, regions,
+                         bytes, 1, stream, &slots, g.table_c != 0));
+        if (device_tail) ZK_CHECK(device_collect(slots, (uint32_t)jobs.size(), stream));
+    }
+    return hipSuccess;
+}
+
 // zk_msm_batch whose results also land in host memory (jac_host: count x 96
 // B) without a copy per item (the prover core's commitments)
 hipError_t msm_batch_host(void *out, const zk_msm_item *items, int count, void *scratch, size_t scratch_bytes,
                           hipStream_t stream, uint8_t *jac_host) {
     if (count <= 0 || !items || !out || !scratch || !jac_host) return hipErrorInvalidValue;
-    std::vector<MsmJob> jobs(count);
-    std::vector<uint8_t *> slices(count);
-    for (int m = 0; m < count; m++) {
-        if (items[m].n <= 0) return hipErrorInvalidValue;
-        slices[m] = (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES;
-        jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n, slices[m]};
-    }
-    uint8_t *regions[1] = {(uint8_t *)scratch};
-    size_t bytes[1] = {scratch_bytes};
-    uint8_t **slots = nullptr;
-    ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, 0, regions, bytes, 1, stream, &slots));
-    if (!device_tail_requested())
-        return host_collect_strided((uint8_t *)out, ZK_MSM_OUT_BYTES, (uint32_t)count, stream, jac_host);
-    ZK_CHECK(device_collect(slots, (uint32_t)count, stream));
+    const bool dev = device_tail_requested();
+    ZK_CHECK(run_items(out, items, count, 0, scratch, scratch_bytes, stream, dev));
+    if (!dev) return host_collect_strided((uint8_t *)out, ZK_MSM_OUT_BYTES, (uint32_t)count, stream, jac_host);
     for (int m = 0; m < count; m++)
-        ZK_CHECK(hipMemcpyAsync(jac_host + (size_t)m * 96, slices[m], 96, hipMemcpyDeviceToHost, stream));
+        ZK_CHECK(hipMemcpyAsync(jac_host + (size_t)m * 96, (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES, 96,
+                                hipMemcpyDeviceToHost, stream));
     ZK_CHECK(hipStreamSynchronize(stream));
     for (int m = 0; m < count; m++) {  // the device tail's no-result marker: 96 bytes of 0xff
         const uint8_t *q = jac_host + (size_t)m * 96;
@@ -1394,19 +1614,24 @@ size_t zk_msm_scratch_bytes(int n, int c) {
     return msm_scratch_bytes(&un, 1, cc) + 256;
 }
 
+// after a device tail: a slice without masked sums carries the no-result
+// marker (every byte 0xff), which makes the call fail
+static hipError_t check_markers(uint8_t *const *slices, uint32_t count, hipStream_t stream) {
+    std::vector<uint32_t> z((size_t)count * 8);
+    for (uint32_t m = 0; m < count; m++)
+        ZK_CHECK(hipMemcpyAsync(&z[(size_t)m * 8], slices[m] + 64, 32, hipMemcpyDeviceToHost, stream));
+    ZK_CHECK(hipStreamSynchronize(stream));
+    for (uint32_t m = 0; m < count; m++)
+        if (z[(size_t)m * 8] == 0xffffffffu && z[(size_t)m * 8 + 7] == 0xffffffffu) return hipErrorInvalidValue;
+    return hipSuccess;
+}
+
 // Combines the masked sums of the given slices into their results (host or,
 // with ZK_MSM_TAIL=device, device tail) and waits for them.
 static hipError_t finish_batch(uint8_t *const *slices, uint8_t **slots_dev, uint32_t count, hipStream_t stream) {
     if (device_tail_requested()) {
         ZK_CHECK(device_collect(slots_dev, count, stream));
-        // a slice without masked sums carries the no-result marker
-        std::vector<uint32_t> z((size_t)count * 8);
-        for (uint32_t m = 0; m < count; m++)
-            ZK_CHECK(hipMemcpyAsync(&z[(size_t)m * 8], slices[m] + 64, 32, hipMemcpyDeviceToHost, stream));
-        ZK_CHECK(hipStreamSynchronize(stream));
-        for (uint32_t m = 0; m < count; m++)
-            if (z[(size_t)m * 8] == 0xffffffffu && z[(size_t)m * 8 + 7] == 0xffffffffu) return hipErrorInvalidValue;
-        return hipSuccess;
+        return check_markers(slices, count, stream);
     }
     return host_collect(slices, count, stream);
 }
@@ -1429,41 +1654,26 @@ hipError_t zk_msm(void *out, const void *points, const void *scalars, int n, int
     });
 }
 
-// Scratch bytes zk_msm_batch needs for these items at window c (0: default).
+// Scratch bytes zk_msm_batch needs for these items at window c (0: default;
+// table items use their table's window).
 size_t zk_msm_batch_scratch_bytes(const zk_msm_item *items, int count, int c) {
-    if (count <= 0 || !items) return 0;
-    std::vector<uint32_t> ns(count);
-    for (int m = 0; m < count; m++) {
-        if (items[m].n <= 0) return 0;
-        ns[m] = (uint32_t)items[m].n;
-    }
-    uint32_t cc = c > 0 ? (uint32_t)c : batch_default_window(ns.data(), (uint32_t)count);
-    return msm_scratch_bytes(ns.data(), (uint32_t)count, cc) + 256;
+    return items_scratch_bytes(items, count, c > 0 ? (uint32_t)c : 0);
 }
 
-// count MSMs in one pass of the pipeline (one launch per phase for the whole
-// batch).  items is a host array; item m's result lands in
-// out + m * ZK_MSM_OUT_BYTES (Jacobian, z = 1).  Returns after the results
-// are written.
+// count MSMs in one pass of the pipeline per kind of item (plain points, and
+// tables of each window; one launch per phase for all items of a kind).
+// items is a host array; item m's result lands in out + m * ZK_MSM_OUT_BYTES
+// (Jacobian, z = 1).  Returns after the results are written.
 hipError_t zk_msm_batch(void *out, const zk_msm_item *items, int count, int c, void *scratch,
                         size_t scratch_bytes, hipStream_t stream) {
     return zk::guard([&]() -> hipError_t {
         if (count <= 0 || !items || !out || !scratch) return hipErrorInvalidValue;
-        std::vector<MsmJob> jobs(count);
+        const bool dev = device_tail_requested();
+        ZK_CHECK(run_items(out, items, count, c > 0 ? (uint32_t)c : 0, scratch, scratch_bytes, stream, dev));
+        if (!dev) return host_collect_strided((uint8_t *)out, ZK_MSM_OUT_BYTES, (uint32_t)count, stream, nullptr);
         std::vector<uint8_t *> slices(count);
-        for (int m = 0; m < count; m++) {
-            if (items[m].n <= 0) return hipErrorInvalidValue;
-            slices[m] = (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES;
-            jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n,
-                             slices[m]};
-        }
-        uint8_t *regions[1] = {(uint8_t *)scratch};
-        size_t bytes[1] = {scratch_bytes};
-        uint8_t **slots = nullptr;
-        ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1,
-                         stream, &slots));
-        if (device_tail_requested()) return finish_batch(slices.data(), slots, (uint32_t)count, stream);
-        return host_collect_strided((uint8_t *)out, ZK_MSM_OUT_BYTES, (uint32_t)count, stream, nullptr);
+        for (int m = 0; m < count; m++) slices[m] = (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES;
+        return check_markers(slices.data(), (uint32_t)count, stream);
     });
 }
 
@@ -1559,18 +1769,45 @@ hipError_t zk_msm_batch_async(void *out, const zk_msm_item *items, int count, in
                               size_t scratch_bytes, hipStream_t stream) {
     return zk::guard([&]() -> hipError_t {
         if (count <= 0 || !items || !out || !scratch) return hipErrorInvalidValue;
-        std::vector<MsmJob> jobs(count);
-        for (int m = 0; m < count; m++) {
-            if (items[m].n <= 0) return hipErrorInvalidValue;
-            jobs[m] = MsmJob{(const G1Aff *)items[m].points, (const Fr *)items[m].scalars, (uint32_t)items[m].n,
-                             (uint8_t *)out + (size_t)m * ZK_MSM_OUT_BYTES};
+        return run_items(out, items, count, c > 0 ? (uint32_t)c : 0, scratch, scratch_bytes, stream, true);
+    });
+}
+
+// Rows of a fixed-base table: ceil(255 / c) per point.
+size_t zk_msm_table_bytes(int n, int c) {
+    if (n <= 0 || c < (int)kFixedMinC || c > (int)kFixedMaxC) return 0;
+    const uint64_t rows = (uint64_t)fixed_windows((uint32_t)c) * (uint64_t)n;
+    return rows < 0x80000000ull ? (size_t)rows * sizeof(G1Aff) : 0;
+}
+
+// The table window for n points: fewest additions, ceil(255 / c) n to fill
+// the buckets plus about 2^(c+1) to reduce them (the larger c on a tie), up
+// to c = 17.  Wider windows need more than 8 + 8 sorted bucket bits, whose
+// two sort levels then write shorter runs over fewer groups; measured on
+// MI355X that costs more than the additions saved (2^20 points, batches of
+// 8: 1.72 ms per MSM at c = 17, 1.84 ms at c = 20).
+int zk_msm_table_window(int n) {
+    if (n <= 0) return 0;
+    int best = 0;
+    double cost = 0;
+    for (uint32_t c = kFixedMinC; c <= kFixedModelMaxC; c++) {
+        if (!zk_msm_table_bytes(n, (int)c)) continue;
+        const double t = (double)fixed_windows(c) * n + (double)(2u << c);
+        if (!best || t <= cost) {
+            best = (int)c;
+            cost = t;
         }
-        uint8_t *regions[1] = {(uint8_t *)scratch};
-        size_t bytes[1] = {scratch_bytes};
-        uint8_t **slots = nullptr;
-        ZK_CHECK(msm_run(jobs.data(), (uint32_t)count, ZK_MSM_OUT_BYTES, c > 0 ? (uint32_t)c : 0, regions, bytes, 1,
-                         stream, &slots));
-        return device_collect(slots, (uint32_t)count, stream);
+    }
+    return best;
+}
+
+hipError_t zk_msm_table_build(void *table, const void *points, int n, int c, hipStream_t stream) {
+    return zk::guard([&]() -> hipError_t {
+        if (!table || !points || !zk_msm_table_bytes(n, c)) return hipErrorInvalidValue;
+        k_msm_table<<<((uint32_t)n + 255) / 256, 256, 0, stream>>>((const G1Aff *)points, (G1Aff *)table,
+                                                                     (uint32_t)n, (uint32_t)c,
+                                                                     fixed_windows((uint32_t)c));
+        return hipGetLastError();
     });
 }
 

@@ -382,9 +382,14 @@ struct Server {
     }
 };
 
+// scratch of one proof's commitment group: plain points or the key's
+// fixed-base tables, whichever needs more
 size_t group_scratch(const zkp_circuit &c) {
     std::vector<zk_msm_item> it(kGroupItems, zk_msm_item{nullptr, nullptr, 1 << c.k, 0});
-    return zk_msm_batch_scratch_bytes(it.data(), kGroupItems, 0);
+    const size_t plain = zk_msm_batch_scratch_bytes(it.data(), kGroupItems, 0);
+    for (auto &x : it) x.table_c = zk_msm_table_window(1 << c.k);
+    const size_t tab = zk_msm_batch_scratch_bytes(it.data(), kGroupItems, 0);
+    return tab > plain ? tab : plain;
 }
 
 }  // namespace
