@@ -32,6 +32,10 @@ constexpr uint32_t kP2x29[9] = {0x10f9fa8eu, 0x0208c16du, 0x18e5469eu, 0x05aa45a
                                 0x05b68181u, 0x014dc282u, 0x1cb84c68u, 0x0060c89cu};
 constexpr uint32_t kP4x29[9] = {0x01f3f51cu, 0x041182dbu, 0x11ca8d3cu, 0x0b548b43u, 0x161765e0u,
                                 0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
+constexpr uint32_t kP6x29[9] = {0x12edefaau, 0x061a4448u, 0x0aafd3dau, 0x10fed0e5u, 0x012318d0u,
+                                0x11238484u, 0x03e94786u, 0x1628e538u, 0x012259d6u};
+constexpr uint32_t kP8x29[9] = {0x03e7ea38u, 0x082305b6u, 0x03951a78u, 0x16a91687u, 0x0c2ecbc0u,
+                                0x16da0605u, 0x05370a08u, 0x12e131a0u, 0x01832273u};
 constexpr uint32_t kPInv29 = 0x04866389u;
 constexpr uint32_t kPInv5 = 9u;
 constexpr uint32_t kOne29[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u,
@@ -209,9 +213,11 @@ __device__ __forceinline__ F29 f29_sub(const F29 &a, const F29 &b) {
 
 __device__ __forceinline__ F29 f29_dbl(const F29 &a) { return f29_add(a, a); }
 
-// Unreduced forms for values that only feed products (any operand < 8p keeps
-// a * b + c * d below 2^261 p) or one final reduction:
-// a + b, and a - b + 2p (b < 2p + a) or a - b + 4p (b < 4p + a)
+// Unreduced forms for values that only feed products or one final
+// reduction: a + b, and a - b + K p for K = 2, 4, 8 (b < K p + a).  A product
+// stays < 2p while a * b < 2^261 p (p < 2^253.6: e.g. 10p x 10p, 32p x 2p),
+// and a * b + c * d likewise; normalised limbs (< 2^29) keep every product
+// column below 2^63 whatever the value.
 __device__ __forceinline__ F29 f29_add_nr(const F29 &a, const F29 &b) {
     F29 r;
     uint32_t c = 0;
@@ -223,13 +229,27 @@ __device__ __forceinline__ F29 f29_add_nr(const F29 &a, const F29 &b) {
     }
     return r;
 }
-template <int K>  // K = 2 or 4: adds K p
+template <int K>  // K = 2, 4 or 8: adds K p
 __device__ __forceinline__ F29 f29_sub_nr(const F29 &a, const F29 &b) {
+    static_assert(K == 2 || K == 4 || K == 8, "K p constants exist for K = 2, 4, 8");
     F29 r;
     int32_t br = 0;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-        int32_t s = (int32_t)a.l[i] - (int32_t)b.l[i] + (int32_t)(K == 4 ? kP4x29[i] : kP2x29[i]) + br;
+        const uint32_t kp = K == 8 ? kP8x29[i] : K == 4 ? kP4x29[i] : kP2x29[i];
+        int32_t s = (int32_t)a.l[i] - (int32_t)b.l[i] + (int32_t)kp + br;
+        r.l[i] = (uint32_t)s & kM29;
+        br = s >> 29;
+    }
+    return r;
+}
+// a - b - 2c + 6p in one carry pass, for a, b, c < 2p: in (0, 8p)
+__device__ __forceinline__ F29 f29_sub_sub2_nr(const F29 &a, const F29 &b, const F29 &c) {
+    F29 r;
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        int32_t s = (int32_t)a.l[i] - (int32_t)b.l[i] - (int32_t)(c.l[i] << 1) + (int32_t)kP6x29[i] + br;
         r.l[i] = (uint32_t)s & kM29;
         br = s >> 29;
     }
@@ -353,6 +373,10 @@ __device__ __forceinline__ void x29_clear(X29 &p, bool &empty) {
 // acc += (x2, y2) (madd-2008-s), x2, y2 < 32p used only as multiplication
 // operands.  An empty acc takes the point itself (U2 = x2 * 1, S2 = y2 * 1 are
 // its reduced coordinates), so lanes starting a bucket run do no extra work.
+// The accumulator's x stays unreduced (< 8p, normalised limbs) between
+// additions: it only enters products and P = U2 - X1 + 8p (< 10p); X3 =
+// R^2 - PPP - 2Q + 6p comes out of one carry pass.  Store it with
+// x29_store_acc, which reduces x below 2p as every other X29 consumer expects.
 __device__ __forceinline__ void x29_acc_aff(X29 &p, bool &empty, const F29 &x2, const F29 &y2) {
     F29 U2 = f29_mul(x2, p.zz);
     F29 S2 = f29_mul(y2, p.zzz);
@@ -362,9 +386,10 @@ __device__ __forceinline__ void x29_acc_aff(X29 &p, bool &empty, const F29 &x2, 
         empty = false;
         return;
     }
-    F29 P = f29_sub(U2, p.x);
-    F29 R = f29_sub_nr<2>(S2, p.y);  // < 4p: only multiplied
-    if (f29_is_zero(P)) {
+    const F29 P = f29_sub_nr<8>(U2, p.x);  // < 10p: only multiplied
+    const F29 R = f29_sub_nr<2>(S2, p.y);  // < 4p: only multiplied
+    const F29 PP = f29_sqr(P);             // < 2p; zero exactly when P is
+    if (f29_is_zero(PP)) {
         if (f29_is_zero(f29_sub2p_if(R))) {
             p = x29_dbl(p);  // the point equals the accumulator
         } else {
@@ -372,15 +397,13 @@ __device__ __forceinline__ void x29_acc_aff(X29 &p, bool &empty, const F29 &x2, 
         }
         return;
     }
-    F29 PP = f29_sqr(P);
-    F29 PPP = f29_mul(P, PP);
-    F29 Q = f29_mul(p.x, PP);
-    // X3 = R^2 - PPP - 2Q: (R^2 - PPP + 2p) - 2Q + 4p < 8p, reduced once
-    F29 X3 = f29_reduce8(f29_sub_nr<4>(f29_sub_nr<2>(f29_sqr(R), PPP), f29_add_nr(Q, Q)));
-    // R (Q - X3) - Y1 PPP as R (Q - X3 + 2p) + (2p - Y1) PPP: one reduction,
-    // operands < 4p, < 4p, <= 2p, < 2p
+    const F29 PPP = f29_mul(P, PP);
+    const F29 Q = f29_mul(p.x, PP);
+    const F29 X3 = f29_sub_sub2_nr(f29_sqr(R), PPP, Q);  // < 8p
+    // R (Q - X3) - Y1 PPP as R (Q - X3 + 8p) + (2p - Y1) PPP: one reduction,
+    // operands < 4p, < 10p, <= 2p, < 2p (sum of products < 44 p^2)
     const F29 zero = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
-    F29 Y3 = f29_mul2(R, f29_sub_nr<2>(Q, X3), f29_sub_nr<2>(zero, p.y), PPP);
+    F29 Y3 = f29_mul2(R, f29_sub_nr<8>(Q, X3), f29_sub_nr<2>(zero, p.y), PPP);
     p.zz = f29_mul(p.zz, PP);
     p.zzz = f29_mul(p.zzz, PPP);
     p.x = X3;
@@ -437,6 +460,11 @@ __device__ __forceinline__ void x29_store(X29S *d, const X29 &p, bool empty) {
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) d->q[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+// x29_store for an accumulator of x29_acc_aff (x < 8p): x reduced first
+__device__ __forceinline__ void x29_store_acc(X29S *d, X29 p, bool empty) {
+    p.x = f29_reduce8(p.x);
+    x29_store(d, p, empty);
 }
 __device__ __forceinline__ X29 x29_load(const X29S *s, bool &empty) {
     uint32_t w[36];

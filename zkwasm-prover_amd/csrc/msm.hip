@@ -712,13 +712,13 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
                 // the run of bucket b ends at this step (stored in the 29-bit
                 // form: the fixup and the reduction continue in it)
                 if (first && before) {
-                    x29_store(&phead[t], acc, empty);
+                    x29_store_acc(&phead[t], acc, empty);
                     head_bucket = b;
                 } else if (bend > e) {
-                    x29_store(&ptail[t], acc, empty);
+                    x29_store_acc(&ptail[t], acc, empty);
                     tail_bucket = b;
                 } else {
-                    x29_store(&buckets[b], acc, empty);
+                    x29_store_acc(&buckets[b], acc, empty);
                 }
                 first = false;
                 x29_clear(acc, empty);
