@@ -74,9 +74,23 @@ __device__ __forceinline__ F29 f29_mul_chain(const F29 &a, const F29 &b) {
     return r;
 }
 
+// The modulus of a 29-bit product: Fp (every curve kernel) or Fr (the NTT's
+// twiddle products).  Both are < 2^254, so every bound stated for p holds for r.
+struct Fp29Mod {
+    static constexpr uint32_t P[9] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u, 0x1585d978u,
+                                      0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+    static constexpr uint32_t PINV = kPInv29;
+};
+struct Fr29Mod {
+    static constexpr uint32_t P[9] = {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u,
+                                      0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+    static constexpr uint32_t PINV = 0x0fffffffu;  // -r^-1 mod 2^29
+};
+
 // the same product with each column's a*b terms (two chains) and m*p terms
 // summed independently: shorter dependency chains of multiply-adds (+6%
 // measured); this is the one the kernels use
+template <class Q = Fp29Mod>
 __device__ __forceinline__ F29 f29_mul(const F29 &a, const F29 &b) {
     uint32_t m[9];
     F29 r;
@@ -95,11 +109,11 @@ __device__ __forceinline__ F29 f29_mul(const F29 &a, const F29 &b) {
         uint64_t u = 0;
 #pragma unroll
         for (int i = lo; i <= hi; i++)
-            if (i < k) u += (uint64_t)m[i] * kP29[k - i];
+            if (i < k) u += (uint64_t)m[i] * Q::P[k - i];
         acc += t0 + t1 + u;
         if (k < 9) {
-            m[k] = ((uint32_t)acc * kPInv29) & kM29;
-            acc += (uint64_t)m[k] * kP29[0];
+            m[k] = ((uint32_t)acc * Q::PINV) & kM29;
+            acc += (uint64_t)m[k] * Q::P[0];
         } else {
             r.l[k - 9] = (uint32_t)acc & kM29;
         }
@@ -281,7 +295,8 @@ __device__ __forceinline__ bool f29_is_zero(const F29 &a) {
 
 // 32-bit Montgomery (x 2^256, < p) -> x 2^261 (the value shifted left by 5;
 // < 32p, not reduced)
-__device__ __forceinline__ F29 f29_from32(const Fp &x) {
+template <class P>
+__device__ __forceinline__ F29 f29_from32(const Fe<P> &x) {
     F29 r;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
@@ -335,6 +350,41 @@ __device__ __forceinline__ Fp f29_to32(const F29 &a) {
         r.v[w] = x;
     }
     return r;
+}
+
+// 32-bit limbs (any value < 2^256) as nine 29-bit limbs of the same value
+__device__ __forceinline__ F29 f29_repack(const uint32_t x[8]) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int bit = 29 * i, w = bit >> 5, o = bit & 31;
+        uint32_t v = x[w] >> o;
+        if (o > 3 && w + 1 < 8) v |= x[w + 1] << (32 - o);
+        r.l[i] = v & kM29;
+    }
+    return r;
+}
+// nine normalised 29-bit limbs of a value < 2^256 as eight 32-bit words
+__device__ __forceinline__ void f29_unpack(const F29 &a, uint32_t x[8]) {
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+        const int bit = 32 * w, i = bit / 29, o = bit % 29;
+        uint32_t v = a.l[i] >> o;
+        if (i + 1 < 9) v |= a.l[i + 1] << (29 - o);
+        if (o > 26 && i + 2 < 9) v |= a.l[i + 2] << (58 - o);
+        x[w] = v;
+    }
+}
+// a * w in Fr's 32-bit Montgomery form for a < r and w29 = f29_from32(w)
+// (w 2^261 mod r up to 32 r): the 29-bit product a w29 2^-261 is a w 2^256
+// mod r itself, < 2r, so repacking both ways and one conditional
+// subtraction replace the 32-bit product (about 270 VALU instructions
+// against 318)
+__device__ __forceinline__ Fr fr_mul_w29(const Fr &a, const F29 &w29) {
+    const F29 t = f29_mul<Fr29Mod>(f29_repack(a.v), w29);
+    Fr r;
+    f29_unpack(t, r.v);
+    return fe_reduce_once<FrParams>(r);
 }
 
 }  // namespace zk

@@ -21,6 +21,7 @@
 #include <atomic>
 #include <vector>
 
+#include "ff29.h"
 #include "guard.h"
 #include "zkcore.h"
 #include "zkinternal.h"
@@ -153,11 +154,13 @@ __device__ __forceinline__ void swap_lane_reg(Fr &ra, Fr &rb, uint32_t lane) {
     }
 }
 
-template <bool MUL>
-__device__ __forceinline__ void butterfly(Fr &a, Fr &c, const Fr &w) {
+// w: a twiddle in the 29-bit form f29_from32 gives (register rounds), unused
+// without MUL
+template <bool MUL, class W>
+__device__ __forceinline__ void butterfly(Fr &a, Fr &c, const W &w) {
     const Fr s = a + c;
     Fr d = a - c;
-    if (MUL) d = d * w;
+    if constexpr (MUL) d = fr_mul_w29(d, w);
     a = s;
     c = d;
 }
@@ -165,14 +168,14 @@ __device__ __forceinline__ void butterfly(Fr &a, Fr &c, const Fr &w) {
 // Rounds RND..DEG-1 (RND >= 2), one template instance per round so that
 // every register index and lane distance is a compile-time constant.
 template <int DEG, int RND>
-__device__ __forceinline__ void dft_rounds(Fr &r0, Fr &r1, Fr &r2, Fr &r3, const Fr *tw, uint32_t lane) {
+__device__ __forceinline__ void dft_rounds(Fr &r0, Fr &r1, Fr &r2, Fr &r3, const F29 *tw, uint32_t lane) {
     if constexpr (RND < DEG) {
         constexpr int b = DEG - 1 - RND;  // i-bit of this round, in lane slot b
         if constexpr ((RND & 1) == 0) {   // swap with rho0: pairs (0,1), (2,3)
             swap_lane_reg<b>(r0, r1, lane);
             swap_lane_reg<b>(r2, r3, lane);
             if constexpr (b > 0) {
-                const Fr w = tw[(lane & ((1u << b) - 1)) << RND];
+                const F29 w = tw[(lane & ((1u << b) - 1)) << RND];
                 butterfly<true>(r0, r1, w);
                 butterfly<true>(r2, r3, w);
             } else {  // last round: twiddle 1
@@ -183,7 +186,7 @@ __device__ __forceinline__ void dft_rounds(Fr &r0, Fr &r1, Fr &r2, Fr &r3, const
             swap_lane_reg<b>(r0, r2, lane);
             swap_lane_reg<b>(r1, r3, lane);
             if constexpr (b > 0) {
-                const Fr w = tw[(lane & ((1u << b) - 1)) << RND];
+                const F29 w = tw[(lane & ((1u << b) - 1)) << RND];
                 butterfly<true>(r0, r2, w);
                 butterfly<true>(r1, r3, w);
             } else {
@@ -196,26 +199,27 @@ __device__ __forceinline__ void dft_rounds(Fr &r0, Fr &r1, Fr &r2, Fr &r3, const
 }
 
 // The DIF of every group of 2^DEG in the wave's 256 elements (s: the wave's
-// LDS region, e-indexed; tw: w_D^j for j < D/2).  Results go back to s in
+// LDS region, e-indexed; tw: w_D^j for j < D/2 in the 29-bit form of
+// f29_from32, each twiddle product a 29-bit one).  Results go back to s in
 // the layout the radix-2 LDS rounds leave (U[bitrev(i)] at i).  quarter:
 // (wave-uniform) every element with i >= D/4 is zero, as in the first pass
 // of a coset extension by 4 -- those are r1..r3 (rho != 0), so rounds 0 and 1
 // reduce to three products and no additions.
 template <int DEG>
-__device__ __forceinline__ void wave_dft_regs(Fr *s, const Fr *tw, uint32_t lane, bool quarter) {
+__device__ __forceinline__ void wave_dft_regs(Fr *s, const F29 *tw, uint32_t lane, bool quarter) {
     static_assert(DEG >= 2 && DEG <= 8, "register DFT covers 2 <= deg <= 8");
     const uint32_t lo = lane & ((1u << (DEG - 2)) - 1);
     Fr r0 = s[slot_elem<DEG, false>(lane, 0)], r1, r2, r3;
     if (quarter) {
         // round 0: (r0, 0) -> (r0, r0 w); round 1: (x, 0) -> (x, x w')
-        r2 = r0 * tw[lo];
+        r2 = fr_mul_w29(r0, tw[lo]);
         if constexpr (DEG == 2) {
             r1 = r0;
             r3 = r2;
         } else {
-            const Fr w = tw[lo << 1];
-            r1 = r0 * w;
-            r3 = r2 * w;
+            const F29 w = tw[lo << 1];
+            r1 = fr_mul_w29(r0, w);
+            r3 = fr_mul_w29(r2, w);
         }
     } else {
         r1 = s[slot_elem<DEG, false>(lane, 1)];
@@ -229,7 +233,7 @@ __device__ __forceinline__ void wave_dft_regs(Fr *s, const Fr *tw, uint32_t lane
             butterfly<false>(r0, r1, r0);
             butterfly<false>(r2, r3, r0);
         } else {
-            const Fr w = tw[lo << 1];
+            const F29 w = tw[lo << 1];
             butterfly<true>(r0, r1, w);
             butterfly<true>(r2, r3, w);
         }
@@ -272,9 +276,17 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
     const uint32_t p = 1u << log_p;
     const uint32_t index0 = blk << log_g;
     Fr *s = lds;
-    Fr *tw = lds + tile;
+    Fr *tw = lds + tile;                              // radix-2 LDS rounds: 32-bit twiddles
+    F29 *tw29 = reinterpret_cast<F29 *>(lds + tile);  // register rounds: 29-bit twiddles
+    const bool regs = a.reg_rounds && use_register_rounds(tile, deg);
 
-    for (uint32_t j = tid; j < D / 2; j += bs) tw[j] = fe_load<FrParams>(&omegas[j << log_t]);
+    for (uint32_t j = tid; j < D / 2; j += bs) {
+        const Fr w = fe_load<FrParams>(&omegas[j << log_t]);
+        if (regs)
+            tw29[j] = f29_from32(w);
+        else
+            tw[j] = w;
+    }
 
     const uint32_t base_shift = log_t - log_p;
     for (uint32_t e = tid; e < tile; e += bs) {
@@ -300,7 +312,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
     }
     __syncthreads();
 
-    if (a.reg_rounds && use_register_rounds(tile, deg)) {
+    if (regs) {
         // bs == kThreads: four waves, each owning 256 elements of the tile
         Fr *ws = s + ((tid >> 6) << 8);
         const uint32_t lane = tid & 63;
@@ -308,13 +320,13 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
         // positions index + i * t >= (D/4) * t = n / 4 >= nz, all zero
         const bool quarter = nz && log_p == 0 && ((size_t)nz << 2) <= ((size_t)1 << log_n);
         switch (deg) {
-            case 2: wave_dft_regs<2>(ws, tw, lane, quarter); break;
-            case 3: wave_dft_regs<3>(ws, tw, lane, quarter); break;
-            case 4: wave_dft_regs<4>(ws, tw, lane, quarter); break;
-            case 5: wave_dft_regs<5>(ws, tw, lane, quarter); break;
-            case 6: wave_dft_regs<6>(ws, tw, lane, quarter); break;
-            case 7: wave_dft_regs<7>(ws, tw, lane, quarter); break;
-            default: wave_dft_regs<8>(ws, tw, lane, quarter); break;
+            case 2: wave_dft_regs<2>(ws, tw29, lane, quarter); break;
+            case 3: wave_dft_regs<3>(ws, tw29, lane, quarter); break;
+            case 4: wave_dft_regs<4>(ws, tw29, lane, quarter); break;
+            case 5: wave_dft_regs<5>(ws, tw29, lane, quarter); break;
+            case 6: wave_dft_regs<6>(ws, tw29, lane, quarter); break;
+            case 7: wave_dft_regs<7>(ws, tw29, lane, quarter); break;
+            default: wave_dft_regs<8>(ws, tw29, lane, quarter); break;
         }
         __syncthreads();
     } else {
@@ -487,7 +499,8 @@ hipError_t ntt_run_batch(const NttSpec *specs, uint32_t count, hipStream_t strea
             j.deg = (uint8_t)deg;
             j.log_g = (uint8_t)log_g;
             blocks += n / tile_elems;
-            size_t l = ((size_t)tile_elems + (1u << deg) / 2) * sizeof(Fr);
+            // the tile, then D/2 twiddles (29-bit form: 36 B each)
+            size_t l = (size_t)tile_elems * sizeof(Fr) + (size_t)((1u << deg) / 2) * sizeof(F29);
             if (l > lds) lds = l;
             logp[m] += deg;
             if (args.count == kJobsPerLaunch) ZK_CHECK(flush());
