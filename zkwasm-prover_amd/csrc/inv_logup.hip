@@ -457,6 +457,32 @@ hipError_t field_scan(Fr *z, uint32_t n, bool mul, const Fr *carry, Fr *last_z, 
 
 }  // namespace zk
 
+namespace zk {
+
+// out = (x + beta)^-1 element-wise (zeros stay zero); temp: n Fr
+hipError_t batch_invert_shifted(void *out, const void *x, const void *beta, void *temp, int n, hipStream_t stream) {
+    if (n <= 0 || !out || !x || !beta || !temp) return hipErrorInvalidValue;
+    return batch_invert_ex((const Fr *)x, (const Fr *)beta, (Fr *)out, OUT_SET, (uint32_t)n, (Fr *)temp, stream);
+}
+
+// eval_logup_z_compose after its table inversion: z from the logup sum
+// `input` (clobbered as scan scratch), the inverted table (t + beta)^-1 and
+// the multiplicities; the same steps and bytes as the drop-in
+hipError_t logup_z_inverted(void *z, void *input, const void *table_inv, const void *multiplicity, void *last_z,
+                            int last_z_index, int n, hipStream_t stream) {
+    if (n <= 0 || !z || !input || !table_inv || !multiplicity || !last_z || last_z_index < 0 || last_z_index >= n)
+        return hipErrorInvalidValue;
+    uint32_t blocks = ((uint32_t)n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    k_logup_step2<<<blocks, 256, 0, stream>>>((Fr *)z, (const Fr *)input, (const Fr *)table_inv,
+                                               (const Fr *)multiplicity, (uint32_t)n);
+    ZK_CHECK(hipGetLastError());
+    return field_scan((Fr *)z, (uint32_t)n, false, (const Fr *)last_z, (Fr *)last_z, last_z_index, (Fr *)input,
+                      stream);
+}
+
+}  // namespace zk
+
 extern "C" {
 
 // In-place batch inversion; temp holds n Fr.

@@ -1519,12 +1519,14 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
 
         // ---- 3. grand products ----------------------------------------------------
         void *beta = j.cf(J_BETA), *gamma = j.cf(J_GAMMA);
+        // every lookup reads the same table: (t + beta)^-1 once per proof
+        // instead of a table copy and its inversion per eval_logup_z_compose
+        if (s.L > 0) PCHECK(zk::batch_invert_shifted(j.w_tab, pk.t_lag, beta, j.w_temp, (int)s.n, st));
         for (int l = 0; l < s.L; l++) {
             PCHECK(logup_sum_input_inv(j.w_sum, j.adv_lag[s.col_lookup + l], j.w_temp, beta, 0, (int)s.n, st));
-            PCHECK(hipMemcpyAsync(j.w_tab, pk.t_lag, (size_t)s.n * FR, hipMemcpyDeviceToDevice, st));
             PCHECK(zk::fill_zero(j.cf(J_LASTZ), FR, st));
-            PCHECK(eval_logup_z_compose(j.zl_lag[l], j.w_sum, j.w_tab, j.m_lag[l], beta, j.cf(J_LASTZ), (int)s.u,
-                                        (int)s.n, st));
+            PCHECK(zk::logup_z_inverted(j.zl_lag[l], j.w_sum, j.w_tab, j.m_lag[l], j.cf(J_LASTZ), (int)s.u, (int)s.n,
+                                        st));
         }
         // device pointer arrays: [values | sigmas | zs], then the gate term list,
         // its rotations, the permutation sets, the permutation columns' cosets and

@@ -28,6 +28,12 @@ hipError_t wit_sigma_rows(void *const *sig_dev, const uint32_t *target, const vo
 hipError_t wit_fill_value(void *col, const void *v, uint32_t n, hipStream_t s);
 hipError_t wit_any_nonzero(const void *col, uint64_t lo, uint64_t hi, uint32_t *flag, hipStream_t s);
 
+// out = (x + beta)^-1 element-wise, temp: n Fr; and eval_logup_z_compose's
+// steps after its in-place table inversion, given that inverse (inv_logup.hip)
+hipError_t batch_invert_shifted(void *out, const void *x, const void *beta, void *temp, int n, hipStream_t stream);
+hipError_t logup_z_inverted(void *z, void *input, const void *table_inv, const void *multiplicity, void *last_z,
+                            int last_z_index, int n, hipStream_t stream);
+
 // NTTs whose first pass reads src[m] (2^log_n elements, left unchanged)
 // instead of buf[m]: the values of copying src into buf and running ntt with
 // tmp.  swapped[m] set: the result is in tmp[m], else in buf[m]. (ntt.hip)
