@@ -101,6 +101,7 @@ struct Arena {
 struct Shape;
 int ptr_slots(const Shape &s);
 constexpr int kPtrSlots = 1024;  // device pointer area of a job (8 KiB)
+constexpr int kMaxGateCols = 32;  // coefficient gates zk::gate_eval_h takes
 
 struct Shape {
     int k, ek, A, F, I, L, P, PA, S, Sp, FG, tlog, slog, blind;
@@ -146,6 +147,7 @@ bool make_shape(const zkp_circuit *c, Shape &s) {
     s.A = c->advice;
     if (s.A < s.col_fixed_gate + s.FG) return false;
     if (ptr_slots(s) > kPtrSlots) return false;  // device pointer area of a job
+    if (s.FG > kMaxGateCols) return false;        // the gate kernel's argument block
     return true;
 }
 
@@ -558,22 +560,14 @@ struct Job {
     zk::CommitFn commit_fn = nullptr;  // batched commitments through a hook (the proof server's)
     void *commit_ctx = nullptr;
     void *cf(int c) const { return cst + (size_t)c * FR; }
-    int j_ypow(int i) const { return J_COUNT_FIXED + pk->sh.P + i; }                  // y^i, i <= FG
-    int j_mypow(int i) const { return J_COUNT_FIXED + pk->sh.P + pk->sh.FG + 1 + i; }  // -y^i
 };
 
 // how many polynomials get an extended coset in phase 4
 int ext_count(const Shape &s) { return 4 + 3 * s.L + s.P + s.Sp + 3 * s.S + s.FG; }
-// gate term list of field_op_batch_mul_sum: pointers (main gate 13 entries,
-// each coefficient gate 9) and rotations (one per column entry)
-int gate_list_len(const Shape &s) { return 13 + 9 * s.FG; }
-int gate_rot_len(const Shape &s) { return 7 + 5 * s.FG; }
-// device pointer area: [values | sigmas | zs] (2P + Sp), gate list, gate
-// rotations (as pointer-size slots), permutation sets (Sp), permutation
-// values and sigma cosets (2P), shuffle inputs and tables (2S)
-int ptr_slots(const Shape &s) {
-    return 2 * s.P + s.Sp + gate_list_len(s) + (gate_rot_len(s) + 1) / 2 + s.Sp + 2 * s.P + 2 * s.S + 8;
-}
+// device pointer area: [values | sigmas | zs] (2P + Sp), permutation sets
+// (Sp), permutation values and sigma cosets (2P), shuffle inputs and tables
+// (2S)
+int ptr_slots(const Shape &s) { return 2 * s.P + s.Sp + s.Sp + 2 * s.P + 2 * s.S + 8; }
 
 // scratch of one batched-MSM call: plain points or the key's tables
 size_t msm_group_bytes(const Shape &s) {
@@ -625,7 +619,7 @@ void job_layout(Job &j, Arena &a) {
     j.r_acc = a.fr(s.N);
     j.aux0 = a.fr(s.N);
     j.aux1 = a.fr(s.N);
-    j.cst = a.take((size_t)(J_COUNT_FIXED + s.P + 2 * (s.FG + 1) + 8) * FR);
+    j.cst = a.take((size_t)(J_COUNT_FIXED + s.P + 8) * FR);
     j.ptrs = (void **)a.take((size_t)kPtrSlots * 8);
     j.msm_bytes = msm_group_bytes(s);
     j.msm_scratch = a.take(j.msm_bytes);
@@ -1532,9 +1526,7 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         // its rotations, the permutation sets, the permutation columns' cosets and
         // the shuffle inputs' / tables' cosets; shuffle z inputs reuse the start
         void **pa = j.ptrs;
-        void **pa_gate = pa + 2 * s.P + s.Sp;
-        int *pa_rots = (int *)(pa_gate + gate_list_len(s));
-        void **pa_sets = pa_gate + gate_list_len(s) + (gate_rot_len(s) + 1) / 2;
+        void **pa_sets = pa + 2 * s.P + s.Sp;
         void **pa_pv = pa_sets + s.Sp, **pa_ps = pa_pv + s.P;
         void **pa_sh = pa_ps + s.P;
         // permutation columns: instance first, then the advice ones
@@ -1578,17 +1570,6 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
             const zkh::Fr yc = tr.challenge();  // y (src/lib.rs:988)
             PCHECK(put_challenges(j, J_Y, &yc, 1, st));
             memcpy(inf.challenges[2], &yc, FR);
-            // the gates' y powers (the gates fold as h = h y + gate)
-            {
-                std::vector<zkh::Fr> yp(2 * (s.FG + 1));
-                zkh::Fr acc = zkh::fr_one();
-                for (int i = 0; i <= s.FG; i++) {
-                    yp[i] = acc;
-                    yp[s.FG + 1 + i] = zkh::fr_neg(acc);
-                    acc = zkh::fr_mul(acc, yc);
-                }
-                PCHECK(put_challenges(j, j.j_ypow(0), yp.data(), (int)yp.size(), st));
-            }
             // closure of every grand product at row u: logup 0, permutation (last
             // set) 1, shuffle 1
             int bad = 0;  // bit 0: a logup sum, bit 1: the permutation, bit 2: a shuffle
@@ -1634,25 +1615,13 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
         void *y = j.cf(J_Y);
         const int rot1 = 1 << kExtBits;  // w X on the coset
         const int N = (int)s.N;
+        // the gates, folded in y: y^FG q (a b + c - d) + sum_j y^(FG-1-j) q (f_j a - e_j)
+        std::vector<const void *> gate_f(s.FG), gate_e(s.FG);
+        for (int g = 0; g < s.FG; g++) {
+            gate_f[g] = pk.fix_ext[g];
+            gate_e[g] = j.ext[E_EG + g];
+        }
         if (!j.p4_staged) {
-            // gates, folded in y: y^FG q (a b + c - d) + sum_j y^(FG-1-j) q (f_j a - e_j)
-            std::vector<void *> hv;
-            auto term = [&](void *coef, std::initializer_list<void *> cols) {
-                hv.push_back(coef);
-                for (void *c : cols) hv.push_back(c);
-                hv.push_back(nullptr);
-            };
-            term(j.cf(j.j_ypow(s.FG)), {pk.q_ext, j.ext[E_GATE], j.ext[E_GATE + 1]});
-            term(j.cf(j.j_ypow(s.FG)), {pk.q_ext, j.ext[E_GATE + 2]});
-            term(j.cf(j.j_mypow(s.FG)), {pk.q_ext, j.ext[E_GATE + 3]});
-            for (int g = 0; g < s.FG; g++) {
-                term(j.cf(j.j_ypow(s.FG - 1 - g)), {pk.q_ext, pk.fix_ext[g], j.ext[E_GATE]});
-                term(j.cf(j.j_mypow(s.FG - 1 - g)), {pk.q_ext, j.ext[E_EG + g]});
-            }
-            if ((int)hv.size() != gate_list_len(s)) return hipErrorUnknown;
-            std::vector<int> rots(gate_rot_len(s), 0);
-            PCHECK(hipMemcpyAsync(pa_gate, hv.data(), hv.size() * 8, hipMemcpyHostToDevice, st));
-            PCHECK(hipMemcpyAsync(pa_rots, rots.data(), rots.size() * 4, hipMemcpyHostToDevice, st));
             std::vector<void *> sets(s.Sp + 2 * s.P + 2 * s.S);
             for (int t = 0; t < s.Sp; t++) sets[t] = j.ext[E_ZP + t];
             for (int c = 0; c < s.P; c++) {
@@ -1681,8 +1650,9 @@ hipError_t zkp_prove(void *job, unsigned long long seed, int flags, void *commit
             for (auto p : j.zs_coef) c.push_back(p);
             for (int g = 0; g < s.FG; g++) c.push_back(j.adv_coef[s.col_fixed_gate + g]);
             PCHECK(extend_all(j, c, 0, st));
-            PCHECK(zk::fill_zero(j.h, (size_t)s.N * FR, st));
-            PCHECK(field_op_batch_mul_sum(j.h, pa_gate, pa_rots, gate_list_len(s), N, st));
+            // h starts as the gate sum (written, not accumulated)
+            PCHECK(zk::gate_eval_h(j.h, pk.q_ext, j.ext[E_GATE], j.ext[E_GATE + 1], j.ext[E_GATE + 2],
+                                   j.ext[E_GATE + 3], gate_f.data(), gate_e.data(), s.FG, y, N, st));
             // permutation: p1, p2, then per set the fused product term (sigma
             // and X resident as cosets)
             if (s.Sp > 0) {

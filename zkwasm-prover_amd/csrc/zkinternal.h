@@ -28,6 +28,11 @@ hipError_t wit_sigma_rows(void *const *sig_dev, const uint32_t *target, const vo
 hipError_t wit_fill_value(void *col, const void *v, uint32_t n, hipStream_t s);
 hipError_t wit_any_nonzero(const void *col, uint64_t lo, uint64_t hi, uint32_t *flag, hipStream_t s);
 
+// h = q (y^G (a b + c - d) + sum_g y^(G-1-g) (f_g a - e_g)) on the extended
+// coset: the benchmark circuit's gates in one pass, G <= 32 (evalh.hip)
+hipError_t gate_eval_h(void *h, const void *q, const void *a, const void *b, const void *c, const void *d,
+                       const void *const *f, const void *const *e, int ng, const void *y, int n, hipStream_t stream);
+
 // out = (x + beta)^-1 element-wise, temp: n Fr; and eval_logup_z_compose's
 // steps after its in-place table inversion, given that inverse (inv_logup.hip)
 hipError_t batch_invert_shifted(void *out, const void *x, const void *beta, void *temp, int n, hipStream_t stream);
