@@ -41,6 +41,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <utility>
 #include <vector>
 
 #include "guard.h"
@@ -629,6 +630,19 @@ void job_layout(Job &j, Arena &a) {
     j.open_cst = a.fr(kOpenCst);
 }
 
+// Commitment groups (first polynomial, count) of a batch of `count`
+// polynomials: kMsmGroup each, except while the inputs still arrive from host
+// memory, where the groups ramp up as 4, 8, 16, ...: the first MSMs start
+// after 4 columns' upload instead of 16 (≈ 9 instead of 37 ms at k=22), and
+// each group's upload overlaps the previous group's MSMs.
+std::vector<std::pair<size_t, int>> commit_groups(size_t count, bool uploaded) {
+    std::vector<std::pair<size_t, int>> g;
+    size_t size = uploaded ? 4 : (size_t)kMsmGroup;
+    for (size_t first = 0; first < count; first += size, size = size * 2 < (size_t)kMsmGroup ? size * 2 : kMsmGroup)
+        g.emplace_back(first, (int)(count - first < size ? count - first : size));
+    return g;
+}
+
 // commitments of polys (device Lagrange or coefficient values of n) against
 // bases; out: affine points (host, 64 B each)
 hipError_t commit(Job &j, const void *bases, const std::vector<const void *> &polys, uint8_t *out, bool batched,
@@ -641,9 +655,11 @@ hipError_t commit(Job &j, const void *bases, const std::vector<const void *> &po
         src = bases == j.pk->g_lag ? j.pk->g_lag_tab : j.pk->g_tab;
         tc = j.pk->tab_c;
     }
-    for (size_t first = 0; first < polys.size(); first += kMsmGroup) {
-        int cnt = (int)(polys.size() - first < (size_t)kMsmGroup ? polys.size() - first : kMsmGroup);
-        if (uploaded) PCHECK(hipStreamWaitEvent(st, j.upload_ev[first / kMsmGroup], 0));
+    const std::vector<std::pair<size_t, int>> groups = commit_groups(polys.size(), uploaded);
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const size_t first = groups[gi].first;
+        const int cnt = groups[gi].second;
+        if (uploaded) PCHECK(hipStreamWaitEvent(st, j.upload_ev[gi], 0));
         if (batched && j.commit_fn) {
             std::vector<zk_msm_item> it(cnt);
             for (int m = 0; m < cnt; m++) it[m] = zk_msm_item{src, polys[first + m], (int)s.n, tc};
@@ -685,7 +701,8 @@ hipError_t commit(Job &j, const void *bases, const std::vector<const void *> &po
 hipError_t upload_witness(Job &j, size_t ncols, hipStream_t st) {
     const Shape &s = j.pk->sh;
     if (!j.upload) PCHECK(hipStreamCreateWithFlags(&j.upload, hipStreamNonBlocking));
-    const size_t groups = (ncols + kMsmGroup - 1) / kMsmGroup;
+    const std::vector<std::pair<size_t, int>> cg = commit_groups(ncols, true);
+    const size_t groups = cg.size();
     while (j.upload_ev.size() < groups + 1) {
         hipEvent_t e;
         PCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -693,12 +710,14 @@ hipError_t upload_witness(Job &j, size_t ncols, hipStream_t st) {
     }
     PCHECK(hipEventRecord(j.upload_ev[groups], st));
     PCHECK(hipStreamWaitEvent(j.upload, j.upload_ev[groups], 0));
-    for (size_t c = 0; c < ncols; c++) {
-        const bool adv = c < (size_t)s.A;
-        void *dst = adv ? j.adv_lag[c] : j.inst_lag[c - s.A];
-        const void *src = adv ? j.host_adv[c] : j.host_inst[c - s.A];
-        PCHECK(hipMemcpyAsync(dst, src, (size_t)s.n * FR, hipMemcpyHostToDevice, j.upload));
-        if ((c + 1) % kMsmGroup == 0 || c + 1 == ncols) PCHECK(hipEventRecord(j.upload_ev[c / kMsmGroup], j.upload));
+    for (size_t gi = 0; gi < groups; gi++) {
+        for (size_t c = cg[gi].first; c < cg[gi].first + (size_t)cg[gi].second; c++) {
+            const bool adv = c < (size_t)s.A;
+            void *dst = adv ? j.adv_lag[c] : j.inst_lag[c - s.A];
+            const void *src = adv ? j.host_adv[c] : j.host_inst[c - s.A];
+            PCHECK(hipMemcpyAsync(dst, src, (size_t)s.n * FR, hipMemcpyHostToDevice, j.upload));
+        }
+        PCHECK(hipEventRecord(j.upload_ev[gi], j.upload));
     }
     return hipSuccess;
 }
