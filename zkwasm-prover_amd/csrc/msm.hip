@@ -1327,7 +1327,30 @@ void host_tail_one(const uint8_t *blob, uint8_t *out96) {
     memcpy(out96, &res, 96);
 }
 
-hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stream) {
+// slice_bytes (optional): every slice holds at least this many bytes, so
+// one copy of min(slice_bytes, kSliceReserve) per slice brings its header and
+// masked sums together (a valid header's sums always fit: msm() placed them
+// within that bound), saving the header-first round trip.
+hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stream, size_t slice_bytes = 0) {
+    if (slice_bytes >= kSliceQ) {
+        const size_t bytes = slice_bytes < kSliceReserve ? slice_bytes : kSliceReserve;
+        std::vector<std::vector<uint8_t>> blob(batch);
+        for (uint32_t m = 0; m < batch; m++) {
+            blob[m].assign(bytes, 0);
+            ZK_CHECK(hipMemcpyAsync(blob[m].data(), slices[m], bytes, hipMemcpyDeviceToHost, stream));
+        }
+        ZK_CHECK(hipStreamSynchronize(stream));
+        std::vector<uint8_t> out((size_t)batch * 96);
+        for (uint32_t m = 0; m < batch; m++) {
+            const SliceHeader *h = reinterpret_cast<const SliceHeader *>(blob[m].data() + 64);
+            // no result (cleared or stale header), or sums beyond what was read
+            if (!header_ok(h) || slice_sums_off(h->W, h->nbits) > bytes) return hipErrorInvalidValue;
+        }
+        for (uint32_t m = 0; m < batch; m++) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
+        for (uint32_t m = 0; m < batch; m++)
+            ZK_CHECK(hipMemcpyAsync(slices[m], &out[(size_t)m * 96], 96, hipMemcpyHostToDevice, stream));
+        return hipStreamSynchronize(stream);  // the host buffers must outlive the copies
+    }
     // the largest slice blob: header + Q for W <= 128, nbits <= 16 is bounded
     // by slice_sums_off(W, nbits); read the header first to size each copy
     std::vector<SliceHeader> hdr(batch);
@@ -1522,14 +1545,15 @@ hipError_t msm_batch_host(void *out, const zk_msm_item *items, int count, void *
     return hipSuccess;
 }
 
-hipError_t msm_collect(uint8_t *const *slices_dev, uint32_t batch, hipStream_t stream) {
+// slice_bytes: as host_collect (0 = unknown)
+hipError_t msm_collect(uint8_t *const *slices_dev, uint32_t batch, hipStream_t stream, size_t slice_bytes = 0) {
     if (!batch) return hipSuccess;
     if (device_tail_requested()) return device_collect(slices_dev, batch, stream);
     std::vector<uint8_t *> slices(batch);
     ZK_CHECK(hipMemcpyAsync(slices.data(), slices_dev, (size_t)batch * sizeof(uint8_t *), hipMemcpyDeviceToHost,
                             stream));
     ZK_CHECK(hipStreamSynchronize(stream));
-    return host_collect(slices.data(), batch, stream);
+    return host_collect(slices.data(), batch, stream, slice_bytes);
 }
 
 }  // namespace zk
@@ -1582,11 +1606,12 @@ hipError_t batch_msm_collect(void *remain_indices, void *remain_acc, void *next_
         (void)remain_acc;
         (void)next_remain_indices;
         (void)next_remain_acc;
-        (void)workers;
-        (void)windows;
-        (void)window_bits;
         if (!buckets && batch_size) return hipErrorInvalidValue;
-        return msm_collect((uint8_t *const *)buckets, batch_size, stream);
+        // each slice is ((windows << window_bits) + 2 workers) Jacobian points
+        // (src/cuda/msm.rs:297), as the caller sized it for msm()
+        const uint64_t pts = window_bits < 40 ? ((uint64_t)windows << window_bits) + 2ull * workers : 0;
+        const size_t slice_bytes = pts && pts < (1ull << 40) ? (size_t)pts * sizeof(G1Jac) : 0;
+        return msm_collect((uint8_t *const *)buckets, batch_size, stream, slice_bytes);
     });
 }
 
