@@ -24,20 +24,34 @@ constexpr uint64_t kPInv = 0x87d20782e4866389ULL;  // -p^-1 mod 2^64
 constexpr uint64_t kOne[4] = {0xd35d438dc58f0d9dULL, 0x0a78eb28f5c70b3dULL, 0x666ea36f7879462cULL,
                               0x0e0a77c19a07df2fULL};  // R mod p
 
+// word-wise add / subtract with carry (compile to add-with-carry chains)
+inline uint64_t addc(uint64_t a, uint64_t b, uint64_t &c) {
+    const u128 t = (u128)a + b + c;
+    c = (uint64_t)(t >> 64);
+    return (uint64_t)t;
+}
+inline uint64_t subb(uint64_t a, uint64_t b, uint64_t &br) {
+    const u128 t = (u128)a - b - br;
+    br = (uint64_t)(t >> 64) & 1;
+    return (uint64_t)t;
+}
+// a - p if a >= p (a < 2^257 as 4 words plus the carry-out `hi`), else a:
+// both values formed and one selected by mask (the tail's values are random,
+// so data-dependent branches mispredict half the time)
+inline void reduce_once(uint64_t a[4], uint64_t hi) {
+    uint64_t d[4], br = 0;
+    for (int i = 0; i < 4; i++) d[i] = subb(a[i], kP[i], br);
+    const uint64_t keep = (uint64_t)0 - (br & (hi ^ 1));  // a < p
+    for (int i = 0; i < 4; i++) a[i] = (a[i] & keep) | (d[i] & ~keep);
+}
 inline bool geq_p(const uint64_t a[4]) {
-    for (int i = 3; i >= 0; i--) {
-        if (a[i] > kP[i]) return true;
-        if (a[i] < kP[i]) return false;
-    }
-    return true;
+    uint64_t br = 0;
+    for (int i = 0; i < 4; i++) (void)subb(a[i], kP[i], br);
+    return br == 0;
 }
 inline void sub_p(uint64_t a[4]) {
-    u128 br = 0;
-    for (int i = 0; i < 4; i++) {
-        u128 t = (u128)a[i] - kP[i] - (uint64_t)br;
-        a[i] = (uint64_t)t;
-        br = (t >> 64) ? 1 : 0;
-    }
+    uint64_t br = 0;
+    for (int i = 0; i < 4; i++) a[i] = subb(a[i], kP[i], br);
 }
 inline Fp fp_zero() { return Fp{{0, 0, 0, 0}}; }
 inline Fp fp_one() { return Fp{{kOne[0], kOne[1], kOne[2], kOne[3]}}; }
@@ -46,31 +60,19 @@ inline bool fp_eq(const Fp &a, const Fp &b) { return memcmp(&a, &b, sizeof(Fp)) 
 
 inline Fp fp_add(const Fp &a, const Fp &b) {
     Fp r;
-    u128 c = 0;
-    for (int i = 0; i < 4; i++) {
-        c += (u128)a.l[i] + b.l[i];
-        r.l[i] = (uint64_t)c;
-        c >>= 64;
-    }
-    if (geq_p(r.l)) sub_p(r.l);
+    uint64_t c = 0;
+    for (int i = 0; i < 4; i++) r.l[i] = addc(a.l[i], b.l[i], c);
+    reduce_once(r.l, c);
     return r;
 }
 inline Fp fp_sub(const Fp &a, const Fp &b) {
     Fp r;
-    u128 br = 0;
-    for (int i = 0; i < 4; i++) {
-        u128 t = (u128)a.l[i] - b.l[i] - (uint64_t)br;
-        r.l[i] = (uint64_t)t;
-        br = (t >> 64) ? 1 : 0;
-    }
-    if (br) {
-        u128 c = 0;
-        for (int i = 0; i < 4; i++) {
-            c += (u128)r.l[i] + kP[i];
-            r.l[i] = (uint64_t)c;
-            c >>= 64;
-        }
-    }
+    uint64_t br = 0;
+    for (int i = 0; i < 4; i++) r.l[i] = subb(a.l[i], b.l[i], br);
+    // p added back under a mask when the difference went negative
+    const uint64_t m = (uint64_t)0 - br, c0 = 0;
+    uint64_t c = c0;
+    for (int i = 0; i < 4; i++) r.l[i] = addc(r.l[i], kP[i] & m, c);
     return r;
 }
 // CIOS Montgomery product on 64-bit limbs
@@ -96,7 +98,7 @@ inline Fp fp_mul(const Fp &a, const Fp &b) {
         t[4] = t[5] + (uint64_t)(c >> 64);
     }
     Fp r{{t[0], t[1], t[2], t[3]}};
-    if (t[4] || geq_p(r.l)) sub_p(r.l);
+    reduce_once(r.l, t[4]);
     return r;
 }
 inline Fp fp_sqr(const Fp &a) { return fp_mul(a, a); }
