@@ -27,6 +27,7 @@
 // thread's inverse over its rows.
 #include <string.h>
 
+#include "ff29.h"
 #include "zkcore.h"
 
 using namespace zk;
@@ -310,13 +311,13 @@ __global__ __launch_bounds__(kT) void k_peval_blocks(const EvalArgs a, uint32_t 
     __shared__ Fr S[kT];
     const uint32_t b = blockIdx.x, m = blockIdx.y;
     const Fr *__restrict__ src = a.poly[m];
-    const Fr x = fe_load<FrParams>(&pw[0]);
+    const F29 x29 = f29_from32(fe_load<FrParams>(&pw[0]));  // x in the 29-bit product's form
     const uint64_t t0 = (uint64_t)b * kBlk + (uint64_t)threadIdx.x * kE;
     Fr L = fe_zero<FrParams>();
 #pragma unroll
     for (int e = kE - 1; e >= 0; e--) {
         const Fr c = t0 + e < n ? fe_load<FrParams>(&src[t0 + e]) : fe_zero<FrParams>();
-        L = c + x * L;
+        L = c + fr_mul_w29(L, x29);
     }
     const Fr v = block_fold(L, S, pw, 3);
     if (threadIdx.x == 0) fe_store<FrParams>(&tot[(size_t)m * nb + b], v);
@@ -354,11 +355,12 @@ __global__ __launch_bounds__(kT) void k_peval_combine(const EvalArgs a, uint32_t
 // pass over out per polynomial
 __global__ __launch_bounds__(kT) void k_horner_combine(const EvalArgs a, uint32_t k, bool first,
                                                        const Fr *__restrict__ vp, Fr *__restrict__ out, uint32_t n) {
-    const Fr v = fe_load<FrParams>(vp);
+    const F29 v29 = f29_from32(fe_load<FrParams>(vp));  // v in the 29-bit product's form
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-        Fr acc = first ? fe_load<FrParams>(&a.poly[0][j]) : fe_load<FrParams>(&out[j]) * v + fe_load<FrParams>(&a.poly[0][j]);
-        for (uint32_t i = 1; i < k; i++) acc = acc * v + fe_load<FrParams>(&a.poly[i][j]);
+        Fr acc = first ? fe_load<FrParams>(&a.poly[0][j])
+                       : fr_mul_w29(fe_load<FrParams>(&out[j]), v29) + fe_load<FrParams>(&a.poly[0][j]);
+        for (uint32_t i = 1; i < k; i++) acc = fr_mul_w29(acc, v29) + fe_load<FrParams>(&a.poly[i][j]);
         fe_store<FrParams>(&out[j], acc);
     }
 }

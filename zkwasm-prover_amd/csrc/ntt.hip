@@ -289,6 +289,13 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
     }
 
     const uint32_t base_shift = log_t - log_p;
+    // the coset scales (at most two distinct values: zeta, zeta^2) in the
+    // 29-bit product's form, converted once
+    F29 cs1{}, cs2{};
+    if (cn > 1) {
+        cs1 = f29_from32(fe_load<FrParams>(&coset[0]));
+        cs2 = cn > 2 ? f29_from32(fe_load<FrParams>(&coset[1])) : cs1;
+    }
     for (uint32_t e = tid; e < tile; e += bs) {
         uint32_t g = e & (G - 1), i = e >> log_g;
         uint32_t index = index0 + g;
@@ -300,13 +307,20 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
             v = fe_load<FrParams>(&x[pos]);
             if (cn > 1) {
                 const uint32_t r = (uint32_t)(pos % cn);
-                if (r) v = v * fe_load<FrParams>(&coset[r - 1]);
+                if (r == 1 || r == 2) {
+                    F29 w;  // selected limb by limb: no indexed local array
+#pragma unroll
+                    for (int q = 0; q < 9; q++) w.l[q] = r == 1 ? cs1.l[q] : cs2.l[q];
+                    v = fr_mul_w29(v, w);
+                } else if (r) {
+                    v = v * fe_load<FrParams>(&coset[r - 1]);
+                }
             }
         }
         if (log_p) {
             uint32_t k = index & (p - 1);
             uint32_t ex = (k * i) << base_shift;
-            if (ex) v = v * fe_load<FrParams>(&omegas[ex]);
+            if (ex) v = fr_mul_w29(v, f29_from32(fe_load<FrParams>(&omegas[ex])));
         }
         s[(g << deg) + i] = v;
     }
@@ -350,7 +364,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
 
     const uint32_t lp = log_p < log_g ? log_p : log_g;
     const bool scale = n_inv != nullptr;
-    Fr ninv = scale ? fe_load<FrParams>(n_inv) : fe_zero<FrParams>();
+    const F29 ninv29 = f29_from32(scale ? fe_load<FrParams>(n_inv) : fe_zero<FrParams>());
     for (uint32_t e = tid; e < tile; e += bs) {
         uint32_t klow = e & ((1u << lp) - 1);
         uint32_t j = (e >> lp) & (D - 1);
@@ -360,7 +374,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(const NttPassArgs a) {
         uint32_t k = index & (p - 1);
         size_t addr = ((size_t)(index - k) << deg) + k + ((size_t)j << log_p);
         Fr v = s[(g << deg) + bitrev32(j, deg)];
-        if (scale) v = v * ninv;
+        if (scale) v = fr_mul_w29(v, ninv29);
         fe_store<FrParams>(&y[addr], v);
     }
 }
