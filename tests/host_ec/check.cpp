@@ -41,3 +41,18 @@ void he_from_xyzz(void *r, const void *xyzz) {
     memcpy(r, &x, 96);
 }
 }
+extern "C" {
+// the MSM tail over W * (nb + 1) XYZZ masked sums (128 B each), window-major
+void he_msm_tail(void *r, const void *q, uint32_t c, uint32_t W, uint32_t nb, uint32_t l1) {
+    Jac x = msm_tail((const Xyzz *)q, c, W, nb, l1);
+    memcpy(r, &x, 96);
+}
+// the same sum split at exponent lo: 2^lo (exponents lo..top) + (exponents 0..lo-1)
+void he_msm_tail_split(void *r, const void *q, uint32_t c, uint32_t W, uint32_t nb, uint32_t l1, int lo) {
+    const Xyzz *p = (const Xyzz *)q;
+    Xyzz a = msm_tail_range(p, c, W, nb, l1, lo, msm_tail_top(c, W, nb, l1));
+    for (int k = 0; k < lo; k++) a = xyzz_dbl(a);
+    Jac x = xyzz_normalize(xyzz_add(a, msm_tail_range(p, c, W, nb, l1, 0, lo - 1)));
+    memcpy(r, &x, 96);
+}
+}

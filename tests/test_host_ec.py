@@ -6,7 +6,9 @@ edge values (0, 1, p - 1, p - 2, values whose Montgomery form is p - 1, carries
 across every word) and seeded random ones; curve operations against the
 oracle's G1 arithmetic on points with non-trivial Z, including the doubling,
 cancellation and identity cases and the XYZZ conversion the tail applies to
-the device's masked sums.
+the device's masked sums, and the MSM tail (one Horner pass over the
+masked sums' exponents) against the oracle's multiple of G for the weighted
+sum of the masked sums' discrete logarithms.
 """
 import ctypes
 import os
@@ -33,6 +35,8 @@ def he(tmp_path_factory):
         getattr(L, f).argtypes = [ctypes.c_void_p] * 3
     for f in ("he_inv", "he_jac_dbl", "he_jac_normalize", "he_from_xyzz"):
         getattr(L, f).argtypes = [ctypes.c_void_p] * 2
+    L.he_msm_tail.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_uint32] * 4
+    L.he_msm_tail_split.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_uint32] * 4 + [ctypes.c_int]
     return L
 
 
@@ -144,3 +148,61 @@ def test_from_xyzz(he, zo):
     assert _affine(he, out) == (x, y)
     he.he_from_xyzz(out.ctypes.data, np.concatenate([mont(1), mont(1), mont(0), mont(0)]).ctypes.data)
     assert _affine(he, out) is None
+
+
+R_ORDER = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+
+
+def _xyzz(zo, k, lam):
+    """k G in XYZZ form (lam^2 x, lam^3 y, lam^2, lam^3); k = 0 -> ZZ = 0"""
+    if k % R_ORDER == 0:
+        return np.concatenate([mont(1), mont(1), mont(0), mont(0)])
+    _, (x, y) = _jac(zo, k % R_ORDER, 1)
+    zz, zzz = lam * lam % P, pow(lam, 3, P)
+    return np.concatenate([mont(x * zz % P), mont(y * zzz % P), mont(zz), mont(zzz)])
+
+
+def _tail(he, zo, ks, c, nb, l1=3, seed=0, split=None):
+    """ks[w][b], b <= nb: discrete logs of the masked sums; returns (got,
+    expected); split: the exponent at which the sum is cut in two parts"""
+    rng = np.random.default_rng(seed)
+    W = len(ks)
+    q = np.concatenate([_xyzz(zo, ks[w][b], int(rng.integers(2, 2**62))) for w in range(W) for b in range(nb + 1)])
+    out = np.zeros(12, dtype=np.uint64)
+    if split is None:
+        he.he_msm_tail(out.ctypes.data, np.ascontiguousarray(q).ctypes.data, c, W, nb, l1)
+    else:
+        he.he_msm_tail_split(out.ctypes.data, np.ascontiguousarray(q).ctypes.data, c, W, nb, l1, split)
+    k = sum((ks[w][nb] + sum(ks[w][b] << (l1 + b) for b in range(nb))) << (c * w) for w in range(W)) % R_ORDER
+    return _affine(he, out), _oracle_affine(zo, k)
+
+
+@pytest.mark.parametrize("c,W,nb", [(17, 15, 13), (16, 16, 12), (5, 51, 1), (4, 64, 0), (2, 128, 0), (13, 20, 9)])
+def test_msm_tail(he, zo, c, W, nb):
+    """seeded random masked sums at the slice shapes of windows 2..17 (nb =
+    c - 1 - 3 bits, 0 when a window has at most 8 buckets), with identities"""
+    rng = np.random.default_rng(c * 1000 + W)
+    ks = [[int.from_bytes(rng.bytes(32), "little") % R_ORDER for _ in range(nb + 1)] for _ in range(W)]
+    ks[W // 2][0] = 0
+    ks[0][nb] = 0
+    ks[W - 1] = [0] * (nb + 1)
+    got, exp = _tail(he, zo, ks, c, nb, seed=c)
+    assert got == exp
+    top = c * (W - 1) + (3 + nb if nb else 1) - 1
+    for lo in (1, top * 11 // 20, top):
+        assert _tail(he, zo, ks, c, nb, seed=c, split=lo) == (exp, exp)
+
+
+def test_msm_tail_doubling_cancel_identity(he, zo):
+    """a term equal to the running sum (doubling inside the addition), its
+    negation (cancellation to the identity), and all-identity sums"""
+    # W = 2, nb = 0, c = 2: acc = 4 Q[1][0] when Q[0][0] is added
+    got, exp = _tail(he, zo, [[5], [5 * pow(4, -1, R_ORDER)]], 2, 0)
+    assert got == exp == _oracle_affine(zo, 10)
+    got, exp = _tail(he, zo, [[R_ORDER - 8], [2]], 2, 0)
+    assert got is None and exp is None
+    got, exp = _tail(he, zo, [[0, 0, 0], [0, 0, 0]], 9, 2)
+    assert got is None and exp is None
+    # two windows whose term ranges overlap (l1 + nb > c): several terms per exponent
+    got, exp = _tail(he, zo, [[3, 1, 4, 1, 5], [9, 2, 6, 5, 3], [5, 8, 9, 7, 9]], 5, 4)
+    assert got == exp

@@ -1297,33 +1297,40 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     return prof_finish(prof);
 }
 
-// Tail on the host: the per-window Horner over the masked sums and the Horner
-// over windows form a serial chain of ~300 point operations that a single
-// GPU wave runs in ~3 ms; host cores run it in well under 0.1 ms.  The Q
-// points are copied down, combined exactly as the device tail would, and the
-// normalised result is written back into each slice.
+// Tail on the host: one Horner pass over the exponents of the masked sums
+// (msm_tail in host_ec.h), a serial chain of ~250 XYZZ doublings and ~210
+// additions that a single GPU wave would run in milliseconds; a host core
+// runs it in ~0.25 ms.  The Q points are copied down and the normalised
+// result is written back into each slice.  split: the exponents below ~0.55
+// of the top go to a second thread (their sum is added after the main chain
+// has doubled its part down to them), for a lone MSM with idle cores.
 // The slice's header has been checked by the caller (header_ok).
-void host_tail_one(const uint8_t *blob, uint8_t *out96) {
+void host_tail_one(const uint8_t *blob, uint8_t *out96, bool split = false) {
     const SliceHeader *h = reinterpret_cast<const SliceHeader *>(blob + 64);
-    zkh::Jac res = zkh::jac_id();
-    {
-        const uint32_t c = h->c, W = h->W, nb = h->nbits;
-        const zkh::Fp *q = reinterpret_cast<const zkh::Fp *>(blob + kSliceQ);  // 4 Fp per XYZZ point
-        auto Q = [&](uint32_t w, uint32_t b) {
-            const zkh::Fp *p = q + ((size_t)w * (nb + 1) + b) * 4;
-            return zkh::jac_from_xyzz(p[0], p[1], p[2], p[3]);
-        };
-        for (int w = (int)W - 1; w >= 0; w--) {
-            zkh::Jac s = nb ? Q(w, nb - 1) : zkh::jac_id();
-            for (int b = (int)nb - 2; b >= 0; b--) s = zkh::jac_add(zkh::jac_dbl(s), Q(w, b));
-            for (uint32_t k = 1; k < kL1; k <<= 1) s = zkh::jac_dbl(s);
-            s = zkh::jac_add(s, Q(w, nb));
-            if (w != (int)W - 1)
-                for (uint32_t k = 0; k < c; k++) res = zkh::jac_dbl(res);
-            res = zkh::jac_add(res, s);
+    static_assert(sizeof(zkh::Xyzz) == sizeof(G1Xyzz), "host and device XYZZ layouts differ");
+    static_assert((kL1 & (kL1 - 1)) == 0, "kL1 is a power of two");
+    const zkh::Xyzz *q = reinterpret_cast<const zkh::Xyzz *>(blob + kSliceQ);
+    const uint32_t c = h->c, W = h->W, nb = h->nbits, l1 = (uint32_t)__builtin_ctz(kL1);
+    const int top = zkh::msm_tail_top(c, W, nb, l1), lo = top * 11 / 20;
+    zkh::Jac res;
+    if (!split || lo < 1) {
+        res = zkh::msm_tail(q, c, W, nb, l1);
+    } else {
+        zkh::Xyzz low;
+        auto low_part = [&]() { low = zkh::msm_tail_range(q, c, W, nb, l1, 0, lo - 1); };
+        std::thread th;
+        try {
+            th = std::thread(low_part);
+        } catch (...) {
         }
+        zkh::Xyzz a = zkh::msm_tail_range(q, c, W, nb, l1, lo, top);
+        for (int k = 0; k < lo; k++) a = zkh::xyzz_dbl(a);
+        if (th.joinable())
+            th.join();
+        else
+            low_part();  // no thread could be started
+        res = zkh::xyzz_normalize(zkh::xyzz_add(a, low));
     }
-    res = zkh::jac_normalize(res);
     memcpy(out96, &res, 96);
 }
 
@@ -1346,7 +1353,7 @@ hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stre
             // no result (cleared or stale header), or sums beyond what was read
             if (!header_ok(h) || slice_sums_off(h->W, h->nbits) > bytes) return hipErrorInvalidValue;
         }
-        for (uint32_t m = 0; m < batch; m++) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
+        for (uint32_t m = 0; m < batch; m++) host_tail_one(blob[m].data(), &out[(size_t)m * 96], batch == 1);
         for (uint32_t m = 0; m < batch; m++)
             ZK_CHECK(hipMemcpyAsync(slices[m], &out[(size_t)m * 96], 96, hipMemcpyHostToDevice, stream));
         return hipStreamSynchronize(stream);  // the host buffers must outlive the copies
@@ -1373,7 +1380,7 @@ hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stre
     unsigned nt = batch < 2 ? 1 : (hw ? (hw < batch ? hw : batch) : 1);
     if (nt > 16) nt = 16;
     if (nt <= 1) {
-        for (uint32_t m = 0; m < batch; m++) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
+        for (uint32_t m = 0; m < batch; m++) host_tail_one(blob[m].data(), &out[(size_t)m * 96], batch == 1);
     } else {
         std::vector<std::thread> th;
         unsigned started = 0;
@@ -1412,7 +1419,8 @@ hipError_t host_collect_strided(uint8_t *base, size_t stride, uint32_t batch, hi
     unsigned nt = batch < 2 ? 1 : (hw ? (hw < batch ? hw : batch) : 1);
     if (nt > 16) nt = 16;
     auto run = [&](unsigned t) {
-        for (uint32_t m = t; m < batch; m += nt) host_tail_one(&img[(size_t)m * stride], &out[(size_t)m * 96]);
+        for (uint32_t m = t; m < batch; m += nt)
+            host_tail_one(&img[(size_t)m * stride], &out[(size_t)m * 96], batch == 1);
     };
     if (nt <= 1) {
         run(0);
