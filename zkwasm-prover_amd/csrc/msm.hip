@@ -838,18 +838,121 @@ __global__ __launch_bounds__(64) void k_msm_reduce_l1(const X29S *__restrict__ b
     x29_store(&sum1[g], run, re);
 }
 
-// grid (nbits + 1, items * W, parts): q < nbits -> sum of sum1[g][s] over s
-// with bit q set; q == nbits -> sum of ws1[g][s] over all s.  With one part
-// the sum goes into the item's curve slice at Q[w][q]; with several, block z
-// sums the z-th slice of s into qpart[(g * (nbits + 1) + q) * parts + z] and
-// k_msm_masked_merge adds the parts (a single MSM has only (nbits+1) * W
-// sums, too few workgroups to fill the device).
+// the item window g's masked sum Q[bit] in its curve slice
 __device__ __forceinline__ G1Xyzz *slot_q(uint8_t *const *slots, uint32_t g, uint32_t W, uint32_t nbits,
                                           uint32_t bit) {
     G1Xyzz *q = reinterpret_cast<G1Xyzz *>(slots[g / W] + kSliceQ);
     return &q[(size_t)(g % W) * (nbits + 1) + bit];
 }
 
+// an X29 point and its identity flag from lane ^ m
+__device__ __forceinline__ void x29_shfl_xor(X29 &o, bool &oe, const X29 &p, bool pe, int m) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        o.x.l[i] = (uint32_t)__shfl_xor((int)p.x.l[i], m);
+        o.y.l[i] = (uint32_t)__shfl_xor((int)p.y.l[i], m);
+        o.zz.l[i] = (uint32_t)__shfl_xor((int)p.zz.l[i], m);
+        o.zzz.l[i] = (uint32_t)__shfl_xor((int)p.zzz.l[i], m);
+    }
+    oe = __shfl_xor((int)pe, m) != 0;
+}
+__device__ __forceinline__ X29 x29_select(bool c, const X29 &a, const X29 &b) {
+    X29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        r.x.l[i] = c ? a.x.l[i] : b.x.l[i];
+        r.y.l[i] = c ? a.y.l[i] : b.y.l[i];
+        r.zz.l[i] = c ? a.zz.l[i] : b.zz.l[i];
+        r.zzz.l[i] = c ? a.zzz.l[i] : b.zzz.l[i];
+    }
+    return r;
+}
+
+// Level-1 reduction with the masked sums folded in (T1 >= 64): one wave per 64
+// consecutive segments s = 64 h + lane of an item window g.  Each lane forms
+// its segment's run (plain sum of its kL1 buckets) and acc (weighted sum) as
+// k_msm_reduce_l1 does; then six butterfly steps over the lanes: at step k a
+// lane with bit k clear adds its partner's run, a lane with bit k set its
+// partner's acc, so every lane does one point addition per step.  Afterwards
+// lane 0 holds the wave's total run T, lane 2^j (j < 6) the sum of the runs
+// of the lanes with bit j set, and lane 63 the total acc A.  They go to
+// part[(g * T1/64 + h) * 8 + {0, 1 + j, 7}] for k_msm_fold_sums.
+__global__ __launch_bounds__(64) void k_msm_reduce_fold(const X29S *__restrict__ buckets,
+                                                        const uint32_t *__restrict__ off, uint32_t B, uint32_t T1,
+                                                        X29S *__restrict__ part) {
+    const uint32_t lane = threadIdx.x, hw = T1 >> 6;
+    const uint32_t g = blockIdx.x / hw, h = blockIdx.x % hw;
+    const uint32_t s = h * 64 + lane;
+    X29 run, acc;
+    bool re = true, ae = true, be;
+    const uint32_t lo = s * kL1;
+    const uint32_t hi = lo + kL1 < B ? lo + kL1 : B;
+    for (uint32_t b = hi; b-- > lo;) {
+        const uint32_t idx = g * B + b;
+        if (off[idx + 1] != off[idx]) {
+            const X29 q = x29_load(&buckets[idx], be);
+            x29_add(run, re, q, be);
+        }
+        x29_add(acc, ae, run, re);
+    }
+#pragma unroll 1
+    for (int k = 0; k < 6; k++) {
+        const bool up = (lane >> k) & 1u;
+        // the partner (bit k flipped) needs the quantity this lane does not update
+        X29 o;
+        bool oe;
+        x29_shfl_xor(o, oe, x29_select(up, run, acc), up ? re : ae, 1 << k);
+        X29 a = x29_select(up, acc, run);
+        bool e = up ? ae : re;
+        x29_add(a, e, o, oe);
+        if (up) {
+            acc = a;
+            ae = e;
+        } else {
+            run = a;
+            re = e;
+        }
+    }
+    X29S *dst = part + ((size_t)g * hw + h) * 8;
+    if (lane == 63)
+        x29_store(&dst[7], acc, ae);
+    else if (lane == 0)
+        x29_store(&dst[0], run, re);
+    else if ((lane & (lane - 1)) == 0)
+        x29_store(&dst[1 + __builtin_ctz(lane)], run, re);
+}
+
+// grid (nbits + 1, gw), one wave each: the item window's masked sum Q[o] from
+// the partials of k_msm_reduce_fold (hw = T1 / 64 per window).  o < 6: sum of
+// the waves' lane-bit partials j = o; 6 <= o < nbits: sum of the totals T_h
+// of the waves with bit o - 6 of h set; o = nbits: sum of the waves' A.
+__global__ __launch_bounds__(64) void k_msm_fold_sums(const X29S *__restrict__ part, uint32_t T1, uint32_t nbits,
+                                                      uint32_t W, uint8_t *const *__restrict__ slots) {
+    const uint32_t o = blockIdx.x, g = blockIdx.y, lane = threadIdx.x, hw = T1 >> 6;
+    const uint32_t slot = o < 6 ? 1 + o : (o < nbits ? 0 : 7);
+    X29 acc;
+    bool ae = true, qe;
+    for (uint32_t h = lane; h < hw; h += 64) {
+        if (o < 6 || o == nbits || ((h >> (o - 6)) & 1u)) {
+            const X29 q = x29_load(&part[((size_t)g * hw + h) * 8 + slot], qe);
+            x29_add(acc, ae, q, qe);
+        }
+    }
+#pragma unroll 1
+    for (int k = 0; k < 6; k++) {
+        X29 q;
+        x29_shfl_xor(q, qe, acc, ae, 1 << k);
+        x29_add(acc, ae, q, qe);
+    }
+    if (lane == 0) xyzz_store(slot_q(slots, g, W, nbits, o), x29_to_xyzz(acc, ae));
+}
+
+// grid (nbits + 1, items * W, parts): q < nbits -> sum of sum1[g][s] over s
+// with bit q set; q == nbits -> sum of ws1[g][s] over all s.  With one part
+// the sum goes into the item's curve slice at Q[w][q]; with several, block z
+// sums the z-th slice of s into qpart[(g * (nbits + 1) + q) * parts + z] and
+// k_msm_masked_merge adds the parts (a single MSM has only (nbits+1) * W
+// sums, too few workgroups to fill the device).
 // LDS tree over the 256 threads' X29 values in sh; the sum ends in sh[0]
 __device__ __forceinline__ void x29_block_tree(X29S *sh) {
     bool ae, qe;
@@ -1267,9 +1370,16 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     ZK_CHECK(prof_mark(prof, PH_REDUCE, stream));
     const uint32_t gw = count * p.W;
     const uint32_t nseg = gw * p.T1;
-    k_msm_reduce_l1<<<(nseg + 63) / 64, 64, 0, stream>>>(buckets, offsets, gw, p.B, p.T1, ws1, sum1);
-    ZK_CHECK(hipGetLastError());
-    {
+    if (p.T1 >= 64) {
+        // one wave per 64 segments; its 8 partials go to the ws1 array (gw *
+        // T1 / 8 entries, within its gw * T1)
+        k_msm_reduce_fold<<<nseg / 64, 64, 0, stream>>>(buckets, offsets, p.B, p.T1, ws1);
+        ZK_CHECK(hipGetLastError());
+        k_msm_fold_sums<<<dim3(p.nbits + 1, gw), 64, 0, stream>>>(ws1, p.T1, p.nbits, p.W, slots);
+        ZK_CHECK(hipGetLastError());
+    } else {
+        k_msm_reduce_l1<<<(nseg + 63) / 64, 64, 0, stream>>>(buckets, offsets, gw, p.B, p.T1, ws1, sum1);
+        ZK_CHECK(hipGetLastError());
         // split each masked sum over `parts` workgroups when there are too
         // few sums to give every SIMD a wave (the sums are latency-bound
         // chains of point additions; more parts than that only adds tree
