@@ -156,6 +156,19 @@ def test_reference_known_answer(gpu, zo, log_n, batch):
         assert (got[b:b + 1] == exp).all(), b
 
 
+@pytest.mark.parametrize("log_n,c", [(20, 6), (22, 10)])
+def test_msm_large_sort_groups(gpu, zo, log_n, c):
+    """windows whose sort groups exceed what one workgroup sorts whole (24576
+    entries): 2^20 points at c = 6 give 32 groups of ~32k entries per window
+    with a single low-bit key, 2^22 at c = 10 128 groups of ~32k; they are
+    sorted tile by tile.  Checked by the known-answer construction."""
+    n = 1 << log_n
+    cand, g, pts = _kat_inputs(zo, n, 4545 + c)
+    sc = zo.rand_fr(n, 7100 + c)
+    got = zo.g1_to_affine(_zk_msm(pts, sc, c))
+    assert (got == zo.g1_to_affine(zo.g1_mul(g, zo.fr_inner_cycled(sc, cand)))).all()
+
+
 @pytest.mark.parametrize("n,batch", [(1000, 1), (1 << 14, 3)])
 def test_device_and_host_tails_agree(gpu, zo, monkeypatch, n, batch):
     """the window combination on host cores (default) and on the GPU

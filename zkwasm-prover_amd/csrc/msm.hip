@@ -504,83 +504,134 @@ __global__ __launch_bounds__(256) void k_msm_scatter_hi(const MsmItemDev *__rest
     }
 }
 
-// per sort group ((item, window, bin), or (item, bin) in fixed-base mode):
-// counting sort on the low bucket bits.  The group histogram gives every
-// bucket's offset; the entries are then placed tile by tile (4096 at a time):
-// each tile is counting-sorted in LDS and written out bucket run by bucket
-// run, so stores come in contiguous runs rather than one 4-byte scatter per
-// entry.  Also emits the bucket offsets.  KEYS: low-bit values the LDS arrays
-// hold (256, or 512 in fixed-base mode).
-constexpr uint32_t kLoTile = 4096;
+// per sort group ((item, window, bin), or (item, bin) in fixed-base mode),
+// one block of 1024 threads: counting sort on the low bucket bits, and the
+// bucket offsets.  The sorted group fills exactly its range [gs, ge) of the
+// entries.  A group of at most kGroupMax entries (a 2^22-point MSM's groups
+// hold ~16k, its top window's ~22k) is sorted whole: loaded once into
+// registers (24 entries per thread, coalesced), its key counts taken by LDS
+// atomics and scanned into the bucket offsets, each entry placed in LDS at its
+// key's cursor, and the group written out in order, fully coalesced.  A
+// larger group takes a counting pass over the whole group for the bucket
+// offsets, then tiles of kGroupMax entries, each counting-sorted in LDS (its
+// keys kept beside it) and written out bucket run by bucket run.  KEYS:
+// low-bit values the LDS arrays hold (256, or 512 in fixed-base mode).
+constexpr uint32_t kGroupThreads = 1024;
+constexpr uint32_t kGroupPer = 24;  // entries per thread
+constexpr uint32_t kGroupMax = kGroupThreads * kGroupPer;
+
+// exclusive scan of x[0..K) in LDS (K <= 1024) by a block of 1024 threads: a
+// shuffle scan inside each wave, the 16 wave totals through t[0..16)
+template <uint32_t K>
+__device__ __forceinline__ void lds_exclusive_scan1024(uint32_t *x, uint32_t *t) {
+    static_assert(K <= kGroupThreads, "one counter per thread");
+    const uint32_t i = threadIdx.x, lane = i & 63, wv = i >> 6;
+    const uint32_t v = i < K ? x[i] : 0u;
+    uint32_t s = v;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)s, d, 64);
+        if (lane >= d) s += o;
+    }
+    if (lane == 63) t[wv] = s;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < wv; k++) off += t[k];
+    if (i < K) x[i] = s - v + off;
+    __syncthreads();
+}
+
 template <bool kPacked, uint32_t KEYS>
-__global__ __launch_bounds__(256) void k_msm_sort_lo(const MsmItemDev *__restrict__ items, uint32_t W, uint32_t Wd,
-                                                     uint32_t nbins, uint32_t fixed,
-                                                     const typename TmpEnt<kPacked>::T *__restrict__ tmp,
-                                                     const uint32_t *__restrict__ starts, uint32_t lo_bits,
-                                                     uint32_t B, uint32_t ngroups, uint32_t *__restrict__ entries,
-                                                     uint32_t *__restrict__ offsets) {
+__global__ __launch_bounds__(kGroupThreads) void k_msm_sort_group(
+    const MsmItemDev *__restrict__ items, uint32_t W, uint32_t Wd, uint32_t nbins, uint32_t fixed,
+    const typename TmpEnt<kPacked>::T *__restrict__ tmp, const uint32_t *__restrict__ starts, uint32_t lo_bits,
+    uint32_t B, uint32_t ngroups, uint32_t *__restrict__ entries, uint32_t *__restrict__ offsets) {
     using E = TmpEnt<kPacked>;
     using KeyT = typename std::conditional<(KEYS > 256), uint16_t, uint8_t>::type;
-    __shared__ uint32_t h[KEYS], cur[KEYS], toff[KEYS], tcur[KEYS], sc[4];
-    __shared__ uint32_t stage[kLoTile];
-    __shared__ KeyT skey[kLoTile];
+    __shared__ uint32_t h[KEYS], cur[KEYS], th[KEYS], tcur[KEYS], sc[kGroupThreads / 64];
+    __shared__ uint32_t stage[kGroupMax];
+    __shared__ KeyT skey[kGroupMax];
     const uint32_t g = blockIdx.x;
     const uint32_t m = g / (W * nbins), r = g % (W * nbins);
     const uint32_t w = r / nbins, bin = r % nbins;
     const MsmItemDev it = items[m];
-    const uint32_t nlo = 1u << lo_bits;
-    const uint32_t tid = threadIdx.x;
     const uint64_t idx = hist_index(it, Wd, nbins, fixed != 0, w, bin, 0);
     const uint32_t gs = starts[idx];
     const uint32_t ge = starts[idx + group_tiles(it, Wd, fixed != 0)];  // the next group's start (or the total)
-    for (uint32_t k = tid; k < KEYS; k += 256) h[k] = 0;
-    __syncthreads();
-    for (uint32_t p = gs + tid; p < ge; p += blockDim.x) atomicAdd(&h[E::key(tmp[p])], 1u);
-    __syncthreads();
-    for (uint32_t k = tid; k < KEYS; k += 256) cur[k] = h[k];
-    __syncthreads();
-    lds_exclusive_scan<KEYS>(cur, sc);
-    for (uint32_t k = tid; k < KEYS; k += 256) cur[k] += gs;
-    __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nlo = 1u << lo_bits;
     const uint64_t base = ((uint64_t)m * W + w) * B + ((uint64_t)bin << lo_bits);
-    for (uint32_t k = tid; k < nlo; k += 256) offsets[base + k] = cur[k];
     if (g == ngroups - 1 && tid == 0) offsets[base + nlo] = ge;  // = total: offsets[NB]
-    for (uint32_t t0 = gs; t0 < ge; t0 += kLoTile) {
-        const uint32_t cnt = ge - t0 < kLoTile ? ge - t0 : kLoTile;
-        // the tile in registers: 16 entries per thread
-        typename E::T v[kLoTile / 256];
-        for (uint32_t k = tid; k < KEYS; k += 256) h[k] = 0;
+    typename E::T v[kGroupPer];
+    if (ge - gs <= kGroupMax) {
+        const uint32_t cnt = ge - gs;
+        for (uint32_t k = tid; k < KEYS; k += kGroupThreads) h[k] = 0;
         __syncthreads();
 #pragma unroll
-        for (uint32_t k = 0; k < kLoTile / 256; k++) {
-            const uint32_t q = k * 256 + tid;
+        for (uint32_t k = 0; k < kGroupPer; k++) {
+            const uint32_t q = k * kGroupThreads + tid;
             if (q < cnt) {
-                v[k] = tmp[t0 + q];
+                v[k] = tmp[gs + q];
                 atomicAdd(&h[E::key(v[k])], 1u);
             }
         }
         __syncthreads();
-        for (uint32_t k = tid; k < KEYS; k += 256) toff[k] = h[k];
+        lds_exclusive_scan1024<KEYS>(h, sc);
+        for (uint32_t k = tid; k < nlo; k += kGroupThreads) offsets[base + k] = gs + h[k];
+        __syncthreads();  // h becomes the cursors
+#pragma unroll
+        for (uint32_t k = 0; k < kGroupPer; k++) {
+            const uint32_t q = k * kGroupThreads + tid;
+            if (q < cnt) stage[atomicAdd(&h[E::key(v[k])], 1u)] = E::val(v[k]);
+        }
         __syncthreads();
-        lds_exclusive_scan<KEYS>(toff, sc);
-        for (uint32_t k = tid; k < KEYS; k += 256) tcur[k] = toff[k];
+        for (uint32_t q = tid; q < cnt; q += kGroupThreads) entries[gs + q] = stage[q];
+        return;
+    }
+    // a larger group: bucket offsets from a counting pass over the group
+    for (uint32_t k = tid; k < KEYS; k += kGroupThreads) cur[k] = 0;
+    __syncthreads();
+    for (uint32_t p = gs + tid; p < ge; p += kGroupThreads) atomicAdd(&cur[E::key(tmp[p])], 1u);
+    __syncthreads();
+    lds_exclusive_scan1024<KEYS>(cur, sc);
+    for (uint32_t k = tid; k < KEYS; k += kGroupThreads) cur[k] += gs;
+    __syncthreads();
+    for (uint32_t k = tid; k < nlo; k += kGroupThreads) offsets[base + k] = cur[k];
+    for (uint32_t t0 = gs; t0 < ge; t0 += kGroupMax) {
+        const uint32_t cnt = ge - t0 < kGroupMax ? ge - t0 : kGroupMax;
+        for (uint32_t k = tid; k < KEYS; k += kGroupThreads) th[k] = 0;
         __syncthreads();
 #pragma unroll
-        for (uint32_t k = 0; k < kLoTile / 256; k++) {
-            const uint32_t q = k * 256 + tid;
+        for (uint32_t k = 0; k < kGroupPer; k++) {
+            const uint32_t q = k * kGroupThreads + tid;
             if (q < cnt) {
-                const uint32_t at = atomicAdd(&tcur[E::key(v[k])], 1u);
-                stage[at] = E::val(v[k]);
-                skey[at] = (KeyT)E::key(v[k]);
+                v[k] = tmp[t0 + q];
+                atomicAdd(&th[E::key(v[k])], 1u);
             }
         }
         __syncthreads();
-        for (uint32_t q = tid; q < cnt; q += blockDim.x) {
-            const uint32_t key = skey[q];
-            entries[cur[key] + (q - toff[key])] = stage[q];
+        for (uint32_t k = tid; k < KEYS; k += kGroupThreads) h[k] = th[k];
+        __syncthreads();
+        lds_exclusive_scan1024<KEYS>(h, sc);  // h: the tile's key offsets
+        for (uint32_t k = tid; k < KEYS; k += kGroupThreads) tcur[k] = h[k];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kGroupPer; k++) {
+            const uint32_t q = k * kGroupThreads + tid;
+            if (q < cnt) {
+                const uint32_t key = E::key(v[k]);
+                const uint32_t at = atomicAdd(&tcur[key], 1u);
+                stage[at] = E::val(v[k]);
+                skey[at] = (KeyT)key;
+            }
         }
         __syncthreads();
-        for (uint32_t k = tid; k < KEYS; k += 256) cur[k] += h[k];
+        for (uint32_t q = tid; q < cnt; q += kGroupThreads) {
+            const uint32_t key = skey[q];
+            entries[cur[key] + (q - h[key])] = stage[q];
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < KEYS; k += kGroupThreads) cur[k] += th[k];
         __syncthreads();
     }
 }
@@ -1271,10 +1322,10 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     uint8_t **slots = (uint8_t **)p.ptr[A_SLOTS];
     uint32_t *hist = (uint32_t *)p.ptr[A_HIST];
     uint32_t *offsets = (uint32_t *)p.ptr[A_OFFSETS];
-    uint32_t *starts = (uint32_t *)p.ptr[A_STARTS];  // dead after k_msm_sort_lo
+    uint32_t *starts = (uint32_t *)p.ptr[A_STARTS];  // dead after k_msm_sort_group
     uint32_t *entries = (uint32_t *)p.ptr[A_ENTRIES];
     uint32_t *digits = entries;  // dead before the final entries are written
-    void *tmp = p.ptr[A_TMP];  // dead after k_msm_sort_lo
+    void *tmp = p.ptr[A_TMP];  // dead after k_msm_sort_group
     X29S *buckets = (X29S *)p.ptr[A_BUCKETS];
     X29S *phead = (X29S *)p.ptr[A_PHEAD];
     X29S *ptail = (X29S *)p.ptr[A_PTAIL];
@@ -1336,21 +1387,20 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
         k_msm_scatter_hi<false, kFixedMaxBins><<<sgrid, 256, 0, stream>>>(items, count, p.Wd, digits, p.lo_bits,
                                                                          p.nbins, fx, hist, starts, (uint2 *)tmp);
         ZK_CHECK(hipGetLastError());
-        k_msm_sort_lo<false, kFixedMaxKeys><<<ngroups, 256, 0, stream>>>(items, p.W, p.Wd, p.nbins, fx,
-                                                                        (const uint2 *)tmp, starts, p.lo_bits, p.B,
-                                                                        ngroups, entries, offsets);
+        k_msm_sort_group<false, kFixedMaxKeys><<<ngroups, kGroupThreads, 0, stream>>>(
+            items, p.W, p.Wd, p.nbins, fx, (const uint2 *)tmp, starts, p.lo_bits, p.B, ngroups, entries, offsets);
     } else if (p.packed) {
         k_msm_scatter_hi<true, 256><<<sgrid, 256, 0, stream>>>(items, count, p.Wd, digits, p.lo_bits, p.nbins, fx,
                                                                hist, starts, (uint32_t *)tmp);
         ZK_CHECK(hipGetLastError());
-        k_msm_sort_lo<true, 256><<<ngroups, 256, 0, stream>>>(items, p.W, p.Wd, p.nbins, fx, (const uint32_t *)tmp,
-                                                              starts, p.lo_bits, p.B, ngroups, entries, offsets);
+        k_msm_sort_group<true, 256><<<ngroups, kGroupThreads, 0, stream>>>(
+            items, p.W, p.Wd, p.nbins, fx, (const uint32_t *)tmp, starts, p.lo_bits, p.B, ngroups, entries, offsets);
     } else {
         k_msm_scatter_hi<false, 256><<<sgrid, 256, 0, stream>>>(items, count, p.Wd, digits, p.lo_bits, p.nbins, fx,
                                                                 hist, starts, (uint2 *)tmp);
         ZK_CHECK(hipGetLastError());
-        k_msm_sort_lo<false, 256><<<ngroups, 256, 0, stream>>>(items, p.W, p.Wd, p.nbins, fx, (const uint2 *)tmp,
-                                                               starts, p.lo_bits, p.B, ngroups, entries, offsets);
+        k_msm_sort_group<false, 256><<<ngroups, kGroupThreads, 0, stream>>>(
+            items, p.W, p.Wd, p.nbins, fx, (const uint2 *)tmp, starts, p.lo_bits, p.B, ngroups, entries, offsets);
     }
     ZK_CHECK(hipGetLastError());
     ZK_CHECK(prof_mark(prof, PH_ACCUMULATE, stream));
