@@ -154,15 +154,61 @@ __device__ __forceinline__ void swap_lane_reg(Fr &ra, Fr &rb, uint32_t lane) {
     }
 }
 
+// Register rounds keep values below 2r (r < 2^254, so 4r < 2^256): a sum is
+// brought below 2r by one conditional subtraction of 2r; the difference
+// that feeds a twiddle product is a - c + 2r (below 4r, no condition), and
+// the 29-bit product of it and a twiddle in the form f29_from32 gives (below
+// 32r) is below 4r * 32r / 2^261 + r < 2r, so its final reduction is left
+// out.  wave_dft_regs reduces every value below r once, after the rounds.
+__device__ __forceinline__ uint32_t fr_2r_limb(int i) {
+    return (FrParams::M[i] << 1) | (i ? FrParams::M[i - 1] >> 31 : 0u);
+}
+// a + b, then - 2r unless that borrows (a, b < 2r -> < 2r)
+__device__ __forceinline__ Fr fr_add_lazy(const Fr &a, const Fr &b) {
+    Fr t, u;
+    uint32_t c = 0, br = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) t.v[i] = add_cc(a.v[i], b.v[i], c);
+#pragma unroll
+    for (int i = 0; i < 8; i++) u.v[i] = sub_bb(t.v[i], fr_2r_limb(i), br);
+#pragma unroll
+    for (int i = 0; i < 8; i++) t.v[i] = br ? t.v[i] : u.v[i];
+    return t;
+}
+// a - b, plus 2r when that borrows (a, b < 2r -> < 2r)
+__device__ __forceinline__ Fr fr_sub_lazy(const Fr &a, const Fr &b) {
+    Fr t;
+    uint32_t br = 0, c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) t.v[i] = sub_bb(a.v[i], b.v[i], br);
+    const uint32_t mask = 0u - br;
+#pragma unroll
+    for (int i = 0; i < 8; i++) t.v[i] = add_cc(t.v[i], fr_2r_limb(i) & mask, c);
+    return t;
+}
+// a + 2r - b (a, b < 2r -> in (0, 4r)), a product operand only
+__device__ __forceinline__ Fr fr_sub_2r(const Fr &a, const Fr &b) {
+    Fr t;
+    uint32_t c = 0, br = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) t.v[i] = add_cc(a.v[i], fr_2r_limb(i), c);
+#pragma unroll
+    for (int i = 0; i < 8; i++) t.v[i] = sub_bb(t.v[i], b.v[i], br);
+    return t;
+}
+
 // w: a twiddle in the 29-bit form f29_from32 gives (register rounds), unused
-// without MUL
+// without MUL.  a, c < 2r in and out.
 template <bool MUL, class W>
 __device__ __forceinline__ void butterfly(Fr &a, Fr &c, const W &w) {
-    const Fr s = a + c;
-    Fr d = a - c;
-    if constexpr (MUL) d = fr_mul_w29(d, w);
+    const Fr s = fr_add_lazy(a, c);
+    if constexpr (MUL) {
+        const F29 t = f29_mul<Fr29Mod>(f29_repack(fr_sub_2r(a, c).v), w);
+        f29_unpack(t, c.v);
+    } else {
+        c = fr_sub_lazy(a, c);
+    }
     a = s;
-    c = d;
 }
 
 // Rounds RND..DEG-1 (RND >= 2), one template instance per round so that
@@ -239,10 +285,11 @@ __device__ __forceinline__ void wave_dft_regs(Fr *s, const F29 *tw, uint32_t lan
         }
     }
     dft_rounds<DEG, 2>(r0, r1, r2, r3, tw, lane);
-    s[slot_elem<DEG, true>(lane, 0)] = r0;
-    s[slot_elem<DEG, true>(lane, 1)] = r1;
-    s[slot_elem<DEG, true>(lane, 2)] = r2;
-    s[slot_elem<DEG, true>(lane, 3)] = r3;
+    // the rounds leave values below 2r: canonical again for the LDS
+    s[slot_elem<DEG, true>(lane, 0)] = fe_reduce_once<FrParams>(r0);
+    s[slot_elem<DEG, true>(lane, 1)] = fe_reduce_once<FrParams>(r1);
+    s[slot_elem<DEG, true>(lane, 2)] = fe_reduce_once<FrParams>(r2);
+    s[slot_elem<DEG, true>(lane, 3)] = fe_reduce_once<FrParams>(r3);
 }
 
 // Register rounds for 1024-element tiles of degree 2..8 (the radix-2 LDS
