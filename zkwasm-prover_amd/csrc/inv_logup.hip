@@ -37,6 +37,19 @@ __device__ __forceinline__ Fr load_plus(const Fr *x, size_t i, const Fr *beta, b
     return has_beta ? v + b : v;
 }
 
+// fe_inv with the binary GCD inlined, so that its limb arrays stay in
+// registers (fe_inv's out-of-line call keeps them in scratch memory); the
+// exact one-bit extended Euclid should the product check fail, as fe_inv.
+// 0 -> 0.
+__device__ __forceinline__ Fr fr_inv_inline(const Fr &a) {
+    if (fe_is_zero<FrParams>(a)) return fe_zero<FrParams>();
+    uint32_t v[8];
+    const bool ok = zkinv::bingcd(v, a.v, FrParams::M, FrParams::INV);
+    const Fr r = fe_from_raw<FrParams>(v) * fe_from_raw<FrParams>(FrParams::INVC);
+    if (ok && fe_eq<FrParams>(a * r, fe_one<FrParams>())) return r;
+    return fe_inv_raw<FrParams>(a) * fe_from_raw<FrParams>(FrParams::R3);
+}
+
 // tot[t] = prod_k x'[t + k*T] (zeros skipped)
 __global__ __launch_bounds__(256) void k_binv_totals(const Fr *__restrict__ x, const Fr *__restrict__ beta, Fr *__restrict__ tot,
                               uint32_t n, uint32_t T) {
@@ -108,7 +121,7 @@ __global__ __launch_bounds__(256) void k_binv_top(const Fr *__restrict__ x, cons
         for (uint32_t i = threadIdx.x; i < w; i += blockDim.x) tree[w + i] = tree[2 * (w + i)] * tree[2 * (w + i) + 1];
         __syncthreads();
     }
-    if (threadIdx.x == 0) tree[1] = fe_inv<FrParams>(tree[1]);
+    if (threadIdx.x == 0) tree[1] = fr_inv_inline(tree[1]);
     __syncthreads();
     for (uint32_t w = 1; w < P; w <<= 1) {
         for (uint32_t i = threadIdx.x; i < w; i += blockDim.x) {
@@ -198,7 +211,7 @@ __global__ __launch_bounds__(64) void k_binv_tiles(Fr *pre, uint32_t T) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= T) return;
     Fr *p = &pre[(size_t)b * kTile];
-    fe_store<FrParams>(p, fe_inv<FrParams>(fe_load<FrParams>(p)));
+    fe_store<FrParams>(p, fr_inv_inline(fe_load<FrParams>(p)));
 }
 
 template <int MODE>
