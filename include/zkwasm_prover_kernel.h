@@ -421,8 +421,10 @@ hipError_t zk_graph_destroy(void *exec);
 
 /* MSM phase timing for the bench: HIP events on the launch stream around each
  * phase (count, scan, scatter, accumulate, fixup, reduce, finalize); reading
- * returns the number of profiled calls.  Profiling waits for each MSM and
- * shares one set of events: enable it only while one thread runs MSMs. */
+ * waits for the profiled calls' events, sums them and returns the number of
+ * profiled calls (-1 on a HIP error).  A profiled call does not wait for its
+ * events (they go to a ring of 64 sets, drained when read or when full); the
+ * state is global: enable it only while one thread runs MSMs. */
 void zk_msm_profile_enable(int on);
 int zk_msm_profile_read(double *times, int ntimes, int reset);
 int zk_msm_default_window(int n);

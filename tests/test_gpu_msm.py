@@ -305,3 +305,31 @@ def test_collect_of_unfinished_slice_is_an_error(gpu, zo, monkeypatch):
     torch.cuda.synchronize()
     monkeypatch.delenv("ZK_MSM_TAIL")
     assert (slice_buf[:96].cpu().numpy() == 0xff).all()
+
+
+def test_msm_phase_profile_ring(gpu, zo):
+    """70 profiled MSMs (more than the profiler's 64 event sets, so the ring
+    is drained once while calls go on): reading returns all 70 calls with
+    positive phase times, and profiling leaves the results unchanged"""
+    from zkprover._lib import lib
+    L = lib()
+    L.zk_msm_profile_enable.argtypes = [ctypes.c_int]
+    L.zk_msm_profile_enable.restype = None
+    L.zk_msm_profile_read.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.zk_msm_profile_read.restype = ctypes.c_int
+    n = 4096
+    pts = zo.gen_bases_random(n, 61)
+    sc = zo.rand_fr(n, 62)
+    plain = zo.g1_to_affine(_zk_msm(pts, sc))
+    times = np.zeros(8, dtype=np.float64)
+    L.zk_msm_profile_read(times.ctypes.data, 8, 1)
+    L.zk_msm_profile_enable(1)
+    try:
+        got = [zo.g1_to_affine(_zk_msm(pts, sc)) for _ in range(70)]
+    finally:
+        L.zk_msm_profile_enable(0)
+    calls = L.zk_msm_profile_read(times.ctypes.data, 8, 1)
+    assert calls == 70
+    assert (times[:7] > 0).all()
+    assert all((g == plain).all() for g in got)
+    assert L.zk_msm_profile_read(times.ctypes.data, 8, 1) == 0

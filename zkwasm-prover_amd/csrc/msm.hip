@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -1189,32 +1190,62 @@ __global__ __launch_bounds__(256) void k_msm_table(const G1Aff *__restrict__ pts
 // launch stream after each phase, waits for the last one and accumulates
 // the per-phase durations (ms) and call counts.
 enum MsmPhase { PH_COUNT = 0, PH_SCAN, PH_SCATTER, PH_ACCUMULATE, PH_FIXUP, PH_REDUCE, PH_HEADER, PH_NUM };
+// Phase timing: each profiled call records its PH_NUM + 1 events into the
+// next of kProfSets event sets and does not wait for them; the sets are read
+// (synchronised and summed) by zk_msm_profile_read, or when a set is about to
+// be reused.  So a profiled MSM is timed as an unprofiled one runs, with no
+// host round trip added.
+constexpr int kProfSets = 64;
 struct PhaseProf {
     bool on = false;
     double ms[PH_NUM] = {0};
     uint64_t calls = 0;
-    hipEvent_t ev[PH_NUM + 1] = {};
+    hipEvent_t ev[kProfSets][PH_NUM + 1] = {};
+    int cur = 0;         // the set of the call in progress
+    int pending = 0;     // recorded sets not yet summed: the last `pending` before cur
     bool init = false;
 };
 PhaseProf g_prof;
 
+// sums set k into ms (waits for its last event)
+hipError_t prof_collect(int k) {
+    ZK_CHECK(hipEventSynchronize(g_prof.ev[k][PH_NUM]));
+    for (int p = 0; p < PH_NUM; p++) {
+        float ms = 0;
+        ZK_CHECK(hipEventElapsedTime(&ms, g_prof.ev[k][p], g_prof.ev[k][p + 1]));
+        g_prof.ms[p] += ms;
+    }
+    g_prof.calls++;
+    return hipSuccess;
+}
+// every pending set, oldest first
+hipError_t prof_drain() {
+    while (g_prof.pending > 0) {
+        const int k = ((g_prof.cur - g_prof.pending) % kProfSets + kProfSets) % kProfSets;
+        g_prof.pending--;
+        ZK_CHECK(prof_collect(k));
+    }
+    return hipSuccess;
+}
+
 hipError_t prof_mark(bool on, int k, hipStream_t stream) {
     if (!on) return hipSuccess;
     if (!g_prof.init) {
-        for (auto &e : g_prof.ev) ZK_CHECK(hipEventCreate(&e));
+        for (auto &set : g_prof.ev)
+            for (auto &e : set) ZK_CHECK(hipEventCreate(&e));
         g_prof.init = true;
     }
-    return hipEventRecord(g_prof.ev[k], stream);
+    if (k == 0 && g_prof.pending == kProfSets) {
+        // the ring is full: the oldest set (the one about to be reused) first
+        g_prof.pending--;
+        ZK_CHECK(prof_collect(g_prof.cur));
+    }
+    return hipEventRecord(g_prof.ev[g_prof.cur][k], stream);
 }
 hipError_t prof_finish(bool on) {
     if (!on) return hipSuccess;
-    ZK_CHECK(hipEventSynchronize(g_prof.ev[PH_NUM]));
-    for (int k = 0; k < PH_NUM; k++) {
-        float ms = 0;
-        ZK_CHECK(hipEventElapsedTime(&ms, g_prof.ev[k], g_prof.ev[k + 1]));
-        g_prof.ms[k] += ms;
-    }
-    g_prof.calls++;
+    g_prof.cur = (g_prof.cur + 1) % kProfSets;
+    g_prof.pending++;
     return hipSuccess;
 }
 
@@ -1494,6 +1525,83 @@ void host_tail_one(const uint8_t *blob, uint8_t *out96, bool split = false) {
     memcpy(out96, &res, 96);
 }
 
+// Pinned host staging for the collects' copies (device-to-host of the masked
+// sums, host-to-device of the results): buffers from hipHostMalloc, taken
+// from a free list per call and returned to it, kept for the process's life,
+// so concurrent collects each stage in their own and small copies go by
+// direct DMA instead of the runtime's pageable staging.  A null buffer (no
+// pinned memory to be had) makes the caller stage in pageable memory.
+struct Stage {
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+};
+std::mutex g_stage_mu;
+std::vector<Stage> g_stage_free;
+
+Stage stage_take(size_t need) {
+    {
+        std::lock_guard<std::mutex> g(g_stage_mu);
+        for (size_t i = 0; i < g_stage_free.size(); i++)
+            if (g_stage_free[i].cap >= need) {
+                const Stage st = g_stage_free[i];
+                g_stage_free.erase(g_stage_free.begin() + (long)i);
+                return st;
+            }
+    }
+    size_t cap = (size_t)1 << 16;
+    while (cap < need) cap <<= 1;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, cap, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();  // not left for the next call to report
+        return Stage{};
+    }
+    return Stage{static_cast<uint8_t *>(p), cap};
+}
+// returns the buffer to the free list (freed if the list cannot grow)
+struct StageHold {
+    Stage st;
+    std::vector<uint8_t> fallback;
+    // the staging area: pinned when available, else pageable
+    uint8_t *take(size_t need) {
+        st = stage_take(need);
+        if (st.p) return st.p;
+        fallback.assign(need, 0);
+        return fallback.data();
+    }
+    ~StageHold() {
+        if (!st.p) return;
+        try {
+            std::lock_guard<std::mutex> g(g_stage_mu);
+            g_stage_free.push_back(st);
+        } catch (...) {
+            (void)hipHostFree(st.p);
+        }
+    }
+};
+
+// the host tails of `batch` slice images at img + m * stride into out + m * 96,
+// over up to 16 threads (a lone MSM splits its own tail in two)
+void host_tails(const uint8_t *img, size_t stride, uint32_t batch, uint8_t *out) {
+    unsigned hw = std::thread::hardware_concurrency();
+    unsigned nt = batch < 2 ? 1 : (hw ? (hw < batch ? hw : batch) : 1);
+    if (nt > 16) nt = 16;
+    auto run = [&](unsigned t) {
+        for (uint32_t m = t; m < batch; m += nt) host_tail_one(img + (size_t)m * stride, out + (size_t)m * 96, batch == 1);
+    };
+    if (nt <= 1) {
+        run(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    unsigned started = 0;
+    try {
+        for (unsigned t = 0; t < nt; t++, started++) th.emplace_back(run, t);
+    } catch (...) {
+    }
+    for (auto &x : th) x.join();
+    for (unsigned t = started; t < nt; t++) run(t);  // shares whose thread could not be started
+}
+
 // slice_bytes (optional): every slice holds at least this many bytes, so
 // one copy of min(slice_bytes, kSliceReserve) per slice brings its header and
 // masked sums together (a valid header's sums always fit: msm() placed them
@@ -1501,22 +1609,21 @@ void host_tail_one(const uint8_t *blob, uint8_t *out96, bool split = false) {
 hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stream, size_t slice_bytes = 0) {
     if (slice_bytes >= kSliceQ) {
         const size_t bytes = slice_bytes < kSliceReserve ? slice_bytes : kSliceReserve;
-        std::vector<std::vector<uint8_t>> blob(batch);
-        for (uint32_t m = 0; m < batch; m++) {
-            blob[m].assign(bytes, 0);
-            ZK_CHECK(hipMemcpyAsync(blob[m].data(), slices[m], bytes, hipMemcpyDeviceToHost, stream));
-        }
+        StageHold hold;
+        uint8_t *img = hold.take((size_t)batch * (bytes + 96));  // slice images, then the results
+        uint8_t *out = img + (size_t)batch * bytes;
+        for (uint32_t m = 0; m < batch; m++)
+            ZK_CHECK(hipMemcpyAsync(img + (size_t)m * bytes, slices[m], bytes, hipMemcpyDeviceToHost, stream));
         ZK_CHECK(hipStreamSynchronize(stream));
-        std::vector<uint8_t> out((size_t)batch * 96);
         for (uint32_t m = 0; m < batch; m++) {
-            const SliceHeader *h = reinterpret_cast<const SliceHeader *>(blob[m].data() + 64);
+            const SliceHeader *h = reinterpret_cast<const SliceHeader *>(img + (size_t)m * bytes + 64);
             // no result (cleared or stale header), or sums beyond what was read
             if (!header_ok(h) || slice_sums_off(h->W, h->nbits) > bytes) return hipErrorInvalidValue;
         }
-        for (uint32_t m = 0; m < batch; m++) host_tail_one(blob[m].data(), &out[(size_t)m * 96], batch == 1);
+        host_tails(img, bytes, batch, out);
         for (uint32_t m = 0; m < batch; m++)
-            ZK_CHECK(hipMemcpyAsync(slices[m], &out[(size_t)m * 96], 96, hipMemcpyHostToDevice, stream));
-        return hipStreamSynchronize(stream);  // the host buffers must outlive the copies
+            ZK_CHECK(hipMemcpyAsync(slices[m], out + (size_t)m * 96, 96, hipMemcpyHostToDevice, stream));
+        return hipStreamSynchronize(stream);  // the staging must outlive the copies
     }
     // the largest slice blob: header + Q for W <= 128, nbits <= 16 is bounded
     // by slice_sums_off(W, nbits); read the header first to size each copy
@@ -1528,36 +1635,22 @@ hipError_t host_collect(uint8_t *const *slices, uint32_t batch, hipStream_t stre
     // an error, not a commitment
     for (uint32_t m = 0; m < batch; m++)
         if (!header_ok(&hdr[m])) return hipErrorInvalidValue;
-    std::vector<std::vector<uint8_t>> blob(batch);
+    size_t stride = 0;  // the largest slice image: each is staged at m * stride
     for (uint32_t m = 0; m < batch; m++) {
         const size_t bytes = slice_sums_off(hdr[m].W, hdr[m].nbits);
-        blob[m].assign(bytes, 0);
-        ZK_CHECK(hipMemcpyAsync(blob[m].data(), slices[m], bytes, hipMemcpyDeviceToHost, stream));
+        if (bytes > stride) stride = bytes;
     }
-    ZK_CHECK(hipStreamSynchronize(stream));
-    std::vector<uint8_t> out((size_t)batch * 96);
-    unsigned hw = std::thread::hardware_concurrency();
-    unsigned nt = batch < 2 ? 1 : (hw ? (hw < batch ? hw : batch) : 1);
-    if (nt > 16) nt = 16;
-    if (nt <= 1) {
-        for (uint32_t m = 0; m < batch; m++) host_tail_one(blob[m].data(), &out[(size_t)m * 96], batch == 1);
-    } else {
-        std::vector<std::thread> th;
-        unsigned started = 0;
-        try {
-            for (unsigned t = 0; t < nt; t++, started++)
-                th.emplace_back([&, t]() {
-                    for (uint32_t m = t; m < batch; m += nt) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
-                });
-        } catch (...) {
-        }
-        for (auto &x : th) x.join();
-        for (unsigned t = started; t < nt; t++)  // shares whose thread could not be started
-            for (uint32_t m = t; m < batch; m += nt) host_tail_one(blob[m].data(), &out[(size_t)m * 96]);
-    }
+    StageHold hold;
+    uint8_t *img = hold.take((size_t)batch * (stride + 96));  // slice images, then the results
+    uint8_t *out = img + (size_t)batch * stride;
     for (uint32_t m = 0; m < batch; m++)
-        ZK_CHECK(hipMemcpyAsync(slices[m], &out[(size_t)m * 96], 96, hipMemcpyHostToDevice, stream));
-    return hipStreamSynchronize(stream);  // the host buffers must outlive the copies
+        ZK_CHECK(hipMemcpyAsync(img + (size_t)m * stride, slices[m], slice_sums_off(hdr[m].W, hdr[m].nbits),
+                                hipMemcpyDeviceToHost, stream));
+    ZK_CHECK(hipStreamSynchronize(stream));
+    host_tails(img, stride, batch, out);
+    for (uint32_t m = 0; m < batch; m++)
+        ZK_CHECK(hipMemcpyAsync(slices[m], out + (size_t)m * 96, 96, hipMemcpyHostToDevice, stream));
+    return hipStreamSynchronize(stream);  // the staging must outlive the copies
 }
 
 // host_collect for slices at base + m * stride (stride >= kSliceReserve, the
@@ -1568,36 +1661,19 @@ hipError_t host_collect_strided(uint8_t *base, size_t stride, uint32_t batch, hi
                                 uint8_t *jac_out) {
     if (stride < kSliceReserve) return hipErrorInvalidValue;
     const size_t bytes = (size_t)(batch - 1) * stride + kSliceReserve;
-    std::vector<uint8_t> img(bytes);
-    ZK_CHECK(hipMemcpyAsync(img.data(), base, bytes, hipMemcpyDeviceToHost, stream));
+    StageHold hold;
+    uint8_t *img = hold.take(bytes + (size_t)batch * 96);  // the region's image, then the results
+    uint8_t *out = img + bytes;
+    ZK_CHECK(hipMemcpyAsync(img, base, bytes, hipMemcpyDeviceToHost, stream));
     ZK_CHECK(hipStreamSynchronize(stream));
     // a slice msm() never finished (cleared or stale header) has no result
     for (uint32_t m = 0; m < batch; m++)
-        if (!header_ok(reinterpret_cast<const SliceHeader *>(&img[(size_t)m * stride + 64]))) return hipErrorInvalidValue;
-    std::vector<uint8_t> out((size_t)batch * 96);
-    unsigned hw = std::thread::hardware_concurrency();
-    unsigned nt = batch < 2 ? 1 : (hw ? (hw < batch ? hw : batch) : 1);
-    if (nt > 16) nt = 16;
-    auto run = [&](unsigned t) {
-        for (uint32_t m = t; m < batch; m += nt)
-            host_tail_one(&img[(size_t)m * stride], &out[(size_t)m * 96], batch == 1);
-    };
-    if (nt <= 1) {
-        run(0);
-    } else {
-        std::vector<std::thread> th;
-        unsigned started = 0;
-        try {
-            for (unsigned t = 0; t < nt; t++, started++) th.emplace_back(run, t);
-        } catch (...) {
-        }
-        for (auto &x : th) x.join();
-        for (unsigned t = started; t < nt; t++) run(t);  // shares whose thread could not be started
-    }
-    for (uint32_t m = 0; m < batch; m++) memcpy(&img[(size_t)m * stride], &out[(size_t)m * 96], 96);
-    ZK_CHECK(hipMemcpyAsync(base, img.data(), bytes, hipMemcpyHostToDevice, stream));
-    if (jac_out) memcpy(jac_out, out.data(), out.size());
-    return hipStreamSynchronize(stream);  // the host image must outlive the copy
+        if (!header_ok(reinterpret_cast<const SliceHeader *>(img + (size_t)m * stride + 64))) return hipErrorInvalidValue;
+    host_tails(img, stride, batch, out);
+    for (uint32_t m = 0; m < batch; m++) memcpy(img + (size_t)m * stride, out + (size_t)m * 96, 96);
+    ZK_CHECK(hipMemcpyAsync(base, img, bytes, hipMemcpyHostToDevice, stream));
+    if (jac_out) memcpy(jac_out, out, (size_t)batch * 96);
+    return hipStreamSynchronize(stream);  // the staging must outlive the copy
 }
 
 bool device_tail_requested() {
@@ -1788,6 +1864,7 @@ hipError_t batch_msm_collect(void *remain_indices, void *remain_acc, void *next_
 // accumulate, fixup, reduce, header); returns the number of profiled calls.
 void zk_msm_profile_enable(int on) { g_prof.on = on != 0; }
 int zk_msm_profile_read(double *times, int ntimes, int reset) {
+    if (prof_drain() != hipSuccess) return -1;
     for (int k = 0; k < ntimes && k < PH_NUM; k++) times[k] = g_prof.ms[k];
     int calls = (int)g_prof.calls;
     if (reset) {
