@@ -256,7 +256,11 @@ uint32_t default_window(uint32_t n) {
     uint32_t lg = log2_exact(n ? n : 1);
     const uint8_t tuned = lg < 32 ? g_window[lg].load(std::memory_order_relaxed) : 0;
     if (tuned) return tuned;
-    int c = lg >= 18 && lg <= 20 ? 15 : (int)lg - 5;
+    // measured per size (zk_msm, one box): 15 for 2^18..2^19 (13 / 14 leave a
+    // narrow top window whose few buckets take half its entries), 16 for
+    // 2^20 (2.06 against 2.11 ms at 15), 17 from 2^21 (3.22 against 3.66 ms
+    // at 16), else lg - 5
+    int c = lg >= 21 ? 17 : (lg == 20 ? 16 : (lg >= 18 ? 15 : (int)lg - 5));
     if (c < 6) c = 6;
     if (c > 17) c = 17;
     return (uint32_t)c;
