@@ -56,6 +56,13 @@ constexpr uint32_t kL1 = 8;          // buckets per first-level reduction segmen
 // cost of a slightly longer last wave (2^22 points: fixup 0.31 -> 0.11 ms,
 // accumulation +0.05 ms)
 constexpr uint32_t kChunk = 96;
+// A pass with few sorted entries would leave most of the device idle with
+// chunks of kChunk (2^12 points at c = 8: 131k entries, ~1.4k threads, each
+// a chain of 96 dependent additions); below kSmallEntries the chunk shrinks
+// toward kSmallThreads threads, down to kMinChunk entries.
+constexpr uint64_t kSmallEntries = (uint64_t)1 << 21;
+constexpr uint64_t kSmallThreads = (uint64_t)1 << 16;
+constexpr uint32_t kMinChunk = 8;
 constexpr uint32_t kWalk = 32;       // chunk partials one fixup thread sums; longer buckets go to a workgroup
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMagic = 0x4d534d32u;  // "MSM2": masked sums present in the slice
@@ -112,6 +119,7 @@ struct MsmPlan {
     bool fixed;      // fixed-base mode: points are window tables, one bucket set per item
     uint32_t hi_bits, lo_bits, nbins;
     uint64_t ntot, ntiles, nhist, NB, max_entries, chunks;
+    uint32_t chunk;  // sorted entries per accumulation thread
     bool packed;  // every item has at most kPackedMaxN scalars: one-word sort entries
     size_t size[A_COUNT];
     uint8_t *ptr[A_COUNT];
@@ -131,7 +139,8 @@ inline uint32_t fixed_hi_bits(uint32_t c) {
     return hi;
 }
 
-MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c, bool fixed = false) {
+// small: the chunk for few entries (see kSmallEntries), else kChunk
+MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c, bool fixed = false, bool small = true) {
     MsmPlan p;
     memset(&p, 0, sizeof(p));
     p.count = count;
@@ -164,7 +173,12 @@ MsmPlan make_plan(const uint32_t *ns, uint32_t count, uint32_t c, bool fixed = f
     p.NB = (uint64_t)count * p.W * p.B;
     p.nhist = (uint64_t)p.Wd * p.nbins * p.ntiles;
     p.max_entries = p.ntot * p.Wd;
-    p.chunks = (p.max_entries + kChunk - 1) / kChunk;
+    p.chunk = kChunk;
+    if (small && p.max_entries < kSmallEntries) {
+        const uint64_t ch = (p.max_entries + kSmallThreads - 1) / kSmallThreads;
+        p.chunk = ch < kMinChunk ? kMinChunk : (uint32_t)ch;
+    }
+    p.chunks = (p.max_entries + p.chunk - 1) / p.chunk;
     p.size[A_ITEMS] = (size_t)(count + 1) * sizeof(MsmItemDev);
     p.size[A_SLOTS] = (size_t)count * sizeof(uint8_t *);
     p.size[A_OFFSETS] = (size_t)(p.NB + 1) * 4;
@@ -722,14 +736,14 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
                                                         const MsmItemDev *__restrict__ items,
                                                         X29S *__restrict__ buckets, X29S *__restrict__ phead,
                                                         X29S *__restrict__ ptail, uint32_t *__restrict__ hb,
-                                                        uint32_t *__restrict__ tb, uint64_t nchunks) {
+                                                        uint32_t *__restrict__ tb, uint64_t nchunks, uint32_t chunk) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nchunks) return;
     const uint32_t total = off[NB];
-    const uint32_t s = (uint32_t)(t * kChunk);
+    const uint32_t s = (uint32_t)(t * chunk);
     uint32_t head_bucket = kNone, tail_bucket = kNone;
     if (s < total) {
-        const uint32_t e = s + kChunk < total ? s + kChunk : total;
+        const uint32_t e = s + chunk < total ? s + chunk : total;
         uint32_t b = find_bucket(off, NB, s);
         uint32_t bend = off[b + 1];
         bool first = true, before = off[b] < s;
@@ -801,7 +815,7 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
 
 // merges the partial sums of buckets that span several chunks.  A bucket b
 // whose run starts in chunk t (tail partial) ends in chunk t' = (off[b+1]-1) /
-// kChunk, and chunks t+1..t' all hold head partials of b.  Spans up to kWalk
+// chunk, and chunks t+1..t' all hold head partials of b.  Spans up to kWalk
 // chunks are summed by the thread; longer ones (few: their count is at most
 // nchunks / kWalk) are listed for k_msm_fixup_long, one workgroup each, so a
 // heavy bucket (e.g. the multiplicities of a lookup, all in a handful of
@@ -809,12 +823,12 @@ __global__ __launch_bounds__(256) void k_msm_accumulate(const uint32_t *__restri
 __global__ __launch_bounds__(256) void k_msm_fixup(X29S *__restrict__ buckets, const X29S *__restrict__ phead,
                                                    const X29S *__restrict__ ptail, const uint32_t *__restrict__ tb,
                                                    const uint32_t *__restrict__ off, uint32_t *__restrict__ longs,
-                                                   uint64_t nchunks) {
+                                                   uint64_t nchunks, uint32_t chunk) {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nchunks) return;
     uint32_t b = tb[t];
     if (b == kNone) return;
-    const uint64_t last = (uint64_t)(off[b + 1] - 1) / kChunk;
+    const uint64_t last = (uint64_t)(off[b + 1] - 1) / chunk;
     if (last - t > kWalk) {
         uint32_t i = atomicAdd(&longs[0], 1u);
         longs[1 + 3 * i] = b;
@@ -1303,11 +1317,15 @@ bool choose_plan(const uint32_t *ns, uint32_t count, size_t slice_bytes, uint32_
         for (uint32_t cc = c0 - 1; cc >= 2; cc--) order[norder++] = cc;
     }
     for (int k = 0; k < norder; k++) {
-        p = make_plan(ns, count, order[k], fixed);
-        // entry positions are u32 and the masked-sum grid's y is < 2^16
-        if (p.max_entries < 0x80000000ull && (uint64_t)count * p.W < 65536 &&
-            slice_bytes_needed(p.W, p.nbits) <= slice_bytes && place(p, regions, region_bytes, nreg))
-            return true;
+        // the small-pass chunk first, kChunk when its partials do not fit
+        for (int small = 1; small >= 0; small--) {
+            p = make_plan(ns, count, order[k], fixed, small != 0);
+            // entry positions are u32 and the masked-sum grid's y is < 2^16
+            if (p.max_entries < 0x80000000ull && (uint64_t)count * p.W < 65536 &&
+                slice_bytes_needed(p.W, p.nbits) <= slice_bytes && place(p, regions, region_bytes, nreg))
+                return true;
+            if (p.chunk == kChunk) break;  // the same plan again
+        }
     }
     return false;
 }
@@ -1441,10 +1459,10 @@ hipError_t msm_run(const MsmJob *jobs, uint32_t count, size_t slice_bytes, uint3
     ZK_CHECK(prof_mark(prof, PH_ACCUMULATE, stream));
     const uint32_t cblocks = (uint32_t)((p.chunks + 255) / 256);
     k_msm_accumulate<<<cblocks, 256, 0, stream>>>(entries, offsets, (uint32_t)p.NB, p.W * p.B, items, buckets,
-                                                  phead, ptail, hb, tb, p.chunks);
+                                                  phead, ptail, hb, tb, p.chunks, p.chunk);
     ZK_CHECK(hipGetLastError());
     ZK_CHECK(prof_mark(prof, PH_FIXUP, stream));
-    k_msm_fixup<<<cblocks, 256, 0, stream>>>(buckets, phead, ptail, tb, offsets, longs, p.chunks);
+    k_msm_fixup<<<cblocks, 256, 0, stream>>>(buckets, phead, ptail, tb, offsets, longs, p.chunks, p.chunk);
     ZK_CHECK(hipGetLastError());
     {
         const uint64_t max_long = p.chunks / kWalk + 1;
